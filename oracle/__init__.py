@@ -1,0 +1,361 @@
+"""CPU FP64 parity oracle for the Bulk_3D_FFT SCFT path -- TEST INFRASTRUCTURE ONLY.
+
+Only the test suite, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` may import this package, and only as the checker.  The product
+library never routes through it.
+
+``ORACLE`` wraps ``liboracle.so`` (plain C, built by ``oracle/Makefile``) with
+numpy-facing helpers.  Arrays follow the reference's Fortran layout: a field
+``(nx,ny,nz)`` is held here as a C-ordered numpy array of shape
+``(nz, ny, nx)`` (x fastest), a field set ``(nx,ny,nz,k)`` as ``(k, nz, ny, nx)``
+and a half spectrum ``complex(8)(nx/2+1,ny,nz)`` as ``(nz, ny, nx//2+1)``
+complex128.
+
+Parity pins (each function is checked by the tests named here):
+  * FFT, Laplace operator: numpy.fft (pocketfft) cross-check, brute-force DFT
+    sums at 4^3..6^3, delta / single-mode known answers (tests/test_oracle.py).
+  * Propagate step and chain: w=0 / constant-w closed forms, single-mode decay,
+    separable-field factorisation into 1-D schemes, translation equivariance,
+    the contour invariant dvol*sum q(s)q+(Ns-s) = Q, mass and swap symmetry,
+    and the independent numpy restatement in ``oracle/np_ref.py``.
+  * Flory-Huggins fields/energy: per-point closed forms.
+  * Mixing (simple/lambda/Anderson) and ludcmp/lubksb: numpy restatement,
+    LAPACK solve, the iter<=m known answer, and -- when it builds -- the
+    reference's own mod_iterate.f90 compiled by ``oracle/build_ref.sh``.
+  * MT19937: numpy's MT19937 driven from the same sgrnd state, and the
+    reference's mod_mt.f90 compiled into ``oracle/_ref`` when it builds.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class _Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+                ("ds", C.c_double)]
+
+
+class _MT(C.Structure):
+    _fields_ = [("mt", C.c_uint * 624), ("mti", C.c_int)]
+
+
+class _Chain(C.Structure):
+    _fields_ = [("nblocks", C.c_int), ("monomer_type", _ip), ("block_segments", _ip),
+                ("ensemble", C.c_int), ("rho", C.c_double), ("mu", C.c_double),
+                ("Qsum", C.c_double), ("energy", C.c_double)]
+
+
+class _FH(C.Structure):
+    _fields_ = [("ncomp", C.c_int), ("kappa", C.c_double), ("chi", _dp), ("volume", _dp)]
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load():
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = C.CDLL(_LIB_PATH)
+    G = C.POINTER(_Grid)
+    sig = {
+        "orc_mt_sgrnd": (None, [C.POINTER(_MT), C.c_int]),
+        "orc_mt_next32": (C.c_uint, [C.POINTER(_MT)]),
+        "orc_mt_grnd": (C.c_double, [C.POINTER(_MT)]),
+        "orc_real2fourier": (None, [G, _dp, _dp]),
+        "orc_fourier2real": (None, [G, _dp, _dp]),
+        "orc_calculate_laplace": (None, [G, _dp, _dp, _dp]),
+        "orc_volume_parameters": (None, [G, _dp, _dp]),
+        "orc_dft_line": (None, [C.c_int, _dp, _dp, C.c_int]),
+        "orc_calculate_expfield": (None, [G, C.c_int, _dp, _dp]),
+        "orc_propagate_step": (None, [G, _dp, _dp]),
+        "orc_propagate_chain": (C.c_int, [G, C.POINTER(_Chain), _dp, _dp, _dp, _dp, _dp]),
+        "orc_block_segments": (C.c_int, [C.c_double, C.c_double]),
+        "orc_mean_fields": (None, [G, C.POINTER(_FH), _dp, _dp]),
+        "orc_interaction_energy": (C.c_double, [G, C.POINTER(_FH), _dp]),
+        "orc_average_volume_fraction": (C.c_double, [G, C.POINTER(_FH), _dp]),
+        "orc_mixing_simple": (None, [_dp, _dp, C.c_long, C.c_int, C.c_double]),
+        "orc_mixing_lambda": (None, [_dp, _dp, C.c_long, C.c_int, C.c_double, C.c_int]),
+        "orc_mixing_anderson": (None, [_dp, _dp, C.c_long, C.c_int, C.c_double, C.c_int]),
+        "orc_ludcmp": (C.c_int, [_dp, C.c_int, _ip, _dp]),
+        "orc_lubksb": (None, [_dp, C.c_int, _ip, _dp]),
+        "orc_solve_lu": (None, [_dp, _dp, _dp, C.c_int]),
+        "orc_solve_cholesky": (None, [_dp, _dp, _dp, C.c_int]),
+        "orc_species_densities": (C.c_int, [G, C.POINTER(_Chain), C.c_int, _dp, _dp]),
+        "orc_total_energy": (C.c_double, [G, C.POINTER(_Chain), C.POINTER(_FH), C.c_int, _dp, _dp]),
+        "orc_scf_anderson": (C.c_int, [G, C.POINTER(_Chain), C.POINTER(_FH), C.c_int, _dp, _dp,
+                                       C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _pi(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+class Grid:
+    """Box geometry: grid (nx,ny,nz), side lengths (lx,ly,lz) and contour step ds."""
+
+    def __init__(self, nx, ny, nz, lx, ly, lz, ds):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.lx, self.ly, self.lz = float(lx), float(ly), float(lz)
+        self.ds = float(ds)
+        self._g = _Grid(self.nx, self.ny, self.nz, self.lx, self.ly, self.lz, self.ds)
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    @property
+    def kshape(self):
+        return (self.nz, self.ny, self.nx // 2 + 1)
+
+    @property
+    def ng(self):
+        return self.nx * self.ny * self.nz
+
+    def volume(self):
+        v, dv = C.c_double(), C.c_double()
+        lib().orc_volume_parameters(C.byref(self._g), C.byref(v), C.byref(dv))
+        return v.value, dv.value
+
+
+class MT:
+    """The reference's MT19937 stream (mod_mt.f90): sgrnd seeding, grnd in [0,1]."""
+
+    def __init__(self, seed):
+        self._s = _MT()
+        lib().orc_mt_sgrnd(C.byref(self._s), int(seed))
+
+    def next32(self, n=None):
+        if n is None:
+            return lib().orc_mt_next32(C.byref(self._s))
+        return np.array([lib().orc_mt_next32(C.byref(self._s)) for _ in range(n)], dtype=np.uint32)
+
+    def grnd(self, n=None):
+        if n is None:
+            return lib().orc_mt_grnd(C.byref(self._s))
+        return np.array([lib().orc_mt_grnd(C.byref(self._s)) for _ in range(n)])
+
+
+def dft_line(x, sign):
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.empty_like(x)
+    lib().orc_dft_line(len(x), x.view(np.float64).ctypes.data_as(_dp), out.view(np.float64).ctypes.data_as(_dp), int(sign))
+    return out
+
+
+def real2fourier(g: Grid, prop):
+    prop = np.ascontiguousarray(prop, dtype=np.float64).reshape(g.shape)
+    out = np.empty(g.kshape, dtype=np.complex128)
+    lib().orc_real2fourier(C.byref(g._g), _p(prop), out.view(np.float64).ctypes.data_as(_dp))
+    return out
+
+
+def fourier2real(g: Grid, spec):
+    spec = np.ascontiguousarray(spec, dtype=np.complex128).reshape(g.kshape)
+    out = np.empty(g.shape, dtype=np.float64)
+    lib().orc_fourier2real(C.byref(g._g), spec.view(np.float64).ctypes.data_as(_dp), _p(out))
+    return out
+
+
+def calculate_laplace(g: Grid):
+    """Returns (laplace, explaplace, nabla[3]) as complex arrays of shape kshape."""
+    lap = np.empty(g.kshape, dtype=np.complex128)
+    el = np.empty(g.kshape, dtype=np.complex128)
+    nab = np.empty((3,) + g.kshape, dtype=np.complex128)
+    lib().orc_calculate_laplace(C.byref(g._g), lap.view(np.float64).ctypes.data_as(_dp),
+                                el.view(np.float64).ctypes.data_as(_dp), nab.view(np.float64).ctypes.data_as(_dp))
+    return lap, el, nab
+
+
+def expfield(g: Grid, field):
+    field = np.ascontiguousarray(field, dtype=np.float64)
+    nt = field.size // g.ng
+    out = np.empty_like(field)
+    lib().orc_calculate_expfield(C.byref(g._g), nt, _p(field), _p(out))
+    return out
+
+
+def propagate_step(g: Grid, prop, expfield_type):
+    q = np.array(prop, dtype=np.float64, order="C").reshape(g.shape)
+    e = np.ascontiguousarray(expfield_type, dtype=np.float64).reshape(g.shape)
+    lib().orc_propagate_step(C.byref(g._g), _p(q), _p(e))
+    return q
+
+
+def block_segments(block_length, ds):
+    return lib().orc_block_segments(float(block_length), float(ds))
+
+
+class ChainResult:
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def propagate_chain(g: Grid, block_types, block_segs, expfield, ensemble=1, rho=1.0, mu=0.0, keep_q=False):
+    """Propagate_Multiblock.  expfield: (ntypes, nz, ny, nx).  Returns a ChainResult with
+    distribution, density (nblocks, nz, ny, nx), Qsum, mu, rho, energy [, q, qdag]."""
+    bt = np.ascontiguousarray(block_types, dtype=np.int32)
+    bs = np.ascontiguousarray(block_segs, dtype=np.int32)
+    nb = len(bt)
+    ns = int(bs.sum())
+    e = np.ascontiguousarray(expfield, dtype=np.float64)
+    dist = np.empty((nb,) + g.shape)
+    den = np.empty((nb,) + g.shape)
+    q = np.empty((ns + 1,) + g.shape) if keep_q else None
+    qd = np.empty((ns + 1,) + g.shape) if keep_q else None
+    ch = _Chain(nb, _pi(bt), _pi(bs), int(ensemble), float(rho), float(mu), 0.0, 0.0)
+    rc = lib().orc_propagate_chain(C.byref(g._g), C.byref(ch), _p(e), _p(dist), _p(den),
+                                   _p(q) if keep_q else None, _p(qd) if keep_q else None)
+    if rc != 0:
+        raise ValueError("Invalid ensemble; Cannot calculate Density")
+    return ChainResult(distribution=dist, density=den, Qsum=ch.Qsum, mu=ch.mu, rho=ch.rho,
+                       energy=ch.energy, q=q, qdag=qd)
+
+
+class FH:
+    """Flory-Huggins parameters: kappa, chi (ncomp x ncomp symmetric), monomer volumes."""
+
+    def __init__(self, kappa, chi, volume):
+        self.chi = np.asfortranarray(np.array(chi, dtype=np.float64))
+        self.vol = np.ascontiguousarray(volume, dtype=np.float64)
+        self.kappa = float(kappa)
+        self._chi_f = np.ascontiguousarray(self.chi.T)  # column-major flat
+        self._fh = _FH(len(self.vol), self.kappa, _p(self._chi_f), _p(self.vol))
+
+
+def mean_fields(g: Grid, fh: FH, density):
+    d = np.ascontiguousarray(density, dtype=np.float64)
+    out = np.empty_like(d)
+    lib().orc_mean_fields(C.byref(g._g), C.byref(fh._fh), _p(d), _p(out))
+    return out
+
+
+def interaction_energy(g: Grid, fh: FH, density):
+    d = np.ascontiguousarray(density, dtype=np.float64)
+    return lib().orc_interaction_energy(C.byref(g._g), C.byref(fh._fh), _p(d))
+
+
+def average_volume_fraction(g: Grid, fh: FH, density):
+    d = np.ascontiguousarray(density, dtype=np.float64)
+    return lib().orc_average_volume_fraction(C.byref(g._g), C.byref(fh._fh), _p(d))
+
+
+def _hist(a):
+    # (m+1, fieldsize) C-order == (fieldsize, 0:m) column-major
+    assert a.ndim == 2 and a.dtype == np.float64 and a.flags.c_contiguous
+    return a
+
+
+def mixing_simple(field, dfield, lam):
+    _hist(field); _hist(dfield)
+    lib().orc_mixing_simple(_p(field), _p(dfield), field.shape[1], field.shape[0] - 1, float(lam))
+
+
+def mixing_lambda(field, dfield, lammax, it):
+    _hist(field); _hist(dfield)
+    lib().orc_mixing_lambda(_p(field), _p(dfield), field.shape[1], field.shape[0] - 1, float(lammax), int(it))
+
+
+def mixing_anderson(field, dfield, lam, it):
+    """In-place Mixing_Anderson on history arrays of shape (m+1, fieldsize)."""
+    _hist(field); _hist(dfield)
+    lib().orc_mixing_anderson(_p(field), _p(dfield), field.shape[1], field.shape[0] - 1, float(lam), int(it))
+
+
+def ludcmp(a):
+    """a: (n,n) numpy matrix (row i, column j).  Returns (lu, indx 1-based, d, singular)."""
+    n = a.shape[0]
+    af = np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)  # column-major
+    indx = np.zeros(n, dtype=np.int32)
+    d = C.c_double()
+    sing = lib().orc_ludcmp(_p(af), n, _pi(indx), C.byref(d))
+    return af.T.copy(), indx, d.value, bool(sing)
+
+
+def solve_lu(a, b):
+    n = a.shape[0]
+    af = np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.empty(n)
+    lib().orc_solve_lu(_p(af), _p(bb), _p(x), n)
+    return x
+
+
+def solve_cholesky(a, b):
+    n = a.shape[0]
+    af = np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.empty(n)
+    lib().orc_solve_cholesky(_p(af), _p(bb), _p(x), n)
+    return x
+
+
+def species_densities(g: Grid, block_types, block_segs, field, ensemble=1, rho=1.0, mu=0.0):
+    bt = np.ascontiguousarray(block_types, dtype=np.int32)
+    bs = np.ascontiguousarray(block_segs, dtype=np.int32)
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    nt = f.shape[0]
+    den = np.empty_like(f)
+    ch = _Chain(len(bt), _pi(bt), _pi(bs), int(ensemble), float(rho), float(mu), 0.0, 0.0)
+    rc = lib().orc_species_densities(C.byref(g._g), C.byref(ch), nt, _p(f), _p(den))
+    if rc != 0:
+        raise ValueError("Invalid ensemble")
+    return den, ChainResult(Qsum=ch.Qsum, mu=ch.mu, rho=ch.rho, energy=ch.energy, chain=ch, bt=bt, bs=bs)
+
+
+def total_energy(g: Grid, chain_info, fh: FH, density, field):
+    d = np.ascontiguousarray(density, dtype=np.float64)
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    return lib().orc_total_energy(C.byref(g._g), C.byref(chain_info.chain), C.byref(fh._fh), d.shape[0], _p(d), _p(f))
+
+
+def scf_anderson(g: Grid, block_types, block_segs, fh: FH, field, m, lam=0.5, tol=1e-8, maxit=100,
+                 ensemble=1, rho=1.0, mu=0.0):
+    """Find_SCF_Saddle_Point with Anderson mixing.  Returns dict(field, density, iters, accuracy,
+    residuals, Qsum, mu, energy) where field is the mixed field the loop ends with."""
+    bt = np.ascontiguousarray(block_types, dtype=np.int32)
+    bs = np.ascontiguousarray(block_segs, dtype=np.int32)
+    f = np.array(field, dtype=np.float64, order="C")
+    den = np.empty_like(f)
+    res = np.zeros(maxit)
+    acc = C.c_double()
+    ch = _Chain(len(bt), _pi(bt), _pi(bs), int(ensemble), float(rho), float(mu), 0.0, 0.0)
+    it = lib().orc_scf_anderson(C.byref(g._g), C.byref(ch), C.byref(fh._fh), f.shape[0], _p(f), _p(den),
+                                int(m), float(lam), float(tol), int(maxit), C.byref(acc), _p(res))
+    n = min(it, maxit)
+    return dict(field=f, density=den, iters=it, accuracy=acc.value, residuals=res[:n],
+                Qsum=ch.Qsum, mu=ch.mu, rho=ch.rho, energy=ch.energy)

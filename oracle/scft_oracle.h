@@ -1,0 +1,115 @@
+/*
+ * CPU FP64 restatement of ADSCF's Bulk_3D_FFT pseudo-spectral SCFT path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity oracle: only the
+ * test suite, the smoke check and the CPU-baseline leg of the benchmark may
+ * load it, and only as the checker.  The product (libadscf_hip.so) never
+ * links, loads or calls it, and has no CPU fallback.
+ *
+ * Conventions (same as the reference Fortran):
+ *   - every field is real(8)(nx,ny,nz[,k]) in column-major order, x fastest;
+ *   - a half spectrum is complex(8)(nx/2+1,ny,nz), stored as interleaved
+ *     {re,im} doubles;
+ *   - monomer types and blocks are 1-based in the arguments that carry them.
+ *
+ * Every function cites the reference file:line it restates.
+ */
+#ifndef ADSCF_ORACLE_H
+#define ADSCF_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int nx, ny, nz;
+  double lx, ly, lz; /* sizex, sizey, sizez */
+  double ds;
+} orc_grid;
+
+/* ---- MT19937 as in General_Routines/mod_mt.f90 ---------------------- */
+typedef struct {
+  unsigned int mt[624];
+  int mti;
+} orc_mt;
+void orc_mt_sgrnd(orc_mt* st, int seed);          /* mod_mt.f90:71-87 */
+unsigned int orc_mt_next32(orc_mt* st);           /* tempered word, mod_mt.f90:109-140 */
+double orc_mt_grnd(orc_mt* st);                   /* mod_mt.f90:142-146: y/(2^32-1) */
+
+/* ---- Fourier module (Geometry_Modules/Bulk/mod_fourier.f90) ---------- */
+/* Real2Fourier: forward r2c, divided by nx*ny*nz (mod_fourier.f90:94-105) */
+void orc_real2fourier(const orc_grid* g, const double* prop, double* propfourier);
+/* Fourier2Real: unnormalised c2r (mod_fourier.f90:75-86) */
+void orc_fourier2real(const orc_grid* g, const double* propfourier, double* prop);
+/* Calculate_Laplace (mod_fourier.f90:165-198); any output may be NULL.
+ * laplace, explaplace: complex (nx/2+1,ny,nz); nabla: complex (nx/2+1,ny,nz,3) */
+void orc_calculate_laplace(const orc_grid* g, double* laplace, double* explaplace, double* nabla);
+/* Calculate_Volume_Parameters (mod_fourier.f90:152-157) */
+void orc_volume_parameters(const orc_grid* g, double* volume, double* dvol);
+/* Full complex DFT of one line (length n, stride 1), sign -1 forward / +1 backward, unnormalised. */
+void orc_dft_line(int n, const double* in, double* out, int sign);
+
+/* ---- Propagate module (General_Routines/mod_propagate.f90) ----------- */
+void orc_calculate_expfield(const orc_grid* g, int ntypes, const double* field, double* expfield); /* :22-32 */
+void orc_propagate_step(const orc_grid* g, double* prop, const double* expfield_type);            /* :40-68 */
+
+/* ---- Chain propagation (Polymer_Modules/mod_multiblock.f90:112-188;
+ *      identical algorithm in mod_diblock.f90:100-174, mod_triblock.f90:103-177) */
+typedef struct {
+  int nblocks;
+  const int* monomer_type;   /* 1-based types, length nblocks */
+  const int* block_segments; /* length nblocks */
+  int ensemble;              /* 1 canonical, 2 grand canonical */
+  double rho, mu;            /* inout as in the reference type */
+  double Qsum, energy;       /* outputs */
+} orc_chain;
+/* expfield: (nx,ny,nz,ntypes); distribution, density: (nx,ny,nz,nblocks).
+ * q_out / qdag_out, when non-NULL, receive all (Ns+1) slices of qq / qqdag.
+ * Returns 0, or -1 for an invalid ensemble (the reference stops there). */
+int orc_propagate_chain(const orc_grid* g, orc_chain* c, const double* expfield,
+                        double* distribution, double* density, double* q_out, double* qdag_out);
+/* block_segments = nint(block_length/ds) (mod_multiblock.f90:85, mod_diblock.f90:74) */
+int orc_block_segments(double block_length, double ds);
+
+/* ---- Flory-Huggins interactions (Interaction_Modules/mod_FloryHuggins.f90) */
+typedef struct {
+  int ncomp;
+  double kappa;
+  const double* chi;    /* ncomp x ncomp, column-major, symmetric, zero diagonal */
+  const double* volume; /* monomer volumes, length ncomp */
+} orc_fh;
+void orc_mean_fields(const orc_grid* g, const orc_fh* fh, const double* density, double* field); /* :104-129 */
+double orc_interaction_energy(const orc_grid* g, const orc_fh* fh, const double* density);      /* :74-96 */
+double orc_average_volume_fraction(const orc_grid* g, const orc_fh* fh, const double* density); /* :139-160 */
+
+/* ---- Iterate module (General_Routines/mod_iterate.f90) -------------- */
+/* field, dfield: (fieldsize, 0:m) column-major */
+void orc_mixing_simple(double* field, const double* dfield, long fieldsize, int m, double lambda);       /* :27-38 */
+void orc_mixing_lambda(double* field, double* dfield, long fieldsize, int m, double lammax, int iter);   /* :46-84 */
+void orc_mixing_anderson(double* field, double* dfield, long fieldsize, int m, double lam_mix, int iter);/* :92-150 */
+int orc_ludcmp(double* a, int n, int* indx, double* d);  /* :232-298, returns 1 if a row was all zero */
+void orc_lubksb(const double* a, int n, const int* indx, double* b);                                      /* :302-337 */
+void orc_solve_lu(const double* A, const double* B, double* X, int n);                                    /* :203-224 */
+void orc_solve_cholesky(const double* A, const double* B, double* X, int n);                              /* :160-195 */
+
+/* ---- SCF drivers ------------------------------------------------------ */
+/* Calculate_Densities (main_multiblock.f90:514-540): species densities (nx,ny,nz,ntypes) */
+int orc_species_densities(const orc_grid* g, orc_chain* c, int ntypes, const double* field,
+                          double* density);
+/* Calculate_Energy (main_multiblock.f90:548-557) with Interaction_Energy */
+double orc_total_energy(const orc_grid* g, const orc_chain* c, const orc_fh* fh, int ntypes,
+                        const double* density, const double* field);
+/* Find_SCF_Saddle_Point (main_multiblock.f90:648-700) with Mix_Fields (708-765, SAVE history)
+ * and Check_Accuracy = max|dfield| (773-783).  Anderson mixing, history m, step lam.
+ * field: inout (nx,ny,nz,ntypes); density: out.  Stops after maxit iterations or when
+ * .not.(accuracy > tol).  Returns the number of iterations performed (the reference's
+ * loop index at exit; maxit+1 if not converged, as with a Fortran DO loop), writes the last
+ * accuracy.  The caller's field holds the mixed field for the next iteration; density
+ * holds the densities of the last evaluated field (main_multiblock.f90:696). */
+int orc_scf_anderson(const orc_grid* g, orc_chain* c, const orc_fh* fh, int ntypes, double* field,
+                     double* density, int m, double lam, double tol, int maxit, double* accuracy,
+                     double* residual_history);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
