@@ -1,0 +1,450 @@
+"""adscf_amd: MI355X (gfx950) pseudo-spectral SCFT propagator path of ADSCF's Bulk_3D_FFT.
+
+Python host binding of ``libadscf_hip.so`` (C-ABI declared in ``include/adscf.h``).
+Every call runs on the GPU; there is no CPU fallback: importing works without a
+GPU, but creating a :class:`Context` raises when the library or the device is
+missing.
+
+Arrays use the reference's Fortran layout expressed as C-ordered numpy arrays:
+a field ``real(8)(nx,ny,nz)`` is ``(nz, ny, nx)``, a field set
+``(nx,ny,nz,k)`` is ``(k, nz, ny, nx)``, a half spectrum
+``complex(8)(nx/2+1,ny,nz)`` is ``(nz, ny, nx//2+1)`` complex128, and an
+Anderson history ``(fieldsize, 0:m)`` is ``(m+1, fieldsize)``.
+
+The reference's procedure names are available as thin aliases
+(:func:`Calculate_Expfield`, :func:`Propagate_Step`, :func:`Propagate_Diblock`,
+...), so parity tests read like calls into the reference modules
+(Bulk_3D_FFT/General_Routines/mod_propagate.f90, Polymer_Modules/mod_diblock.f90,
+Interaction_Modules/mod_FloryHuggins.f90, General_Routines/mod_iterate.f90).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field as dc_field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libadscf_hip.so")
+
+FP64 = 0
+FP32 = 1
+
+OK = 0
+ERR_ARG = -1
+ERR_HIP = -2
+ERR_ENSEMBLE = -3
+ERR_STATE = -4
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_ctxp = C.c_void_p
+
+# name -> (restype, argtypes); the single source of truth for the exported surface
+SIGNATURES = {
+    "adscf_init": (C.c_int, [C.POINTER(_ctxp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                             C.c_double, C.c_double, C.c_int]),
+    "adscf_finalize": (C.c_int, [_ctxp]),
+    "adscf_set_box": (C.c_int, [_ctxp, C.c_double, C.c_double, C.c_double]),
+    "adscf_set_ds": (C.c_int, [_ctxp, C.c_double]),
+    "adscf_set_precision": (C.c_int, [_ctxp, C.c_int]),
+    "adscf_volume": (C.c_int, [_ctxp, _dp, _dp]),
+    "adscf_last_error": (C.c_char_p, [_ctxp]),
+    "adscf_real2fourier": (C.c_int, [_ctxp, _dp, _dp]),
+    "adscf_fourier2real": (C.c_int, [_ctxp, _dp, _dp]),
+    "adscf_laplace": (C.c_int, [_ctxp, _dp, _dp, _dp]),
+    "adscf_expfield": (C.c_int, [_ctxp, _dp, _dp]),
+    "adscf_propagate_step": (C.c_int, [_ctxp, _dp, _dp]),
+    "adscf_propagate_chain": (C.c_int, [_ctxp, C.c_int, _ip, _ip, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "adscf_set_interactions": (C.c_int, [_ctxp, C.c_double, _dp, _dp]),
+    "adscf_mean_fields": (C.c_int, [_ctxp, _dp, _dp]),
+    "adscf_interaction_energy": (C.c_int, [_ctxp, _dp, _dp]),
+    "adscf_average_volume_fraction": (C.c_int, [_ctxp, _dp, _dp]),
+    "adscf_mixing_simple": (C.c_int, [_ctxp, _dp, _dp, C.c_long, C.c_int, C.c_double]),
+    "adscf_mixing_lambda": (C.c_int, [_ctxp, _dp, _dp, C.c_long, C.c_int, C.c_double, C.c_int]),
+    "adscf_mixing_anderson": (C.c_int, [_ctxp, _dp, _dp, C.c_long, C.c_int, C.c_double, C.c_int]),
+    "adscf_ludcmp": (C.c_int, [_dp, C.c_int, _ip, _dp]),
+    "adscf_lubksb": (C.c_int, [_dp, C.c_int, _ip, _dp]),
+    "adscf_solve_lu": (C.c_int, [_dp, _dp, _dp, C.c_int]),
+    "adscf_solve_cholesky": (C.c_int, [_dp, _dp, _dp, C.c_int]),
+    "adscf_scf_setup": (C.c_int, [_ctxp, C.c_int, _ip, _ip, C.c_int, C.c_double, C.c_double, C.c_int,
+                                  C.c_double]),
+    "adscf_scf_set_field": (C.c_int, [_ctxp, _dp]),
+    "adscf_scf_iterate": (C.c_int, [_ctxp, C.c_int, C.c_double, _ip, _dp, _ip]),
+    "adscf_scf_evaluate": (C.c_int, [_ctxp]),
+    "adscf_scf_get": (C.c_int, [_ctxp, _dp, _dp]),
+    "adscf_scf_energy": (C.c_int, [_ctxp, _dp, _dp, _dp, _dp, _dp]),
+    "adscf_scf_timing": (C.c_int, [_ctxp, _dp, _dp]),
+}
+
+
+class AdscfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"adscf error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libadscf_hip.so.  Raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m adscf_amd.build` "
+                           "(the SCFT path runs only on the GPU)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous, "expected C-contiguous float64"
+    return a.ctypes.data_as(_dp)
+
+
+def _pc(a):
+    assert a.dtype == np.complex128 and a.flags.c_contiguous
+    return a.view(np.float64).ctypes.data_as(_dp)
+
+
+def _pi(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+@dataclass
+class ChainResult:
+    """Outputs of Propagate_{Diblock,Triblock,Multiblock} (mod_multiblock.f90:19-33 components)."""
+    distribution: np.ndarray
+    density: np.ndarray
+    Qsum: float
+    mu: float
+    rho: float
+    energy: float
+
+
+@dataclass
+class ScfStatus:
+    iters: int
+    accuracy: float
+    converged: bool
+
+
+class Context:
+    """A grid, box and contour step on one GPU (Initialize, sub_wrap.f90:8-40)."""
+
+    def __init__(self, nx, ny, nz, lx, ly, lz, ds, ntypes=2, precision=FP64):
+        self._lib = load_library()
+        self.nx, self.ny, self.nz, self.ntypes = int(nx), int(ny), int(nz), int(ntypes)
+        h = _ctxp()
+        rc = self._lib.adscf_init(C.byref(h), self.nx, self.ny, self.nz, self.ntypes, float(lx), float(ly),
+                                  float(lz), float(ds), int(precision))
+        if rc != OK:
+            raise AdscfError(rc, "adscf_init failed (is a gfx950 GPU visible?)")
+        self._h = h
+        self.ds = float(ds)
+        self.box = (float(lx), float(ly), float(lz))
+        self.precision = int(precision)
+
+    # -- lifetime --------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.adscf_finalize(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != OK:
+            raise AdscfError(rc, self._lib.adscf_last_error(self._h).decode())
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    @property
+    def kshape(self):
+        return (self.nz, self.ny, self.nx // 2 + 1)
+
+    @property
+    def ng(self):
+        return self.nx * self.ny * self.nz
+
+    def set_box(self, lx, ly, lz):
+        self._chk(self._lib.adscf_set_box(self._h, float(lx), float(ly), float(lz)))
+        self.box = (float(lx), float(ly), float(lz))
+
+    def set_ds(self, ds):
+        self._chk(self._lib.adscf_set_ds(self._h, float(ds)))
+        self.ds = float(ds)
+
+    def set_precision(self, precision):
+        self._chk(self._lib.adscf_set_precision(self._h, int(precision)))
+        self.precision = int(precision)
+
+    def volume(self):
+        v, dv = C.c_double(), C.c_double()
+        self._chk(self._lib.adscf_volume(self._h, C.byref(v), C.byref(dv)))
+        return v.value, dv.value
+
+    # -- Fourier_fftw3 ----------------------------------------------------
+    def real2fourier(self, prop):
+        prop = _f64(prop, self.shape)
+        out = np.empty(self.kshape, dtype=np.complex128)
+        self._chk(self._lib.adscf_real2fourier(self._h, _p(prop), _pc(out)))
+        return out
+
+    def fourier2real(self, spec):
+        spec = np.ascontiguousarray(spec, dtype=np.complex128).reshape(self.kshape)
+        out = np.empty(self.shape)
+        self._chk(self._lib.adscf_fourier2real(self._h, _pc(spec), _p(out)))
+        return out
+
+    def laplace(self):
+        lap = np.empty(self.kshape, dtype=np.complex128)
+        el = np.empty(self.kshape, dtype=np.complex128)
+        nab = np.empty((3,) + self.kshape, dtype=np.complex128)
+        self._chk(self._lib.adscf_laplace(self._h, _pc(lap), _pc(el), _pc(nab)))
+        return lap, el, nab
+
+    # -- Propagate ------------------------------------------------------
+    def expfield(self, field):
+        f = _f64(field, (self.ntypes,) + self.shape)
+        out = np.empty_like(f)
+        self._chk(self._lib.adscf_expfield(self._h, _p(f), _p(out)))
+        return out
+
+    def propagate_step(self, prop, expfield_type):
+        q = np.array(prop, dtype=np.float64, order="C").reshape(self.shape)
+        e = _f64(expfield_type, self.shape)
+        self._chk(self._lib.adscf_propagate_step(self._h, _p(q), _p(e)))
+        return q
+
+    def propagate_chain(self, block_types, block_segments, expfield, ensemble=1, rho=1.0, mu=0.0,
+                        want_distribution=True, want_density=True):
+        bt = np.ascontiguousarray(block_types, dtype=np.int32)
+        bs = np.ascontiguousarray(block_segments, dtype=np.int32)
+        e = _f64(expfield, (self.ntypes,) + self.shape)
+        nb = len(bt)
+        dist = np.empty((nb,) + self.shape) if want_distribution else None
+        den = np.empty((nb,) + self.shape) if want_density else None
+        r, m, q, en = C.c_double(rho), C.c_double(mu), C.c_double(), C.c_double()
+        self._chk(self._lib.adscf_propagate_chain(self._h, nb, _pi(bt), _pi(bs), int(ensemble), C.byref(r),
+                                                  C.byref(m), _p(e), _p(dist), _p(den), C.byref(q),
+                                                  C.byref(en)))
+        return ChainResult(dist, den, q.value, m.value, r.value, en.value)
+
+    # -- Interactions_FloryHuggins ----------------------------------------
+    def set_interactions(self, kappa, chi, monomer_volume):
+        chi = np.asarray(chi, dtype=np.float64)
+        chi_f = np.ascontiguousarray(chi.T)  # column-major
+        vol = _f64(monomer_volume)
+        self._chk(self._lib.adscf_set_interactions(self._h, float(kappa), _p(chi_f), _p(vol)))
+
+    def mean_fields(self, density):
+        d = _f64(density, (self.ntypes,) + self.shape)
+        out = np.empty_like(d)
+        self._chk(self._lib.adscf_mean_fields(self._h, _p(d), _p(out)))
+        return out
+
+    def interaction_energy(self, density):
+        d = _f64(density, (self.ntypes,) + self.shape)
+        e = C.c_double()
+        self._chk(self._lib.adscf_interaction_energy(self._h, _p(d), C.byref(e)))
+        return e.value
+
+    def average_volume_fraction(self, density):
+        d = _f64(density, (self.ntypes,) + self.shape)
+        e = C.c_double()
+        self._chk(self._lib.adscf_average_volume_fraction(self._h, _p(d), C.byref(e)))
+        return e.value
+
+    # -- Iterate (in place on (m+1, fieldsize) history arrays) ------------
+    def mixing_simple(self, field, dfield, lam):
+        self._chk(self._lib.adscf_mixing_simple(self._h, _p(field), _p(dfield), field.shape[1],
+                                                field.shape[0] - 1, float(lam)))
+
+    def mixing_lambda(self, field, dfield, lammax, it):
+        self._chk(self._lib.adscf_mixing_lambda(self._h, _p(field), _p(dfield), field.shape[1],
+                                                field.shape[0] - 1, float(lammax), int(it)))
+
+    def mixing_anderson(self, field, dfield, lam, it):
+        self._chk(self._lib.adscf_mixing_anderson(self._h, _p(field), _p(dfield), field.shape[1],
+                                                  field.shape[0] - 1, float(lam), int(it)))
+
+    # -- device-resident SCF ------------------------------------------------
+    def scf_setup(self, block_types, block_segments, ensemble=1, rho=1.0, mu=0.0, mixing_dim=10, lam_mix=0.5):
+        bt = np.ascontiguousarray(block_types, dtype=np.int32)
+        bs = np.ascontiguousarray(block_segments, dtype=np.int32)
+        self._chk(self._lib.adscf_scf_setup(self._h, len(bt), _pi(bt), _pi(bs), int(ensemble), float(rho),
+                                            float(mu), int(mixing_dim), float(lam_mix)))
+
+    def scf_set_field(self, field):
+        f = _f64(field, (self.ntypes,) + self.shape)
+        self._chk(self._lib.adscf_scf_set_field(self._h, _p(f)))
+
+    def scf_iterate(self, niter, tol=0.0):
+        it, acc, conv = C.c_int(), C.c_double(), C.c_int()
+        self._chk(self._lib.adscf_scf_iterate(self._h, int(niter), float(tol), C.byref(it), C.byref(acc),
+                                              C.byref(conv)))
+        return ScfStatus(it.value, acc.value, bool(conv.value))
+
+    def scf_evaluate(self):
+        self._chk(self._lib.adscf_scf_evaluate(self._h))
+
+    def scf_get(self):
+        f = np.empty((self.ntypes,) + self.shape)
+        d = np.empty((self.ntypes,) + self.shape)
+        self._chk(self._lib.adscf_scf_get(self._h, _p(f), _p(d)))
+        return f, d
+
+    def scf_energy(self):
+        F, Fc, q, mu, rho = (C.c_double() for _ in range(5))
+        self._chk(self._lib.adscf_scf_energy(self._h, C.byref(F), C.byref(Fc), C.byref(q), C.byref(mu),
+                                             C.byref(rho)))
+        return dict(F=F.value, F_per_chain=Fc.value, Qsum=q.value, mu=mu.value, rho=rho.value)
+
+    def scf_timing(self):
+        a, b = C.c_double(), C.c_double()
+        self._chk(self._lib.adscf_scf_timing(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+
+# ---- host routines of module Iterate (no GPU needed) -----------------------
+
+def ludcmp(a):
+    """ludcmp (mod_iterate.f90:232-298) on a numpy (n,n) matrix; returns (lu, indx, d)."""
+    lib = load_library()
+    n = a.shape[0]
+    af = np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)
+    indx = np.zeros(n, dtype=np.int32)
+    d = C.c_double()
+    rc = lib.adscf_ludcmp(_p(af), n, _pi(indx), C.byref(d))
+    if rc != OK:
+        raise AdscfError(rc, "ludcmp")
+    return af.T.copy(), indx, d.value
+
+
+def solve_lu(a, b):
+    lib = load_library()
+    n = a.shape[0]
+    af = np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)
+    x = np.empty(n)
+    rc = lib.adscf_solve_lu(_p(af), _p(_f64(b)), _p(x), n)
+    if rc != OK:
+        raise AdscfError(rc, "solve_lu")
+    return x
+
+
+def solve_cholesky(a, b):
+    lib = load_library()
+    n = a.shape[0]
+    af = np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)
+    x = np.empty(n)
+    rc = lib.adscf_solve_cholesky(_p(af), _p(_f64(b)), _p(x), n)
+    if rc != OK:
+        raise AdscfError(rc, "solve_cholesky")
+    return x
+
+
+# ---- reference-named aliases -------------------------------------------------
+
+@dataclass
+class Copolymer:
+    """Mirror of diblock_/triblock_/multiblock_copolymer (mod_multiblock.f90:19-33)."""
+    label: int
+    monomer_type: list
+    block_length: list
+    ensemble: int
+    rho: float
+    mu: float
+    block_segments: list = dc_field(default_factory=list)
+    segments: int = 0
+    energy: float = 0.0
+    Qsum: float = 0.0
+    distribution: np.ndarray = None
+    density: np.ndarray = None
+
+
+def _nint(x):
+    # Fortran NINT: round half away from zero (mod_multiblock.f90:85)
+    return int(np.floor(abs(x) + 0.5)) * (1 if x >= 0 else -1)
+
+
+def Initialize_Multiblock(ctx: Context, label, monomer_type, block_length, ensemble, rho, mu):
+    """Initialize_Multiblock (mod_multiblock.f90:48-99); also stands for _Diblock / _Triblock."""
+    if ensemble not in (1, 2):
+        print("Invalid Ensemble")  # mod_multiblock.f90:75
+    segs = [_nint(b / ctx.ds) for b in block_length]
+    return Copolymer(label, list(monomer_type), list(block_length), ensemble, rho, mu, segs, sum(segs))
+
+
+Initialize_Diblock = Initialize_Multiblock
+Initialize_Triblock = Initialize_Multiblock
+
+
+def Calculate_Expfield(ctx: Context, field):
+    """Calculate_Expfield (mod_propagate.f90:22-32)."""
+    return ctx.expfield(field)
+
+
+def Propagate_Step(ctx: Context, prop, expfield_type):
+    """Propagate_Step (mod_propagate.f90:40-68); returns the new prop."""
+    return ctx.propagate_step(prop, expfield_type)
+
+
+def Propagate_Multiblock(ctx: Context, copolymer: Copolymer, expfield):
+    """Propagate_Multiblock (mod_multiblock.f90:112-188): fills Qsum, mu/rho, energy,
+    distribution and density of the copolymer.  An invalid ensemble raises AdscfError
+    (the reference prints and stops, :181-183)."""
+    r = ctx.propagate_chain(copolymer.monomer_type, copolymer.block_segments, expfield, copolymer.ensemble,
+                            copolymer.rho, copolymer.mu)
+    copolymer.Qsum, copolymer.mu, copolymer.rho, copolymer.energy = r.Qsum, r.mu, r.rho, r.energy
+    copolymer.distribution, copolymer.density = r.distribution, r.density
+    return copolymer
+
+
+Propagate_Diblock = Propagate_Multiblock
+Propagate_Triblock = Propagate_Multiblock
+
+
+def Calculate_Mean_Fields(ctx: Context, density):
+    """Calculate_Mean_Fields (mod_FloryHuggins.f90:104-129)."""
+    return ctx.mean_fields(density)
+
+
+def Interaction_Energy(ctx: Context, density):
+    """Interaction_Energy (mod_FloryHuggins.f90:74-96)."""
+    return ctx.interaction_energy(density)
+
+
+def Mixing_Anderson(ctx: Context, field, dfield, lam_mix, it):
+    """Mixing_Anderson (mod_iterate.f90:92-150), in place on (m+1, fieldsize) arrays."""
+    ctx.mixing_anderson(field, dfield, lam_mix, it)

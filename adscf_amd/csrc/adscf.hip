@@ -1,0 +1,1499 @@
+// libadscf_hip.so: host side of the MI355X SCFT propagator path.
+//
+// The context owns every device buffer and one HIP stream.  Entry points that
+// take host arrays (the reference-compatible ones) copy in, run on the GPU and
+// copy out; the device-resident SCF loop keeps fields, propagators and the
+// Anderson history in HBM and replays one captured HIP graph per iteration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/adscf.h"
+#include "kernels.hpp"
+#include "lu.hpp"
+#include "scf_kernels.hpp"
+
+using namespace adscf;
+
+namespace {
+
+constexpr int kRedBlocks = 512;  // blocks of the grid-stride reduction kernels
+
+struct HipError {
+  hipError_t e;
+  const char* what;
+  int line;
+};
+
+#define HIPCK(x)                                      \
+  do {                                                \
+    hipError_t e__ = (x);                             \
+    if (e__ != hipSuccess) throw HipError{e__, #x, __LINE__}; \
+  } while (0)
+
+struct ArgError {
+  std::string msg;
+};
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void ensure(size_t want) {
+    if (want <= n) return;
+    release();
+    HIPCK(hipMalloc((void**)&p, std::max<size_t>(want, 1) * sizeof(T)));
+    n = want;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+struct AxisTables {
+  int n = 1;
+  LinePlan plan{};
+  int L = 1;
+};
+
+// Per-precision engine: the propagator arrays and FFT tables in T.
+template <typename T>
+struct Engine {
+  DevBuf<C2<T>> tw[3];
+  DevBuf<T> kf[3];
+  DevBuf<T> E;       // nt x ng
+  DevBuf<C2<T>> tmp; // ng complex, in-place line passes
+  DevBuf<T> h1, h2;  // (ns+1) x ng each
+  size_t hist_slices = 0;
+  DevBuf<T> cq;      // compat single-field propagate
+  DevBuf<C2<T>> ch;  // compat half spectrum
+  DevBuf<T> cmixE;   // E output of the compat mixing kernel (unused values)
+};
+
+}  // namespace
+
+struct adscf_ctx {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int nx = 1, ny = 1, nz = 1, nt = 2;
+  long ng = 1;
+  double lx = 1, ly = 1, lz = 1, ds = 0.01, volume = 1, dvol = 1;
+  int prec = ADSCF_FP64;
+  std::string err;
+  AxisTables ax[3];
+  Engine<double> e64;
+  Engine<float> e32;
+  bool tables64 = false, tables32 = false;
+  // shared FP64 / scalar state
+  FHDesc fh{};
+  bool fh_set = false;
+  DevBuf<double> wring, dring, phi, dens_b, dist_b, io, io2;
+  DevBuf<double> qpart, rpart, spart, e2part, gpart, epart, resid;
+  DevBuf<int> e1, e2, d1, d2;
+  DevBuf<ScfState> st, cst;
+  // SCF configuration
+  bool scf_ready = false, field_set = false, phi_fresh = false;
+  ChainDesc cd{};
+  int m = 10;
+  double lam_mix = 0.5;
+  double rho = 1.0, mu = 0.0;
+  hipGraphExec_t graph = nullptr;
+  hipGraph_t graph_src = nullptr;
+  int graph_prec = -1;
+  std::vector<const void*> graph_key;  // buffers baked into the captured iteration
+  double last_ms_iter = 0, last_ms_sweep = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+void set_err(adscf_ctx* c, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+}
+
+template <typename F>
+int guarded(adscf_ctx* c, F&& f) {
+  if (!c) return ADSCF_ERR_ARG;
+  try {
+    return f();
+  } catch (const HipError& e) {
+    set_err(c, "HIP error %d (%s) in %s [adscf.hip:%d]", (int)e.e, hipGetErrorString(e.e), e.what, e.line);
+    return ADSCF_ERR_HIP;
+  } catch (const ArgError& e) {
+    c->err = e.msg;
+    return ADSCF_ERR_ARG;
+  } catch (const std::bad_alloc&) {
+    set_err(c, "host allocation failed");
+    return ADSCF_ERR_HIP;
+  }
+}
+
+int grid_for(long n) {
+  long b = (n + kThreads - 1) / kThreads;
+  return (int)std::max<long>(1, std::min<long>(b, kRedBlocks));
+}
+
+// Wave number of FFT bin k on an axis of n points and length len, folded as
+// Calculate_Laplace folds y and z (mod_fourier.f90:181-186); for x the bins
+// above n/2 mirror the Hermitian half, so the same q^2 applies.
+double wavenumber(int k, int n, double len) {
+  const double pi = 3.141592653589793;  // Global: pi (Main_Diblock_Melt/mod_global.f90:10)
+  const int kk = (k + 1 > n / 2 + 1) ? k - n : k;
+  return 2.0 * pi * (double)kk / len;
+}
+
+int choose_tile(int axis, int n, int nx, int ny, int nz) {
+  int L = std::max(1, 1024 / std::max(n, 1));
+  if (axis == 0) {
+    L = std::min(L, ny * nz);
+  } else {
+    L = std::max(L, 4);
+    L = std::min(L, 16);
+    L = std::min(L, nx);
+  }
+  return std::max(L, 1);
+}
+
+template <typename T>
+Engine<T>& eng(adscf_ctx* c);
+template <>
+Engine<double>& eng<double>(adscf_ctx* c) { return c->e64; }
+template <>
+Engine<float>& eng<float>(adscf_ctx* c) { return c->e32; }
+
+template <typename T>
+void upload_tables(adscf_ctx* c) {
+  Engine<T>& E = eng<T>(c);
+  const int dims[3] = {c->nx, c->ny, c->nz};
+  const double lens[3] = {c->lx, c->ly, c->lz};
+  for (int a = 0; a < 3; ++a) {
+    const int n = dims[a];
+    std::vector<C2<T>> tw(n);
+    std::vector<T> kf(n);
+    for (int t = 0; t < n; ++t) {
+      const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)n;
+      tw[t] = {(T)cosl(ang), (T)sinl(ang)};
+      const double q = wavenumber(t, n, lens[a]);
+      kf[t] = (T)(exp(-c->ds * q * q) / (double)n);
+    }
+    E.tw[a].ensure(n);
+    E.kf[a].ensure(n);
+    HIPCK(hipMemcpyAsync(E.tw[a].p, tw.data(), sizeof(C2<T>) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipMemcpyAsync(E.kf[a].p, kf.data(), sizeof(T) * n, hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCK(hipStreamSynchronize(c->stream));
+}
+
+void invalidate_tables(adscf_ctx* c) {
+  c->tables64 = c->tables32 = false;
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  if (c->graph_src) {
+    (void)hipGraphDestroy(c->graph_src);
+    c->graph_src = nullptr;
+  }
+}
+
+template <typename T>
+void ensure_tables(adscf_ctx* c) {
+  bool& ok = std::is_same<T, double>::value ? c->tables64 : c->tables32;
+  if (!ok) {
+    upload_tables<T>(c);
+    ok = true;
+  }
+}
+
+template <typename T>
+PassArgs<T> base_pass(adscf_ctx* c, int axis) {
+  Engine<T>& E = eng<T>(c);
+  PassArgs<T> p{};
+  p.nx = c->nx;
+  p.ny = c->ny;
+  p.nz = c->nz;
+  p.axis = axis;
+  const int n = c->ax[axis].n;
+  p.plan = c->ax[axis].plan;
+  p.L = c->ax[axis].L;
+  p.ldn = n + 1;
+  p.gx = (c->nx + p.L - 1) / p.L;
+  p.stride = axis == 0 ? 1 : (axis == 1 ? (long)c->nx : (long)c->nx * c->ny);
+  p.tw = E.tw[axis].p;
+  p.kf = E.kf[axis].p;
+  p.nops = 1;
+  return p;
+}
+
+template <typename T>
+void launch_pass(adscf_ctx* c, const PassArgs<T>& p) {
+  int items;
+  if (p.axis == 0) items = (int)(((long)c->ny * c->nz + p.L - 1) / p.L);
+  else if (p.axis == 1) items = c->nz * p.gx;
+  else items = c->ny * p.gx;
+  const size_t smem = 2 * (size_t)p.L * p.ldn * sizeof(C2<T>);
+  hipLaunchKernelGGL(k_line_pass<T>, dim3(items), dim3(kThreads), smem, c->stream, p);
+  HIPCK(hipGetLastError());
+}
+
+std::vector<int> active_axes(const adscf_ctx* c) {
+  std::vector<int> A;
+  const int d[3] = {c->nx, c->ny, c->nz};
+  for (int a = 0; a < 3; ++a)
+    if (d[a] > 1) A.push_back(a);
+  if (A.empty()) A.push_back(0);
+  return A;
+}
+
+// Block type of contour step s (1..ns) for the forward chain (c == 0) and
+// the complementary chain (blocks reversed, mod_multiblock.f90:145).
+int step_type(const ChainDesc& cd, int chain, int s) {
+  int acc = 0;
+  for (int bb = 0; bb < cd.nblocks; ++bb) {
+    const int b = chain == 0 ? bb : cd.nblocks - 1 - bb;
+    acc += cd.seg[b];
+    if (s <= acc) return cd.type[b];
+  }
+  return cd.type[chain == 0 ? cd.nblocks - 1 : 0];
+}
+
+// Build the line passes of a forward + complementary sweep.
+template <typename T>
+std::vector<PassArgs<T>> build_sweep(adscf_ctx* c, const ChainDesc& cd, const T* E, T* h1, T* h2, const int* dexp1,
+                                     const int* dexp2, const int* done) {
+  Engine<T>& En = eng<T>(c);
+  const long ng = c->ng;
+  const std::vector<int> A = active_axes(c);
+  const int ns = cd.ns;
+  auto Ep = [&](int chain, int s) { return E + (long)step_type(cd, chain, s) * ng; };
+  std::vector<PassArgs<T>> passes;
+  if (A.size() == 1) {
+    for (int s = 1; s <= ns; ++s) {
+      PassArgs<T> p = base_pass<T>(c, A[0]);
+      p.in_kind = s == 1 ? IN_ONES : IN_PLANAR;
+      p.q1_in = h1 + (long)(s - 1) * ng;
+      p.q2_in = h2 + (long)(s - 1) * ng;
+      p.pre_E = 1;
+      p.Ea1 = Ep(0, s);
+      p.Ea2 = Ep(1, s);
+      p.out_kind = OUT_PLANAR;
+      p.h1 = h1 + (long)s * ng;
+      p.h2 = h2 + (long)s * ng;
+      p.dexp1 = dexp1;
+      p.dexp2 = dexp2;
+      p.sidx = s;
+      p.done = done;
+      passes.push_back(p);
+    }
+    return passes;
+  }
+  for (int s = 1; s <= ns; ++s) {
+    std::vector<int> order = A;
+    if ((s - 1) % 2 == 1) std::reverse(order.begin(), order.end());
+    for (size_t idx = 0; idx < order.size(); ++idx) {
+      const int a = order[idx];
+      if (idx == 0 && s > 1) {
+        PassArgs<T>& p = passes.back();  // last op of step s-1, same axis
+        p.nops = 2;
+        p.out_kind = OUT_TMP;
+        p.tmp_out = En.tmp.p;
+        p.h1 = h1 + (long)(s - 1) * ng;
+        p.h2 = h2 + (long)(s - 1) * ng;
+        p.Ea1 = Ep(0, s - 1);
+        p.Ea2 = Ep(1, s - 1);
+        p.Eb1 = Ep(0, s);
+        p.Eb2 = Ep(1, s);
+        p.dexp1 = dexp1;
+        p.dexp2 = dexp2;
+        p.sidx = s - 1;
+        continue;
+      }
+      PassArgs<T> p = base_pass<T>(c, a);
+      p.done = done;
+      if (idx == 0) {  // s == 1
+        p.in_kind = IN_ONES;
+        p.pre_E = 1;
+        p.Ea1 = Ep(0, 1);
+        p.Ea2 = Ep(1, 1);
+        p.h2 = h2;  // marks the second chain present for IN_ONES
+      } else {
+        p.in_kind = IN_TMP;
+        p.tmp_in = En.tmp.p;
+      }
+      if (idx + 1 == order.size() && s == ns) {
+        p.out_kind = OUT_PLANAR;
+        p.h1 = h1 + (long)ns * ng;
+        p.h2 = h2 + (long)ns * ng;
+        p.Ea1 = Ep(0, ns);
+        p.Ea2 = Ep(1, ns);
+        p.dexp1 = dexp1;
+        p.dexp2 = dexp2;
+        p.sidx = ns;
+      } else {
+        p.out_kind = OUT_TMP;
+        p.tmp_out = En.tmp.p;
+      }
+      passes.push_back(p);
+    }
+  }
+  return passes;
+}
+
+ChainDesc make_chain(adscf_ctx* c, int nblocks, const int* monomer_type, const int* block_segments, int ensemble) {
+  if (nblocks < 1 || nblocks > kMaxBlocks) throw ArgError{"nblocks must be in [1, 64]"};
+  ChainDesc cd{};
+  cd.nblocks = nblocks;
+  cd.ensemble = ensemble;
+  int s = 0;
+  for (int b = 0; b < nblocks; ++b) {
+    if (monomer_type[b] < 1 || monomer_type[b] > c->nt) throw ArgError{"monomer_type out of range 1..ntypes"};
+    if (block_segments[b] < 0) throw ArgError{"negative block_segments"};
+    cd.type[b] = monomer_type[b] - 1;
+    cd.seg[b] = block_segments[b];
+    cd.start[b] = s;
+    s += block_segments[b];
+  }
+  if (s < 1) throw ArgError{"chain has no segments"};
+  cd.ns = s;
+  return cd;
+}
+
+template <typename T>
+void ensure_hist(adscf_ctx* c, int ns) {
+  Engine<T>& E = eng<T>(c);
+  const size_t want = (size_t)(ns + 1) * c->ng;
+  if (E.h1.n < want || E.h2.n < want) {
+    E.h1.ensure(want);
+    E.h2.ensure(want);
+  }
+  E.tmp.ensure(c->ng);
+  E.E.ensure((size_t)c->nt * c->ng);
+  c->e1.ensure(ns + 1);
+  c->e2.ensure(ns + 1);
+  c->d1.ensure(ns + 1);
+  c->d2.ensure(ns + 1);
+}
+
+void ensure_partials(adscf_ctx* c) {
+  c->qpart.ensure(kRedBlocks);
+  c->rpart.ensure(kRedBlocks);
+  c->spart.ensure(kRedBlocks);
+  c->e2part.ensure((size_t)kRedBlocks * kMaxTypes);
+  c->epart.ensure((size_t)kRedBlocks * 16);
+  c->st.ensure(1);
+  c->cst.ensure(1);
+}
+
+// sweep + Q + normalisation for the chain described by cd, E already in place
+template <typename T>
+void run_chain(adscf_ctx* c, const ChainDesc& cd, ScfState* st, const int* done, double rho, double mu) {
+  Engine<T>& En = eng<T>(c);
+  const long ng = c->ng;
+  auto passes = build_sweep<T>(c, cd, En.E.p, En.h1.p, En.h2.p, c->d1.p, c->d2.p, done);
+  for (auto& p : passes) launch_pass<T>(c, p);
+  const int nb = grid_for(ng);
+  hipLaunchKernelGGL(k_qsum<T>, dim3(nb), dim3(kThreads), 0, c->stream, En.h1.p + (long)cd.ns * ng, ng, c->qpart.p,
+                     done);
+  hipLaunchKernelGGL(k_qfinal, dim3(1), dim3(kThreads), 0, c->stream, c->qpart.p, nb, c->e1.p, cd.ns, c->dvol,
+                     c->volume, cd.ensemble, rho, mu, st, done);
+  HIPCK(hipGetLastError());
+}
+
+template <typename T>
+ContourArgs<T> contour_args(adscf_ctx* c, const ChainDesc& cd, const ScfState* st) {
+  Engine<T>& En = eng<T>(c);
+  ContourArgs<T> a{};
+  a.ng = c->ng;
+  a.cd = cd;
+  a.fh = c->fh;
+  a.fh.nt = c->nt;
+  a.ds = c->ds;
+  a.h1 = En.h1.p;
+  a.h2 = En.h2.p;
+  a.e1 = c->e1.p;
+  a.e2 = c->e2.p;
+  a.st = st;
+  a.m = c->m;
+  a.rpart = c->rpart.p;
+  a.spart = c->spart.p;
+  return a;
+}
+
+template <typename T>
+void fill_ones_slot0(adscf_ctx* c) {
+  Engine<T>& En = eng<T>(c);
+  hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(c->ng)), dim3(kThreads), 0, c->stream, En.h1.p, T(1), c->ng);
+  hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(c->ng)), dim3(kThreads), 0, c->stream, En.h2.p, T(1), c->ng);
+  HIPCK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// compat: Propagate_Multiblock
+// ---------------------------------------------------------------------------
+template <typename T>
+int propagate_chain_impl(adscf_ctx* c, const ChainDesc& cd, double* rho, double* mu, const double* expfield,
+                         double* distribution, double* density, double* qsum, double* energy) {
+  ensure_tables<T>(c);
+  ensure_hist<T>(c, cd.ns);
+  ensure_partials(c);
+  Engine<T>& En = eng<T>(c);
+  const long ng = c->ng;
+  const size_t nE = (size_t)c->nt * ng;
+  c->io.ensure(nE);
+  HIPCK(hipMemcpyAsync(c->io.p, expfield, nE * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_convert<T>, dim3(grid_for((long)nE)), dim3(kThreads), 0, c->stream, c->io.p, En.E.p, (long)nE);
+  const int nb = grid_for(ng);
+  hipLaunchKernelGGL(k_e2<T>, dim3(nb), dim3(kThreads), 0, c->stream, En.E.p, ng, c->nt, c->e2part.p);
+  hipLaunchKernelGGL(k_schedule, dim3(1), dim3(64), 0, c->stream, cd, c->e2part.p, nb, ng, c->nt, c->e1.p, c->e2.p,
+                     c->d1.p, c->d2.p, (ScfState*)nullptr, 0, 0, (const int*)nullptr);
+  fill_ones_slot0<T>(c);
+  ScfState init{};
+  HIPCK(hipMemcpyAsync(c->cst.p, &init, sizeof init, hipMemcpyHostToDevice, c->stream));
+  run_chain<T>(c, cd, c->cst.p, nullptr, *rho, *mu);
+  const size_t nbl = (size_t)cd.nblocks * ng;
+  c->dist_b.ensure(nbl);
+  c->dens_b.ensure(nbl);
+  ContourArgs<T> a = contour_args<T>(c, cd, c->cst.p);
+  a.mode = CONTOUR_BLOCKS;
+  a.dist = c->dist_b.p;
+  a.dens = c->dens_b.p;
+  a.rpart = nullptr;
+  hipLaunchKernelGGL(k_contour<T>, dim3(nb), dim3(kThreads), 0, c->stream, a);
+  HIPCK(hipGetLastError());
+  ScfState st{};
+  std::vector<double> sp(nb);
+  HIPCK(hipMemcpyAsync(&st, c->cst.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipMemcpyAsync(sp.data(), c->spart.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (distribution)
+    HIPCK(hipMemcpyAsync(distribution, c->dist_b.p, nbl * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (density) HIPCK(hipMemcpyAsync(density, c->dens_b.p, nbl * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  *qsum = st.qsum;
+  if (cd.ensemble == 1) {
+    *mu = st.mu;
+    *energy = st.energy;
+  } else {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += sp[b];
+    *rho = s * c->dvol / c->volume / (c->ds * cd.ns);
+    *energy = -*rho * c->volume;
+  }
+  return ADSCF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// compat: Propagate_Step on one field
+// ---------------------------------------------------------------------------
+template <typename T>
+int propagate_step_impl(adscf_ctx* c, double* prop, const double* Et) {
+  ensure_tables<T>(c);
+  Engine<T>& En = eng<T>(c);
+  const long ng = c->ng;
+  En.cq.ensure(2 * ng);
+  En.tmp.ensure(ng);
+  c->io.ensure(2 * ng);
+  HIPCK(hipMemcpyAsync(c->io.p, prop, ng * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCK(hipMemcpyAsync(c->io.p + ng, Et, ng * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_convert<T>, dim3(grid_for(2 * ng)), dim3(kThreads), 0, c->stream, c->io.p, En.cq.p, 2 * ng);
+  T* q = En.cq.p;
+  const T* E = En.cq.p + ng;
+  const std::vector<int> A = active_axes(c);
+  for (size_t i = 0; i < A.size(); ++i) {
+    PassArgs<T> p = base_pass<T>(c, A[i]);
+    p.Ea1 = p.Ea2 = E;
+    if (i == 0) {
+      p.in_kind = IN_PLANAR;
+      p.q1_in = q;
+      p.pre_E = 1;
+    } else {
+      p.in_kind = IN_TMP;
+      p.tmp_in = En.tmp.p;
+    }
+    if (i + 1 == A.size()) {
+      p.out_kind = OUT_PLANAR;
+      p.h1 = q;
+    } else {
+      p.out_kind = OUT_TMP;
+      p.tmp_out = En.tmp.p;
+    }
+    launch_pass<T>(c, p);
+  }
+  hipLaunchKernelGGL(k_widen<T>, dim3(grid_for(ng)), dim3(kThreads), 0, c->stream, q, c->io.p, ng);
+  HIPCK(hipMemcpyAsync(prop, c->io.p, ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return ADSCF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// compat: Real2Fourier / Fourier2Real
+// ---------------------------------------------------------------------------
+template <typename T>
+void launch_fft(adscf_ctx* c, FftArgs<T> a) {
+  int items;
+  if (a.axis == 0) items = (int)(((long)a.d1 * a.d2 + a.L - 1) / a.L);
+  else if (a.axis == 1) items = a.d2 * a.gx;
+  else items = a.d1 * a.gx;
+  const size_t smem = 2 * (size_t)a.L * a.ldn * sizeof(C2<T>);
+  hipLaunchKernelGGL(k_fft_axis<T>, dim3(items), dim3(kThreads), smem, c->stream, a);
+  HIPCK(hipGetLastError());
+}
+
+template <typename T>
+FftArgs<T> fft_args(adscf_ctx* c, int axis, int d0, int dir) {
+  Engine<T>& En = eng<T>(c);
+  FftArgs<T> a{};
+  a.d0 = d0;
+  a.d1 = c->ny;
+  a.d2 = c->nz;
+  a.axis = axis;
+  a.n = c->ax[axis].n;
+  a.plan = c->ax[axis].plan;
+  a.L = axis == 0 ? std::max(1, std::min(1024 / a.n, c->ny * c->nz)) : std::max(1, std::min(16, d0));
+  a.ldn = a.n + 1;
+  a.gx = (d0 + a.L - 1) / a.L;
+  a.stride = axis == 1 ? (long)d0 : (long)d0 * c->ny;
+  a.tw = En.tw[axis].p;
+  a.dir = dir;
+  a.scale = T(1);
+  return a;
+}
+
+template <typename T>
+int r2c_impl(adscf_ctx* c, const double* prop, double* out) {
+  ensure_tables<T>(c);
+  Engine<T>& En = eng<T>(c);
+  const long ng = c->ng;
+  const int nh = c->nx / 2 + 1;
+  const long nk = (long)nh * c->ny * c->nz;
+  En.cq.ensure(2 * ng);
+  En.ch.ensure(nk);
+  c->io.ensure(std::max(ng, 2 * nk));
+  HIPCK(hipMemcpyAsync(c->io.p, prop, ng * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_convert<T>, dim3(grid_for(ng)), dim3(kThreads), 0, c->stream, c->io.p, En.cq.p, ng);
+  FftArgs<T> a = fft_args<T>(c, 0, nh, -1);
+  a.load = LOAD_REAL;
+  a.rin = En.cq.p;
+  a.store = STORE_HALF;
+  a.cout = En.ch.p;
+  launch_fft<T>(c, a);
+  const T inv = (T)(1.0 / (double)(c->nx * c->ny * c->nz));  // mod_fourier.f90:102
+  const bool doy = c->ny > 1, doz = c->nz > 1;
+  if (doy) {
+    FftArgs<T> b = fft_args<T>(c, 1, nh, -1);
+    b.load = LOAD_CPLX;
+    b.cin = En.ch.p;
+    b.store = STORE_CPLX;
+    b.cout = En.ch.p;
+    if (!doz) b.scale = inv;
+    launch_fft<T>(c, b);
+  }
+  if (doz) {
+    FftArgs<T> b = fft_args<T>(c, 2, nh, -1);
+    b.load = LOAD_CPLX;
+    b.cin = En.ch.p;
+    b.store = STORE_CPLX;
+    b.cout = En.ch.p;
+    b.scale = inv;
+    launch_fft<T>(c, b);
+  }
+  if (!doy && !doz) {
+    // scale through a trivial z "pass" of length 1
+    FftArgs<T> b = fft_args<T>(c, 2, nh, -1);
+    b.load = LOAD_CPLX;
+    b.cin = En.ch.p;
+    b.store = STORE_CPLX;
+    b.cout = En.ch.p;
+    b.scale = inv;
+    launch_fft<T>(c, b);
+  }
+  hipLaunchKernelGGL(k_widen<T>, dim3(grid_for(2 * nk)), dim3(kThreads), 0, c->stream, (const T*)En.ch.p, c->io.p,
+                     2 * nk);
+  HIPCK(hipMemcpyAsync(out, c->io.p, 2 * nk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return ADSCF_OK;
+}
+
+template <typename T>
+int c2r_impl(adscf_ctx* c, const double* in, double* prop) {
+  ensure_tables<T>(c);
+  Engine<T>& En = eng<T>(c);
+  const long ng = c->ng;
+  const int nh = c->nx / 2 + 1;
+  const long nk = (long)nh * c->ny * c->nz;
+  En.cq.ensure(2 * ng);
+  En.ch.ensure(nk);
+  c->io.ensure(std::max(ng, 2 * nk));
+  HIPCK(hipMemcpyAsync(c->io.p, in, 2 * nk * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_convert<T>, dim3(grid_for(2 * nk)), dim3(kThreads), 0, c->stream, c->io.p, (T*)En.ch.p,
+                     2 * nk);
+  if (c->nz > 1) {
+    FftArgs<T> b = fft_args<T>(c, 2, nh, +1);
+    b.load = LOAD_CPLX;
+    b.cin = En.ch.p;
+    b.store = STORE_CPLX;
+    b.cout = En.ch.p;
+    launch_fft<T>(c, b);
+  }
+  if (c->ny > 1) {
+    FftArgs<T> b = fft_args<T>(c, 1, nh, +1);
+    b.load = LOAD_CPLX;
+    b.cin = En.ch.p;
+    b.store = STORE_CPLX;
+    b.cout = En.ch.p;
+    launch_fft<T>(c, b);
+  }
+  FftArgs<T> a = fft_args<T>(c, 0, nh, +1);
+  a.load = LOAD_HERM;
+  a.cin = En.ch.p;
+  a.store = STORE_REAL;
+  a.rout = En.cq.p;
+  launch_fft<T>(c, a);
+  hipLaunchKernelGGL(k_widen<T>, dim3(grid_for(ng)), dim3(kThreads), 0, c->stream, (const T*)En.cq.p, c->io.p, ng);
+  HIPCK(hipMemcpyAsync(prop, c->io.p, ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return ADSCF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device-resident SCF iteration
+// ---------------------------------------------------------------------------
+template <typename T>
+void enqueue_iteration(adscf_ctx* c) {
+  Engine<T>& En = eng<T>(c);
+  const long ng = c->ng;
+  const long nvar = ng * c->nt;
+  const int* done = &c->st.p->done;
+  run_chain<T>(c, c->cd, c->st.p, done, c->rho, c->mu);
+  const int nb = grid_for(ng);
+  ContourArgs<T> a = contour_args<T>(c, c->cd, c->st.p);
+  a.mode = CONTOUR_SCF;
+  a.phi = c->phi.p;
+  a.dring = c->dring.p;
+  a.done = done;
+  a.w0 = c->wring.p;  // ring base; the kernel reads slot `head` from the device state
+  hipLaunchKernelGGL(k_contour<T>, dim3(nb), dim3(kThreads), 0, c->stream, a);
+  HIPCK(hipGetLastError());
+  const int m = c->m;
+  const int np = (m + 1) * (m + 2) / 2;
+  const int ngram = grid_for(nvar / kGramTile + 1);
+  const size_t gsm = (size_t)(m + 1) * (kGramTile + 1) * sizeof(double);
+  hipLaunchKernelGGL(k_gram, dim3(ngram), dim3(kThreads), gsm, c->stream, c->dring.p, nvar, m,
+                     (const ScfState*)c->st.p, 0, c->gpart.p, done);
+  SolveArgs s{};
+  s.m = m;
+  s.lam_mix = c->lam_mix;
+  s.nblk_contour = nb;
+  s.rpart = c->rpart.p;
+  s.spart = c->spart.p;
+  s.nblk_gram = ngram;
+  s.gpart = c->gpart.p;
+  s.resid_hist = c->resid.p;
+  s.resid_cap = (int)c->resid.n;
+  s.dvol = c->dvol;
+  s.volume = c->volume;
+  s.ds = c->ds;
+  s.ns = c->cd.ns;
+  s.ensemble = c->cd.ensemble;
+  s.st = c->st.p;
+  hipLaunchKernelGGL(k_solve, dim3(1), dim3(kThreads), 0, c->stream, s);
+  hipLaunchKernelGGL(k_mix<T>, dim3(nb), dim3(kThreads), 0, c->stream, c->wring.p, (const double*)c->dring.p, ng,
+                     c->nt, m, (const ScfState*)c->st.p, c->ds, En.E.p, c->e2part.p);
+  hipLaunchKernelGGL(k_schedule, dim3(1), dim3(64), 0, c->stream, c->cd, (const double*)c->e2part.p, nb, ng, c->nt,
+                     c->e1.p, c->e2.p, c->d1.p, c->d2.p, c->st.p, 1, m, done);
+  HIPCK(hipGetLastError());
+  (void)np;
+}
+
+template <typename T>
+void enqueue_evaluate(adscf_ctx* c) {
+  const long ng = c->ng;
+  run_chain<T>(c, c->cd, c->st.p, nullptr, c->rho, c->mu);
+  ContourArgs<T> a = contour_args<T>(c, c->cd, c->st.p);
+  a.mode = CONTOUR_SCF;
+  a.w0 = c->wring.p;
+  a.phi = c->phi.p;
+  a.dring = nullptr;
+  a.done = nullptr;
+  hipLaunchKernelGGL(k_contour<T>, dim3(grid_for(ng)), dim3(kThreads), 0, c->stream, a);
+  HIPCK(hipGetLastError());
+}
+
+template <typename T>
+void scf_set_field_impl(adscf_ctx* c, const double* field) {
+  ensure_tables<T>(c);
+  ensure_hist<T>(c, c->cd.ns);
+  ensure_partials(c);
+  Engine<T>& En = eng<T>(c);
+  const long ng = c->ng, nvar = ng * c->nt;
+  const int m = c->m;
+  c->wring.ensure((size_t)(m + 1) * nvar);
+  c->dring.ensure((size_t)(m + 1) * nvar);
+  c->phi.ensure(nvar);
+  c->gpart.ensure((size_t)kRedBlocks * (m + 1) * (m + 2) / 2);
+  c->resid.ensure(200000);
+  HIPCK(hipMemsetAsync(c->wring.p, 0, sizeof(double) * (m + 1) * nvar, c->stream));
+  HIPCK(hipMemsetAsync(c->dring.p, 0, sizeof(double) * (m + 1) * nvar, c->stream));
+  HIPCK(hipMemcpyAsync(c->wring.p, field, nvar * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  ScfState init{};
+  init.head = 0;
+  HIPCK(hipMemcpyAsync(c->st.p, &init, sizeof init, hipMemcpyHostToDevice, c->stream));
+  fill_ones_slot0<T>(c);
+  const int nb = grid_for(ng);
+  hipLaunchKernelGGL(k_expfield<T>, dim3(nb), dim3(kThreads), 0, c->stream, (const double*)c->wring.p, En.E.p, ng,
+                     c->nt, c->ds, c->e2part.p, (const int*)nullptr);
+  hipLaunchKernelGGL(k_schedule, dim3(1), dim3(64), 0, c->stream, c->cd, (const double*)c->e2part.p, nb, ng, c->nt,
+                     c->e1.p, c->e2.p, c->d1.p, c->d2.p, (ScfState*)nullptr, 0, m, (const int*)nullptr);
+  HIPCK(hipGetLastError());
+  HIPCK(hipStreamSynchronize(c->stream));
+}
+
+template <typename T>
+std::vector<const void*> graph_key(adscf_ctx* c) {
+  Engine<T>& En = eng<T>(c);
+  return {En.E.p, En.tmp.p, En.h1.p, En.h2.p, En.tw[0].p, En.tw[1].p, En.tw[2].p, En.kf[0].p, En.kf[1].p,
+          En.kf[2].p, c->wring.p, c->dring.p, c->phi.p, c->qpart.p, c->rpart.p, c->spart.p, c->e2part.p,
+          c->gpart.p, c->resid.p, c->e1.p, c->e2.p, c->d1.p, c->d2.p, c->st.p};
+}
+
+// E and the rescale schedule of the current field in precision T (the history is kept)
+template <typename T>
+void refresh_E(adscf_ctx* c) {
+  ensure_tables<T>(c);
+  ensure_hist<T>(c, c->cd.ns);
+  ensure_partials(c);
+  fill_ones_slot0<T>(c);
+  ScfState st{};
+  HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  Engine<T>& En = eng<T>(c);
+  const long ng = c->ng;
+  const int nb = grid_for(ng);
+  hipLaunchKernelGGL(k_expfield<T>, dim3(nb), dim3(kThreads), 0, c->stream,
+                     (const double*)(c->wring.p + (size_t)st.head * ng * c->nt), En.E.p, ng, c->nt, c->ds,
+                     c->e2part.p, (const int*)nullptr);
+  hipLaunchKernelGGL(k_schedule, dim3(1), dim3(64), 0, c->stream, c->cd, (const double*)c->e2part.p, nb, ng, c->nt,
+                     c->e1.p, c->e2.p, c->d1.p, c->d2.p, (ScfState*)nullptr, 0, c->m, (const int*)nullptr);
+  HIPCK(hipGetLastError());
+  HIPCK(hipStreamSynchronize(c->stream));
+}
+
+template <typename T>
+void build_graph(adscf_ctx* c) {
+  ensure_tables<T>(c);
+  std::vector<const void*> key = graph_key<T>(c);
+  if (c->graph && c->graph_prec == c->prec && key == c->graph_key) return;
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  if (c->graph_src) {
+    (void)hipGraphDestroy(c->graph_src);
+    c->graph_src = nullptr;
+  }
+  HIPCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    enqueue_iteration<T>(c);
+  } catch (...) {
+    hipGraph_t g;
+    (void)hipStreamEndCapture(c->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  HIPCK(hipStreamEndCapture(c->stream, &c->graph_src));
+  HIPCK(hipGraphInstantiate(&c->graph, c->graph_src, nullptr, nullptr, 0));
+  c->graph_prec = c->prec;
+  c->graph_key = key;
+}
+
+template <typename T>
+int scf_iterate_impl(adscf_ctx* c, int niter, double tol, int* iters, double* acc, int* conv) {
+  build_graph<T>(c);
+  HIPCK(hipMemcpyAsync(&c->st.p->tol, &tol, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  ScfState st{};
+  int done_iters = 0;
+  float ms_total = 0.f;
+  while (done_iters < niter) {
+    const int chunk = std::min(niter - done_iters, tol > 0 ? 25 : niter);
+    HIPCK(hipEventRecord(c->ev0, c->stream));
+    for (int i = 0; i < chunk; ++i) HIPCK(hipGraphLaunch(c->graph, c->stream));
+    HIPCK(hipEventRecord(c->ev1, c->stream));
+    HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    ms_total += ms;
+    done_iters += chunk;
+    if (st.done) break;
+  }
+  c->last_ms_iter = done_iters > 0 ? ms_total / done_iters : 0.0;
+  c->phi_fresh = st.done != 0;
+  if (iters) *iters = st.iter;
+  if (acc) *acc = st.accuracy;
+  if (conv) *conv = st.done;
+  return ADSCF_OK;
+}
+
+template <typename T>
+double sweep_timing(adscf_ctx* c, int reps) {
+  // time the propagator sweep alone (the line passes of one SCF iteration)
+  Engine<T>& En = eng<T>(c);
+  auto passes = build_sweep<T>(c, c->cd, En.E.p, En.h1.p, En.h2.p, c->d1.p, c->d2.p, nullptr);
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  HIPCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  for (auto& p : passes) launch_pass<T>(c, p);
+  HIPCK(hipStreamEndCapture(c->stream, &g));
+  HIPCK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  HIPCK(hipGraphLaunch(ge, c->stream));
+  HIPCK(hipEventRecord(c->ev0, c->stream));
+  for (int r = 0; r < reps; ++r) HIPCK(hipGraphLaunch(ge, c->stream));
+  HIPCK(hipEventRecord(c->ev1, c->stream));
+  HIPCK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  (void)hipGraphExecDestroy(ge);
+  (void)hipGraphDestroy(g);
+  return ms / reps;
+}
+
+void reset_axes(adscf_ctx* c) {
+  const int d[3] = {c->nx, c->ny, c->nz};
+  for (int a = 0; a < 3; ++a) {
+    c->ax[a].n = d[a];
+    c->ax[a].plan = make_line_plan(d[a]);
+    if (c->ax[a].plan.npass < 0) throw ArgError{"grid dimension has more than 12 prime factors"};
+    c->ax[a].L = choose_tile(a, d[a], c->nx, c->ny, c->nz);
+  }
+}
+
+void host_energy_parts(adscf_ctx* c, const double* d_phi, const double* d_w, std::vector<double>& out) {
+  const int nb = grid_for(c->ng);
+  hipLaunchKernelGGL(k_energy_parts, dim3(nb), dim3(kThreads), 0, c->stream, d_phi, d_w, c->ng, c->fh,
+                     c->epart.p, 16);
+  HIPCK(hipGetLastError());
+  std::vector<double> part((size_t)nb * 16);
+  HIPCK(hipMemcpyAsync(part.data(), c->epart.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  const int nq = 2 + c->nt * (c->nt - 1) / 2;
+  out.assign(nq, 0.0);
+  for (int b = 0; b < nb; ++b)
+    for (int q = 0; q < nq; ++q) out[q] += part[(size_t)b * 16 + q];
+}
+
+double interaction_from_parts(const adscf_ctx* c, const std::vector<double>& p) {
+  double e = 0.5 * c->fh.kappa * c->dvol * p[1];
+  int q = 2;
+  for (int t = 0; t < c->nt; ++t)
+    for (int t2 = t + 1; t2 < c->nt; ++t2) e = e + c->fh.chi[t + t2 * c->nt] * c->dvol * p[q++];
+  return e;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+int adscf_init(adscf_ctx** out, int nx, int ny, int nz, int ntypes, double lx, double ly, double lz, double ds,
+               int precision) {
+  if (!out) return ADSCF_ERR_ARG;
+  *out = nullptr;
+  if (nx < 1 || ny < 1 || nz < 1 || ntypes < 1 || ntypes > kMaxTypes || !(lx > 0) || !(ly > 0) || !(lz > 0) ||
+      !(ds > 0) || (precision != ADSCF_FP64 && precision != ADSCF_FP32))
+    return ADSCF_ERR_ARG;
+  auto* c = new (std::nothrow) adscf_ctx();
+  if (!c) return ADSCF_ERR_HIP;
+  int rc = guarded(c, [&] {
+    int ndev = 0;
+    HIPCK(hipGetDeviceCount(&ndev));
+    if (ndev < 1) throw ArgError{"no HIP device"};
+    HIPCK(hipGetDevice(&c->dev));
+    HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCK(hipEventCreate(&c->ev0));
+    HIPCK(hipEventCreate(&c->ev1));
+    c->nx = nx;
+    c->ny = ny;
+    c->nz = nz;
+    c->nt = ntypes;
+    c->ng = (long)nx * ny * nz;
+    c->lx = lx;
+    c->ly = ly;
+    c->lz = lz;
+    c->ds = ds;
+    c->volume = lx * ly * lz;                      // mod_fourier.f90:154
+    c->dvol = c->volume / (double)(nx * ny * nz);  // :155
+    c->prec = precision;
+    c->fh.nt = ntypes;
+    reset_axes(c);
+    return ADSCF_OK;
+  });
+  if (rc != ADSCF_OK) {
+    fprintf(stderr, "adscf_init: %s\n", c->err.c_str());
+    adscf_finalize(c);
+    return rc;
+  }
+  *out = c;
+  return ADSCF_OK;
+}
+
+int adscf_finalize(adscf_ctx* c) {
+  if (!c) return ADSCF_ERR_ARG;
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  if (c->graph_src) (void)hipGraphDestroy(c->graph_src);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  hipStream_t s = c->stream;
+  delete c;  // DevBuf destructors free device memory
+  if (s) (void)hipStreamDestroy(s);
+  return ADSCF_OK;
+}
+
+int adscf_set_box(adscf_ctx* c, double lx, double ly, double lz) {
+  return guarded(c, [&] {
+    if (!(lx > 0) || !(ly > 0) || !(lz > 0)) throw ArgError{"box lengths must be positive"};
+    c->lx = lx;
+    c->ly = ly;
+    c->lz = lz;
+    c->volume = lx * ly * lz;
+    c->dvol = c->volume / (double)(c->nx * c->ny * c->nz);
+    invalidate_tables(c);
+    return ADSCF_OK;
+  });
+}
+
+int adscf_set_ds(adscf_ctx* c, double ds) {
+  return guarded(c, [&] {
+    if (!(ds > 0)) throw ArgError{"ds must be positive"};
+    c->ds = ds;
+    invalidate_tables(c);
+    return ADSCF_OK;
+  });
+}
+
+int adscf_set_precision(adscf_ctx* c, int precision) {
+  return guarded(c, [&] {
+    if (precision != ADSCF_FP64 && precision != ADSCF_FP32) throw ArgError{"precision must be 0 (FP64) or 1 (FP32)"};
+    if (precision == c->prec) return ADSCF_OK;
+    c->prec = precision;
+    if (c->field_set) {  // continue the same SCF run (history kept) in the new precision
+      if (precision == ADSCF_FP64) refresh_E<double>(c);
+      else refresh_E<float>(c);
+      c->phi_fresh = false;
+    }
+    return ADSCF_OK;
+  });
+}
+
+int adscf_volume(const adscf_ctx* c, double* volume, double* dvol) {
+  if (!c) return ADSCF_ERR_ARG;
+  if (volume) *volume = c->volume;
+  if (dvol) *dvol = c->dvol;
+  return ADSCF_OK;
+}
+
+const char* adscf_last_error(const adscf_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int adscf_real2fourier(adscf_ctx* c, const double* prop, double* pf) {
+  return guarded(c, [&] {
+    if (!prop || !pf) throw ArgError{"null array"};
+    return c->prec == ADSCF_FP64 ? r2c_impl<double>(c, prop, pf) : r2c_impl<float>(c, prop, pf);
+  });
+}
+
+int adscf_fourier2real(adscf_ctx* c, const double* pf, double* prop) {
+  return guarded(c, [&] {
+    if (!prop || !pf) throw ArgError{"null array"};
+    return c->prec == ADSCF_FP64 ? c2r_impl<double>(c, pf, prop) : c2r_impl<float>(c, pf, prop);
+  });
+}
+
+int adscf_laplace(adscf_ctx* c, double* laplace, double* explaplace, double* nabla) {
+  // Host tables, as Calculate_Laplace fills module arrays (mod_fourier.f90:175-194).
+  return guarded(c, [&] {
+    const double pi = 3.141592653589793;
+    const int nh = c->nx / 2 + 1;
+    const long nk = (long)nh * c->ny * c->nz;
+    const double dx = c->lx / c->nx, dy = c->ly / c->ny, dz = c->lz / c->nz;
+    for (int z = 1; z <= c->nz; ++z)
+      for (int y = 1; y <= c->ny; ++y)
+        for (int x = 1; x <= nh; ++x) {
+          double qx = 2.0 * pi * (double)(x - 1) / c->lx;
+          double qy = 2.0 * pi * (double)(y - 1) / c->ly;
+          double qz = 2.0 * pi * (double)(z - 1) / c->lz;
+          if (y > c->ny / 2 + 1) qy = 2.0 * pi * ((double)(y - 1) - (double)c->ny) / c->ly;
+          if (z > c->nz / 2 + 1) qz = 2.0 * pi * ((double)(z - 1) - (double)c->nz) / c->lz;
+          const long i = ((long)(z - 1) * c->ny + (y - 1)) * nh + (x - 1);
+          const double k2 = qx * qx + qy * qy + qz * qz;
+          if (laplace) {
+            laplace[2 * i] = -k2;
+            laplace[2 * i + 1] = 0.0;
+          }
+          if (explaplace) {
+            explaplace[2 * i] = exp(-c->ds * k2);
+            explaplace[2 * i + 1] = 0.0;
+          }
+          if (nabla) {
+            nabla[2 * i] = 0.0;
+            nabla[2 * i + 1] = sin(qx * dx) / dx;
+            nabla[2 * (i + nk)] = 0.0;
+            nabla[2 * (i + nk) + 1] = sin(qy * dy) / dy;
+            nabla[2 * (i + 2 * nk)] = 0.0;
+            nabla[2 * (i + 2 * nk) + 1] = sin(qz * dz) / dz;
+          }
+        }
+    return ADSCF_OK;
+  });
+}
+
+int adscf_expfield(adscf_ctx* c, const double* field, double* expfield) {
+  return guarded(c, [&] {
+    if (!field || !expfield) throw ArgError{"null array"};
+    const long n = (long)c->nt * c->ng;
+    c->io.ensure(n);
+    c->io2.ensure(n);
+    HIPCK(hipMemcpyAsync(c->io.p, field, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_expfield<double>, dim3(grid_for(c->ng)), dim3(kThreads), 0, c->stream,
+                       (const double*)c->io.p, c->io2.p, c->ng, c->nt, c->ds, (double*)nullptr, (const int*)nullptr);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(expfield, c->io2.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    return ADSCF_OK;
+  });
+}
+
+int adscf_propagate_step(adscf_ctx* c, double* prop, const double* E) {
+  return guarded(c, [&] {
+    if (!prop || !E) throw ArgError{"null array"};
+    return c->prec == ADSCF_FP64 ? propagate_step_impl<double>(c, prop, E) : propagate_step_impl<float>(c, prop, E);
+  });
+}
+
+int adscf_propagate_chain(adscf_ctx* c, int nblocks, const int* monomer_type, const int* block_segments, int ensemble,
+                          double* rho, double* mu, const double* expfield, double* distribution, double* density,
+                          double* qsum, double* energy) {
+  return guarded(c, [&]() -> int {
+    if (!monomer_type || !block_segments || !rho || !mu || !expfield || !qsum || !energy)
+      throw ArgError{"null argument"};
+    if (ensemble != 1 && ensemble != 2) {
+      c->err = "Invalid ensemble; Cannot calculate Density";  // mod_multiblock.f90:182
+      return ADSCF_ERR_ENSEMBLE;
+    }
+    ChainDesc cd = make_chain(c, nblocks, monomer_type, block_segments, ensemble);
+    return c->prec == ADSCF_FP64
+               ? propagate_chain_impl<double>(c, cd, rho, mu, expfield, distribution, density, qsum, energy)
+               : propagate_chain_impl<float>(c, cd, rho, mu, expfield, distribution, density, qsum, energy);
+  });
+}
+
+int adscf_set_interactions(adscf_ctx* c, double kappa, const double* chi, const double* vol) {
+  return guarded(c, [&] {
+    if (!chi || !vol) throw ArgError{"null array"};
+    c->fh.nt = c->nt;
+    c->fh.kappa = kappa;
+    for (int i = 0; i < c->nt * c->nt; ++i) c->fh.chi[i] = chi[i];
+    for (int i = 0; i < c->nt; ++i) c->fh.vol[i] = vol[i];
+    c->fh_set = true;
+    invalidate_tables(c);  // the captured iteration embeds the parameters
+    c->tables64 = c->tables32 = false;
+    return ADSCF_OK;
+  });
+}
+
+int adscf_mean_fields(adscf_ctx* c, const double* density, double* field) {
+  return guarded(c, [&] {
+    if (!density || !field) throw ArgError{"null array"};
+    if (!c->fh_set) throw ArgError{"adscf_set_interactions has not been called"};
+    const long n = (long)c->nt * c->ng;
+    c->io.ensure(n);
+    c->io2.ensure(n);
+    HIPCK(hipMemcpyAsync(c->io.p, density, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_mean_fields, dim3(grid_for(c->ng)), dim3(kThreads), 0, c->stream, (const double*)c->io.p,
+                       c->io2.p, c->ng, c->fh);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(field, c->io2.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    return ADSCF_OK;
+  });
+}
+
+int adscf_interaction_energy(adscf_ctx* c, const double* density, double* energy) {
+  return guarded(c, [&] {
+    if (!density || !energy) throw ArgError{"null array"};
+    if (!c->fh_set) throw ArgError{"adscf_set_interactions has not been called"};
+    ensure_partials(c);
+    const long n = (long)c->nt * c->ng;
+    c->io.ensure(n);
+    HIPCK(hipMemcpyAsync(c->io.p, density, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    std::vector<double> p;
+    host_energy_parts(c, c->io.p, nullptr, p);
+    *energy = interaction_from_parts(c, p);
+    return ADSCF_OK;
+  });
+}
+
+int adscf_average_volume_fraction(adscf_ctx* c, const double* density, double* average) {
+  return guarded(c, [&] {
+    if (!density || !average) throw ArgError{"null array"};
+    if (!c->fh_set) throw ArgError{"adscf_set_interactions has not been called"};
+    ensure_partials(c);
+    const long n = (long)c->nt * c->ng;
+    c->io.ensure(n);
+    HIPCK(hipMemcpyAsync(c->io.p, density, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const int nb = grid_for(c->ng);
+    // sum(volume_fraction) / Ng with volume_fraction_t = density_t * v_t (mod_FloryHuggins.f90:147-151)
+    double tot = 0.0;
+    for (int t = 0; t < c->nt; ++t) {
+      hipLaunchKernelGGL(k_qsum<double>, dim3(nb), dim3(kThreads), 0, c->stream,
+                         (const double*)(c->io.p + (long)t * c->ng), c->ng, c->qpart.p, (const int*)nullptr);
+      std::vector<double> part(nb);
+      HIPCK(hipMemcpyAsync(part.data(), c->qpart.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCK(hipStreamSynchronize(c->stream));
+      double s = 0.0;
+      for (int b = 0; b < nb; ++b) s += part[b];
+      tot += s * c->fh.vol[t];
+    }
+    *average = tot / (double)(c->nx * c->ny * c->nz);
+    return ADSCF_OK;
+  });
+}
+
+int adscf_mixing_simple(adscf_ctx* c, double* field, const double* dfield, long fieldsize, int mixing_dim,
+                        double lambda) {
+  return guarded(c, [&] {
+    if (!field || !dfield || fieldsize < 1 || mixing_dim < 0) throw ArgError{"bad mixing arguments"};
+    ensure_partials(c);
+    c->io.ensure(fieldsize);
+    c->io2.ensure(fieldsize);
+    HIPCK(hipMemcpyAsync(c->io.p, field, fieldsize * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipMemcpyAsync(c->io2.p, dfield, fieldsize * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // field(:,0) = field(:,0) + lambda*dfield(:,0): one mixing step with no history
+    ScfState st{};
+    st.mode = 0;
+    st.lam = lambda;
+    HIPCK(hipMemcpyAsync(c->cst.p, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
+    c->e32.cmixE.ensure(fieldsize);
+    // with m = 0 the ring has one slot and the new field overwrites it in place
+    hipLaunchKernelGGL(k_mix<float>, dim3(grid_for(fieldsize)), dim3(kThreads), 0, c->stream, c->io.p,
+                       (const double*)c->io2.p, fieldsize, 1, 0, (const ScfState*)c->cst.p, 0.0, c->e32.cmixE.p,
+                       c->e2part.p);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(field, c->io.p, fieldsize * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    return ADSCF_OK;
+  });
+}
+
+int adscf_mixing_lambda(adscf_ctx* c, double* field, double* dfield, long fieldsize, int mixing_dim, double lammax,
+                        int iter) {
+  return guarded(c, [&] {
+    if (!field || !dfield || fieldsize < 1 || mixing_dim < 1) throw ArgError{"bad mixing arguments"};
+    ensure_partials(c);
+    // Mixing_Lambda (mod_iterate.f90:46-84): lambda from two norms, then shift column 0 into 1
+    const double lammin = 0.00001;
+    double lambda = lammin;
+    if (iter != 1) {
+      c->io.ensure(2 * fieldsize);
+      c->io2.ensure(2 * fieldsize);
+      HIPCK(hipMemcpyAsync(c->io.p, field, 2 * fieldsize * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      HIPCK(hipMemcpyAsync(c->io2.p, dfield, 2 * fieldsize * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      // the m = 1 Gram kernel over the two-slot rings gives <D0-D1, D0-D1>; the field
+      // differences use the same kernel on the field ring
+      c->gpart.ensure((size_t)kRedBlocks * 3);
+      ScfState st{};
+      HIPCK(hipMemcpyAsync(c->cst.p, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
+      const int ng2 = grid_for(fieldsize / kGramTile + 1);
+      const size_t gsm = 2 * (kGramTile + 1) * sizeof(double);
+      double sums[2];
+      for (int which = 0; which < 2; ++which) {
+        hipLaunchKernelGGL(k_gram, dim3(ng2), dim3(kThreads), gsm, c->stream,
+                           (const double*)(which == 0 ? c->io.p : c->io2.p), fieldsize, 1,
+                           (const ScfState*)c->cst.p, 1, c->gpart.p, (const int*)nullptr);
+        HIPCK(hipGetLastError());
+        std::vector<double> part((size_t)ng2 * 3);
+        HIPCK(hipMemcpyAsync(part.data(), c->gpart.p, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+        double s = 0.0;
+        for (int b = 0; b < ng2; ++b) s += part[(size_t)b * 3 + 2];  // pair (1,1)
+        sums[which] = s;
+      }
+      const double num = sums[0], nom = sums[1];
+      if (nom > 1.0e-50) {
+        lambda = sqrt(num / nom);
+        if (lambda < lammin) lambda = lammin;
+        else if (lambda > lammax) lambda = lammax;
+      }
+    }
+    // shift column 0 into 1 and update column 0 (host-array semantics, done on device)
+    c->io.ensure(fieldsize);
+    c->io2.ensure(fieldsize);
+    HIPCK(hipMemcpyAsync(c->io.p, field, fieldsize * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipMemcpyAsync(c->io2.p, dfield, fieldsize * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    ScfState st{};
+    st.mode = 0;
+    st.lam = lambda;
+    HIPCK(hipMemcpyAsync(c->cst.p, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
+    c->e32.cmixE.ensure(fieldsize);
+    hipLaunchKernelGGL(k_mix<float>, dim3(grid_for(fieldsize)), dim3(kThreads), 0, c->stream, c->io.p,
+                       (const double*)c->io2.p, fieldsize, 1, 0, (const ScfState*)c->cst.p, 0.0, c->e32.cmixE.p,
+                       c->e2part.p);
+    HIPCK(hipGetLastError());
+    std::memcpy(field + fieldsize, field, fieldsize * sizeof(double));
+    std::memcpy(dfield + fieldsize, dfield, fieldsize * sizeof(double));
+    HIPCK(hipMemcpyAsync(field, c->io.p, fieldsize * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    return ADSCF_OK;
+  });
+}
+
+int adscf_mixing_anderson(adscf_ctx* c, double* field, double* dfield, long fieldsize, int m, double lam_mix,
+                          int iter) {
+  return guarded(c, [&] {
+    if (!field || !dfield || fieldsize < 1 || m < 1 || m > kMaxHist) throw ArgError{"bad mixing arguments"};
+    ensure_partials(c);
+    const size_t tot = (size_t)(m + 1) * fieldsize;
+    c->io.ensure(tot);
+    c->io2.ensure(tot);
+    HIPCK(hipMemcpyAsync(c->io.p, field, tot * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipMemcpyAsync(c->io2.p, dfield, tot * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    ScfState st{};
+    HIPCK(hipMemcpyAsync(c->cst.p, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
+    const int np = (m + 1) * (m + 2) / 2;
+    const int ngram = grid_for(fieldsize / kGramTile + 1);
+    c->gpart.ensure((size_t)kRedBlocks * np);
+    const int status = iter < 1000 ? iter : 1000;
+    if (status > m) {
+      const size_t gsm = (size_t)(m + 1) * (kGramTile + 1) * sizeof(double);
+      hipLaunchKernelGGL(k_gram, dim3(ngram), dim3(kThreads), gsm, c->stream, (const double*)c->io2.p, fieldsize, m,
+                         (const ScfState*)c->cst.p, 1, c->gpart.p, (const int*)nullptr);
+    }
+    SolveArgs s{};
+    s.m = m;
+    s.lam_mix = lam_mix;
+    s.nblk_gram = ngram;
+    s.gpart = c->gpart.p;
+    s.iter_override = iter;
+    s.st = c->cst.p;
+    hipLaunchKernelGGL(k_solve, dim3(1), dim3(kThreads), 0, c->stream, s);
+    c->e32.cmixE.ensure(fieldsize);
+    hipLaunchKernelGGL(k_mix<float>, dim3(grid_for(fieldsize)), dim3(kThreads), 0, c->stream, c->io.p,
+                       (const double*)c->io2.p, fieldsize, 1, m, (const ScfState*)c->cst.p, 0.0, c->e32.cmixE.p,
+                       c->e2part.p);
+    HIPCK(hipGetLastError());
+    // ring with head 0: the new field sits in slot m; the shift (:143-146) maps slot i-1 to column i
+    const size_t fb = fieldsize * sizeof(double);
+    HIPCK(hipMemcpyAsync(field, c->io.p + (size_t)m * fieldsize, fb, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipMemcpyAsync(field + fieldsize, c->io.p, (size_t)m * fb, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipMemcpyAsync(dfield + fieldsize, c->io2.p, (size_t)m * fb, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    return ADSCF_OK;
+  });
+}
+
+int adscf_ludcmp(double* a, int n, int* indx, double* d) {
+  if (!a || !indx || !d || n < 1) return ADSCF_ERR_ARG;
+  std::vector<double> vv(n + 1);
+  lu_decompose(a, n, indx, d, vv.data());
+  return ADSCF_OK;
+}
+
+int adscf_lubksb(const double* a, int n, const int* indx, double* b) {
+  if (!a || !indx || !b || n < 1) return ADSCF_ERR_ARG;
+  lu_backsubstitute(a, n, indx, b);
+  return ADSCF_OK;
+}
+
+int adscf_solve_lu(const double* a, const double* b, double* x, int n) {
+  if (!a || !b || !x || n < 1) return ADSCF_ERR_ARG;
+  std::vector<double> aa(a, a + (size_t)n * n), vv(n + 1);
+  std::vector<int> indx(n);
+  double d;
+  for (int i = 0; i < n; ++i) x[i] = b[i];
+  lu_decompose(aa.data(), n, indx.data(), &d, vv.data());
+  lu_backsubstitute(aa.data(), n, indx.data(), x);
+  return ADSCF_OK;
+}
+
+int adscf_solve_cholesky(const double* A, const double* B, double* X, int n) {
+  // Solve_LinearSystem_Cholesky (mod_iterate.f90:160-195), A = L L^T
+  if (!A || !B || !X || n < 1) return ADSCF_ERR_ARG;
+  std::vector<double> L((size_t)n * n, 0.0), Y(n);
+  auto a = [&](int i, int j) { return A[(i - 1) + (size_t)(j - 1) * n]; };
+  auto l = [&](int i, int j) -> double& { return L[(i - 1) + (size_t)(j - 1) * n]; };
+  l(1, 1) = sqrt(a(1, 1));
+  for (int i = 2; i <= n; ++i) {
+    for (int j = 1; j <= i - 1; ++j) {
+      double s = 0.0;
+      for (int k = 1; k <= j - 1; ++k) s += l(i, k) * l(j, k);
+      l(i, j) = (a(i, j) - s) / l(j, j);
+    }
+    double s = 0.0;
+    for (int k = 1; k <= i - 1; ++k) s += l(i, k) * l(i, k);
+    l(i, i) = sqrt(a(i, i) - s);
+  }
+  for (int i = 1; i <= n; ++i) {
+    double s = 0.0;
+    for (int k = 1; k <= i - 1; ++k) s += l(i, k) * Y[k - 1];
+    Y[i - 1] = (B[i - 1] - s) / l(i, i);
+  }
+  for (int j = n; j >= 1; --j) {
+    double s = 0.0;
+    for (int k = j + 1; k <= n; ++k) s += l(k, j) * X[k - 1];
+    X[j - 1] = (Y[j - 1] - s) / l(j, j);
+  }
+  return ADSCF_OK;
+}
+
+int adscf_scf_setup(adscf_ctx* c, int nblocks, const int* monomer_type, const int* block_segments, int ensemble,
+                    double rho, double mu, int mixing_dim, double lam_mix) {
+  return guarded(c, [&]() -> int {
+    if (!monomer_type || !block_segments) throw ArgError{"null array"};
+    if (ensemble != 1 && ensemble != 2) {
+      c->err = "Invalid ensemble; Cannot calculate Density";
+      return ADSCF_ERR_ENSEMBLE;
+    }
+    if (mixing_dim < 1 || mixing_dim > kMaxHist) throw ArgError{"mixing_dim must be in [1, 40]"};
+    if (!c->fh_set) throw ArgError{"adscf_set_interactions has not been called"};
+    c->cd = make_chain(c, nblocks, monomer_type, block_segments, ensemble);
+    c->m = mixing_dim;
+    c->lam_mix = lam_mix;
+    c->rho = rho;
+    c->mu = mu;
+    c->scf_ready = true;
+    c->field_set = false;
+    invalidate_tables(c);
+    return ADSCF_OK;
+  });
+}
+
+int adscf_scf_set_field(adscf_ctx* c, const double* field) {
+  return guarded(c, [&]() -> int {
+    if (!field) throw ArgError{"null array"};
+    if (!c->scf_ready) {
+      c->err = "adscf_scf_setup has not been called";
+      return ADSCF_ERR_STATE;
+    }
+    if (c->prec == ADSCF_FP64) scf_set_field_impl<double>(c, field);
+    else scf_set_field_impl<float>(c, field);
+    c->field_set = true;
+    c->phi_fresh = false;
+    return ADSCF_OK;
+  });
+}
+
+int adscf_scf_iterate(adscf_ctx* c, int niter, double tol, int* iters, double* accuracy, int* converged) {
+  return guarded(c, [&]() -> int {
+    if (!c->field_set) {
+      c->err = "adscf_scf_set_field has not been called";
+      return ADSCF_ERR_STATE;
+    }
+    if (niter < 0) throw ArgError{"niter must be >= 0"};
+    return c->prec == ADSCF_FP64 ? scf_iterate_impl<double>(c, niter, tol, iters, accuracy, converged)
+                                 : scf_iterate_impl<float>(c, niter, tol, iters, accuracy, converged);
+  });
+}
+
+int adscf_scf_evaluate(adscf_ctx* c) {
+  return guarded(c, [&]() -> int {
+    if (!c->field_set) {
+      c->err = "adscf_scf_set_field has not been called";
+      return ADSCF_ERR_STATE;
+    }
+    if (c->prec == ADSCF_FP64) enqueue_evaluate<double>(c);
+    else enqueue_evaluate<float>(c);
+    HIPCK(hipStreamSynchronize(c->stream));
+    c->phi_fresh = true;
+    return ADSCF_OK;
+  });
+}
+
+int adscf_scf_get(adscf_ctx* c, double* field, double* density) {
+  return guarded(c, [&]() -> int {
+    if (!c->field_set) {
+      c->err = "adscf_scf_set_field has not been called";
+      return ADSCF_ERR_STATE;
+    }
+    ScfState st{};
+    HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    const long nvar = c->ng * c->nt;
+    if (field)
+      HIPCK(hipMemcpyAsync(field, c->wring.p + (size_t)st.head * nvar, nvar * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    if (density) HIPCK(hipMemcpyAsync(density, c->phi.p, nvar * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    return ADSCF_OK;
+  });
+}
+
+int adscf_scf_energy(adscf_ctx* c, double* F, double* Fc, double* qsum, double* mu, double* rho) {
+  return guarded(c, [&]() -> int {
+    if (!c->field_set) {
+      c->err = "adscf_scf_set_field has not been called";
+      return ADSCF_ERR_STATE;
+    }
+    if (!c->phi_fresh) {
+      if (c->prec == ADSCF_FP64) enqueue_evaluate<double>(c);
+      else enqueue_evaluate<float>(c);
+      c->phi_fresh = true;
+    }
+    ScfState st{};
+    HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    double rho_v = st.rho, energy = st.energy;
+    if (c->cd.ensemble == 2) {
+      // grand canonical rho from the densities of this evaluation
+      std::vector<double> sp(grid_for(c->ng));
+      HIPCK(hipMemcpyAsync(sp.data(), c->spart.p, sp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCK(hipStreamSynchronize(c->stream));
+      double s = 0.0;
+      for (double v : sp) s += v;
+      rho_v = s * c->dvol / c->volume / (c->ds * c->cd.ns);
+      energy = -rho_v * c->volume;
+    }
+    std::vector<double> p;
+    host_energy_parts(c, c->phi.p, c->wring.p + (size_t)st.head * c->ng * c->nt, p);
+    const double chain_energy = energy - c->dvol * p[0];
+    const double f = chain_energy + interaction_from_parts(c, p);
+    if (F) *F = f;
+    if (Fc) *Fc = f / (rho_v * c->volume);
+    if (qsum) *qsum = st.qsum;
+    if (mu) *mu = st.mu;
+    if (rho) *rho = rho_v;
+    return ADSCF_OK;
+  });
+}
+
+int adscf_scf_timing(adscf_ctx* c, double* ms_per_iter, double* ms_sweep) {
+  return guarded(c, [&]() -> int {
+    if (!c->field_set) {
+      c->err = "adscf_scf_set_field has not been called";
+      return ADSCF_ERR_STATE;
+    }
+    if (ms_per_iter) *ms_per_iter = c->last_ms_iter;
+    if (ms_sweep) {
+      c->last_ms_sweep = c->prec == ADSCF_FP64 ? sweep_timing<double>(c, 5) : sweep_timing<float>(c, 5);
+      *ms_sweep = c->last_ms_sweep;
+    }
+    return ADSCF_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,315 @@
+// Device kernels of the SCFT propagator path (HIP, gfx950).
+//
+// The diffusion operator of one contour step, F^-1 exp(-ds k^2) F, is the
+// tensor product Lx Ly Lz of three 1-D circulant operators because
+// exp(-ds k^2) factorises over the axes.  A step is therefore applied as
+// E * L_a L_b L_c * E with one "line pass" per axis; each pass loads a tile of
+// lines into LDS, runs a forward Stockham FFT, multiplies by K_axis/n, runs the
+// inverse FFT and writes the lines back.  Steps alternate the axis order
+// (x,y,z then z,y,x), so the last pass of a step and the first pass of the
+// next act on the same axis and merge into one pass that also multiplies by
+// E, stores q(s) and multiplies by the next step's E.  A contour step thus
+// costs two passes over the grid.  The forward propagator q and the
+// complementary propagator q+ travel together as the real and imaginary parts
+// of one complex line (both operators are real and even in k).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fft_line.hpp"
+
+namespace adscf {
+
+constexpr int kThreads = 256;
+constexpr int kMaxBlocks = 64;  // chain blocks
+constexpr int kMaxTypes = 4;    // monomer types
+constexpr int kMaxHist = 40;    // Anderson history depth
+
+// ---------------------------------------------------------------------------
+// line pass
+// ---------------------------------------------------------------------------
+enum InKind : int { IN_TMP = 0, IN_ONES = 1, IN_PLANAR = 2 };
+enum OutKind : int { OUT_TMP = 0, OUT_PLANAR = 1 };
+
+template <typename T>
+struct PassArgs {
+  int nx, ny, nz;
+  int axis;     // 0 x, 1 y, 2 z
+  int L;        // lines per tile
+  int ldn;      // LDS leading dimension (n + 1)
+  int gx;       // tiles along x for the y/z axes: ceil(nx / L)
+  long stride;  // point stride along the line
+  LinePlan plan;
+  const C2<T>* tw;  // n forward twiddles
+  const T* kf;      // n multipliers exp(-ds q_k^2) / n
+  int nops;         // 1, or 2 for a merged pass (op, mid store, op)
+  int in_kind;
+  int pre_E;        // multiply the loaded lines by (Ea1, Ea2)
+  int out_kind;
+  const C2<T>* tmp_in;
+  const T* q1_in;
+  const T* q2_in;   // may be null (treated as 0)
+  const T* Ea1;     // E of the step that ends in this pass (and starts, with pre_E)
+  const T* Ea2;
+  const T* Eb1;     // E of the step that starts after the mid store
+  const T* Eb2;
+  T* h1;            // planar output q(s) / q+(s)
+  T* h2;            // may be null
+  const int* dexp1; // power-of-two rescale exponents, indexed by sidx
+  const int* dexp2;
+  int sidx;
+  C2<T>* tmp_out;
+  const int* done;  // skip all work once the SCF loop has converged
+};
+
+template <typename T>
+__device__ __forceinline__ long pass_line_base(const PassArgs<T>& a, int item, int l, bool& valid) {
+  if (a.axis == 0) {
+    const long r = (long)item * a.L + l;
+    valid = r < (long)a.ny * a.nz;
+    return r * a.nx;
+  }
+  const int x0 = (item % a.gx) * a.L;
+  const int o = item / a.gx;
+  valid = (x0 + l) < a.nx;
+  if (a.axis == 1) return (long)o * a.nx * a.ny + x0 + l;  // o = z
+  return (long)o * a.nx + x0 + l;                            // o = y
+}
+
+template <typename T>
+__device__ __forceinline__ void pass_elem(const PassArgs<T>& a, int e, int n, int& l, int& j) {
+  if (a.axis == 0) {
+    l = e / n;
+    j = e - l * n;
+  } else {
+    l = e % a.L;
+    j = e / a.L;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T exp2i(int e) {
+  return (T)ldexp(1.0, e);
+}
+
+template <typename T>
+__device__ C2<T>* pass_line_op(const PassArgs<T>& a, C2<T>* b0, C2<T>* b1) {
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int n = a.plan.n;
+  auto sync = [] { __syncthreads(); };
+  C2<T>* s = line_fft<T>(b0, b1, a.plan, a.ldn, a.L, a.tw, -1, tid, nthr, sync);
+  const int tot = a.L * n;
+  for (int e = tid; e < tot; e += nthr) {
+    const int l = e / n, k = e - l * n;
+    const T f = a.kf[k];
+    C2<T> v = s[l * a.ldn + k];
+    s[l * a.ldn + k] = {v.x * f, v.y * f};
+  }
+  __syncthreads();
+  C2<T>* o = (s == b0) ? b1 : b0;
+  return line_fft<T>(s, o, a.plan, a.ldn, a.L, a.tw, +1, tid, nthr, sync);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_line_pass(PassArgs<T> a) {
+  if (a.done && *a.done) return;
+  extern __shared__ unsigned char smem_raw[];
+  C2<T>* b0 = reinterpret_cast<C2<T>*>(smem_raw);
+  C2<T>* b1 = b0 + a.L * a.ldn;
+  const int n = a.plan.n;
+  const int tot = a.L * n;
+  const int item = blockIdx.x;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  // ---- load
+  for (int e = tid; e < tot; e += nthr) {
+    int l, j;
+    pass_elem(a, e, n, l, j);
+    bool valid;
+    const long base = pass_line_base(a, item, l, valid);
+    C2<T> v = {T(0), T(0)};
+    if (valid) {
+      const long g = base + (long)j * a.stride;
+      if (a.in_kind == IN_TMP) {
+        v = a.tmp_in[g];
+      } else if (a.in_kind == IN_ONES) {
+        v = {T(1), a.h2 ? T(1) : T(0)};
+      } else {
+        v.x = a.q1_in[g];
+        v.y = a.q2_in ? a.q2_in[g] : T(0);
+      }
+      if (a.pre_E) {
+        v.x *= a.Ea1[g];
+        v.y *= a.Ea2[g];
+      }
+    }
+    b0[l * a.ldn + j] = v;
+  }
+  __syncthreads();
+  C2<T>* r = pass_line_op(a, b0, b1);
+  if (a.nops == 2) {
+    const T sc1 = exp2i<T>(a.dexp1[a.sidx]);
+    const T sc2 = exp2i<T>(a.dexp2[a.sidx]);
+    for (int e = tid; e < tot; e += nthr) {
+      int l, j;
+      pass_elem(a, e, n, l, j);
+      bool valid;
+      const long base = pass_line_base(a, item, l, valid);
+      if (!valid) continue;
+      const long g = base + (long)j * a.stride;
+      C2<T> v = r[l * a.ldn + j];
+      v.x = v.x * a.Ea1[g] * sc1;
+      v.y = v.y * a.Ea2[g] * sc2;
+      a.h1[g] = v.x;
+      a.h2[g] = v.y;
+      v.x *= a.Eb1[g];
+      v.y *= a.Eb2[g];
+      r[l * a.ldn + j] = v;
+    }
+    __syncthreads();
+    C2<T>* other = (r == b0) ? b1 : b0;
+    r = pass_line_op(a, r, other);
+  }
+  // ---- store
+  if (a.out_kind == OUT_TMP) {
+    for (int e = tid; e < tot; e += nthr) {
+      int l, j;
+      pass_elem(a, e, n, l, j);
+      bool valid;
+      const long base = pass_line_base(a, item, l, valid);
+      if (valid) a.tmp_out[base + (long)j * a.stride] = r[l * a.ldn + j];
+    }
+  } else {
+    const T sc1 = a.dexp1 ? exp2i<T>(a.dexp1[a.sidx]) : T(1);
+    const T sc2 = a.dexp2 ? exp2i<T>(a.dexp2[a.sidx]) : T(1);
+    for (int e = tid; e < tot; e += nthr) {
+      int l, j;
+      pass_elem(a, e, n, l, j);
+      bool valid;
+      const long base = pass_line_base(a, item, l, valid);
+      if (!valid) continue;
+      const long g = base + (long)j * a.stride;
+      C2<T> v = r[l * a.ldn + j];
+      a.h1[g] = v.x * a.Ea1[g] * sc1;
+      if (a.h2) a.h2[g] = v.y * a.Ea2[g] * sc2;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// generic line FFT along one axis of a complex array (compat r2c / c2r)
+// ---------------------------------------------------------------------------
+enum FftLoad : int { LOAD_CPLX = 0, LOAD_REAL = 1, LOAD_HERM = 2 };
+enum FftStore : int { STORE_CPLX = 0, STORE_HALF = 1, STORE_REAL = 2 };
+
+template <typename T>
+struct FftArgs {
+  int d0, d1, d2;  // array dims of the complex input, x fastest
+  int axis;
+  int n;           // transform length
+  int L, ldn, gx;
+  long stride;
+  LinePlan plan;
+  const C2<T>* tw;
+  int dir;
+  T scale;
+  int load, store;
+  const T* rin;       // LOAD_REAL: real rows of length n (x axis)
+  const C2<T>* cin;   // LOAD_CPLX / LOAD_HERM (rows of length d0 = n/2+1)
+  C2<T>* cout;        // STORE_CPLX / STORE_HALF (rows of length n/2+1)
+  T* rout;            // STORE_REAL
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_fft_axis(FftArgs<T> a) {
+  extern __shared__ unsigned char smem_raw[];
+  C2<T>* b0 = reinterpret_cast<C2<T>*>(smem_raw);
+  C2<T>* b1 = b0 + a.L * a.ldn;
+  const int n = a.n, tot = a.L * n, item = blockIdx.x;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int nh = n / 2 + 1;
+  for (int e = tid; e < tot; e += nthr) {
+    int l, j;
+    if (a.axis == 0) { l = e / n; j = e - l * n; } else { l = e % a.L; j = e / a.L; }
+    C2<T> v = {T(0), T(0)};
+    if (a.axis == 0) {
+      const long r = (long)item * a.L + l;
+      if (r < (long)a.d1 * a.d2) {
+        if (a.load == LOAD_REAL) v = {a.rin[r * n + j], T(0)};
+        else if (a.load == LOAD_HERM) {
+          if (j < nh) v = a.cin[r * nh + j];
+          else { C2<T> c = a.cin[r * nh + (n - j)]; v = {c.x, -c.y}; }
+        } else v = a.cin[r * n + j];
+      }
+    } else {
+      const int x0 = (item % a.gx) * a.L, o = item / a.gx;
+      if (x0 + l < a.d0) {
+        const long base = (a.axis == 1) ? (long)o * a.d0 * a.d1 + x0 + l : (long)o * a.d0 + x0 + l;
+        v = a.cin[base + (long)j * a.stride];
+      }
+    }
+    b0[l * a.ldn + j] = v;
+  }
+  __syncthreads();
+  auto sync = [] { __syncthreads(); };
+  C2<T>* r = line_fft<T>(b0, b1, a.plan, a.ldn, a.L, a.tw, a.dir, tid, nthr, sync);
+  for (int e = tid; e < tot; e += nthr) {
+    int l, j;
+    if (a.axis == 0) { l = e / n; j = e - l * n; } else { l = e % a.L; j = e / a.L; }
+    C2<T> v = r[l * a.ldn + j];
+    v = {v.x * a.scale, v.y * a.scale};
+    if (a.axis == 0) {
+      const long rr = (long)item * a.L + l;
+      if (rr < (long)a.d1 * a.d2) {
+        if (a.store == STORE_HALF) { if (j < nh) a.cout[rr * nh + j] = v; }
+        else if (a.store == STORE_REAL) a.rout[rr * n + j] = v.x;
+        else a.cout[rr * n + j] = v;
+      }
+    } else {
+      const int x0 = (item % a.gx) * a.L, o = item / a.gx;
+      if (x0 + l < a.d0) {
+        const long base = (a.axis == 1) ? (long)o * a.d0 * a.d1 + x0 + l : (long)o * a.d0 + x0 + l;
+        a.cout[base + (long)j * a.stride] = v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// device state of the SCF loop and chain description
+// ---------------------------------------------------------------------------
+struct ChainDesc {
+  int nblocks;
+  int ns;                      // total segments
+  int type[kMaxBlocks];        // 0-based monomer type per block
+  int seg[kMaxBlocks];         // block_segments
+  int start[kMaxBlocks];       // first segment index of each block
+  int ensemble;                // 1 canonical, 2 grand canonical
+};
+
+struct FHDesc {
+  int nt;
+  double kappa;
+  double chi[kMaxTypes * kMaxTypes];  // column-major
+  double vol[kMaxTypes];
+};
+
+// Scalars shared by the kernels of one SCF iteration; lives in device memory.
+struct ScfState {
+  int head;        // physical ring slot of logical history index 0
+  int iter;        // iterations evaluated (reference loop index of the last one)
+  int done;        // converged: every later kernel returns at once
+  int mode;        // 0 simple (lam00), 1 Anderson
+  int singular;    // ludcmp saw an all-zero row
+  int pad_;
+  double tol;
+  double accuracy; // max|dfield| of the last evaluation
+  double lam;      // step used by the last mix
+  double qsum;     // Qsum of the last evaluation
+  double norm;     // density = norm * distribution
+  double mu, rho, energy;  // chain scalars of the last evaluation
+  double x[kMaxHist];      // Anderson coefficients
+};
+
+}  // namespace adscf

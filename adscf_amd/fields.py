@@ -1,0 +1,94 @@
+"""Seeded synthetic fields for the benchmark and parity workloads.
+
+Random numbers come from MT19937 seeded the way the reference seeds it
+(sgrnd: mt[0] = seed, mt[i] = 69069*mt[i-1] mod 2^32,
+General_Routines/mod_mt.f90:71-87), drawn through numpy's MT19937 whose state
+is set to that key.  grnd() = y / (2^32 - 1) (mod_mt.f90:142-146).  Normal
+deviates use Box-Muller on u = (y + 0.5) / 2^32, which is never 0.  Grid point
+i (0-based) sits at i*dx, as in Calculate_Laplace's phase convention
+(Geometry_Modules/Bulk/mod_fourier.f90:178).  Arrays are (k, nz, ny, nx),
+x fastest.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def sgrnd_key(seed: int) -> np.ndarray:
+    key = np.empty(624, dtype=np.uint64)
+    key[0] = seed & 0xFFFFFFFF
+    for i in range(1, 624):
+        key[i] = (69069 * int(key[i - 1])) & 0xFFFFFFFF
+    return key.astype(np.uint32)
+
+
+class MTStream:
+    """The reference's MT19937 word stream for a seed."""
+
+    def __init__(self, seed: int):
+        self.bg = np.random.MT19937()
+        self.bg.state = {"bit_generator": "MT19937", "state": {"key": sgrnd_key(seed), "pos": 624}}
+
+    def words(self, n: int) -> np.ndarray:
+        return self.bg.random_raw(n).astype(np.uint64)
+
+    def grnd(self, n: int) -> np.ndarray:
+        return self.words(n).astype(np.float64) / (2.0 ** 32 - 1.0)
+
+    def normals(self, n: int) -> np.ndarray:
+        m = (n + 1) // 2
+        y = self.words(2 * m).astype(np.float64)
+        u1 = (y[0::2] + 0.5) / 2.0 ** 32
+        u2 = (y[1::2] + 0.5) / 2.0 ** 32
+        r = np.sqrt(-2.0 * np.log(u1))
+        z = np.empty(2 * m)
+        z[0::2] = r * np.cos(2.0 * np.pi * u2)
+        z[1::2] = r * np.sin(2.0 * np.pi * u2)
+        return z[:n]
+
+
+def _coords(n, L):
+    return np.arange(n) * (L / n)
+
+
+def lamellar_seed(n, L, chiN, sigma=0.1, seed=2007):
+    """w_A = -w_B = 0.5*chiN*(cos(2*pi*x/L) + eta), eta ~ N(0, sigma^2) per point."""
+    x = _coords(n, L)
+    eta = MTStream(seed).normals(n ** 3).reshape(n, n, n) * sigma
+    wa = 0.5 * chiN * (np.cos(2 * np.pi * x / L)[None, None, :] + eta)
+    return np.stack([wa, -wa])
+
+
+def cylinder_seed(n, L, chiN, sigma=0.2, seed=2007):
+    """w_A - w_B = chiN*[(cos(k.x) + cos(k(x/2+sqrt3 y/2)) + cos(k(x/2-sqrt3 y/2)))/3 + eta],
+    split as w_A = -w_B; k = 2*pi/L, eta ~ N(0, sigma^2)."""
+    k = 2 * np.pi / L
+    x = _coords(n, L)[None, None, :]
+    y = _coords(n, L)[None, :, None]
+    hexa = (np.cos(k * x) + np.cos(k * (x / 2 + np.sqrt(3) * y / 2)) + np.cos(k * (x / 2 - np.sqrt(3) * y / 2))) / 3
+    eta = MTStream(seed).normals(n ** 3).reshape(n, n, n) * sigma
+    diff = chiN * (hexa + eta)
+    return np.stack([0.5 * diff, -0.5 * diff])
+
+
+def bcc_seed(n, L, chiN, sigma=0.1, seed=2007):
+    """BCC sphere seed: sum of cos(k.r) over the 12 {110} wavevectors (2*pi/L)(+-1,+-1,0) and
+    permutations, normalised by 12, plus eta ~ N(0, sigma^2); split as w_A = -w_B."""
+    k = 2 * np.pi / L
+    x = _coords(n, L)[None, None, :]
+    y = _coords(n, L)[None, :, None]
+    z = _coords(n, L)[:, None, None]
+    s = np.zeros((n, n, n))
+    for a, b in ((x, y), (y, z), (x, z)):
+        for sa in (1, -1):
+            for sb in (1, -1):
+                s = s + np.cos(k * (sa * a + sb * b))
+    eta = MTStream(seed).normals(n ** 3).reshape(n, n, n) * sigma
+    diff = chiN * (s / 12.0 + eta)
+    return np.stack([0.5 * diff, -0.5 * diff])
+
+
+def block_segments(fractions, ns):
+    """nint(block_length/ds) with ds = 1/ns (Fortran NINT, mod_multiblock.f90:85)."""
+    ds = 1.0 / ns
+    return [int(np.floor(f / ds + 0.5)) for f in fractions]

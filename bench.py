@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""SCF iterations/s of the device-resident SCFT loop at 64^3 x 200 contour steps.
+
+Workload (AB diblock melt, the plain-method GPU configuration): 64^3 grid, box
+4.5 Rg, Ns = 200 (ds = 0.005, block segments nint(0.3*200) = 60 / 140), f_A = 0.3,
+chiN = 20, Helfand kappa = 25, monomer volumes 1, canonical ensemble rho = 1,
+Anderson mixing with history 20 and lam_mix = 0.5.  Initial fields: seeded
+N(0, 0.2^2) noise plus the 6-fold cylinder seed (adscf_amd/fields.py).
+
+A "step" is one SCF iteration: the forward and complementary propagator
+sweeps (2 x 200 contour steps), Q, the contour integral, the mean fields and
+residual, and the Anderson update, all resident in HBM (one replay of the
+captured HIP graph).  value = SCF iterations/s of the FP32-mixed path; the
+plain FP64 path is timed beside it.  The roofline entry covers the dominant
+kernel, the propagator line pass: its algorithmic bytes are 3*Ng*P per contour
+step per chain (read q_s, read E, write q_{s+1}), so a sweep of both chains
+moves 2*Ns*3*Ng*P bytes over 2*Ns+1 passes; the pass duration comes from HIP
+events around replays of the captured sweep on the library's stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SCF iters/s at 64³×200: FP32-mixed vs FP64 GPU at |ΔF|<1e-6; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+N, L, NS, FA, CHIN, KAPPA, M, LAM = 64, 4.5, 200, 0.3, 20.0, 25.0, 20, 0.5
+
+
+def algorithmic_bytes_iter(ng, ns, m, p):
+    """B_iter: sweeps, contour integral (read every q/q+ slice, write phi), Q, E from w,
+    and the FP64 field update (read phi, w, write D; Gram row over m D's; mix over m W's and
+    m D's; write w; push W and D)."""
+    return (2 * ns * 3 * ng * p + (2 * (ns + 1) * ng * p + 2 * ng * 8) + ng * p + 2 * ng * (8 + p)
+            + (3 * m + 6) * 2 * ng * 8)
+
+
+def run_gpu(adscf_amd, fields, precision, steps, warmup, w0):
+    segs = fields.block_segments([FA, 1 - FA], NS)
+    ctx = adscf_amd.Context(N, N, N, L, L, L, 1.0 / NS, precision=precision)
+    chi = [[0.0, CHIN], [CHIN, 0.0]]
+    ctx.set_interactions(KAPPA, chi, [1.0, 1.0])
+    ctx.scf_setup([1, 2], segs, ensemble=1, rho=1.0, mixing_dim=M, lam_mix=LAM)
+    ctx.scf_set_field(w0)
+    ctx.scf_iterate(warmup, tol=-1.0)
+    t0 = time.perf_counter()
+    st = ctx.scf_iterate(steps, tol=-1.0)  # synchronises the library stream on return
+    t1 = time.perf_counter()
+    ms_iter_ev, ms_sweep = ctx.scf_timing()
+    wall = t1 - t0
+    out = dict(iters_per_s=steps / wall, ms_per_iter=1000.0 * wall / steps, ms_per_iter_events=ms_iter_ev,
+               ms_sweep=ms_sweep, iterations_run=st.iters, stopped_early=st.converged)
+    return ctx, out
+
+
+def converge(adscf_amd, fields, w0, max_iter):
+    """FP64 run and FP32-mixed run (FP64 refinement once max|dfield| < 1e-5) to tol 1e-8."""
+    segs = fields.block_segments([FA, 1 - FA], NS)
+    res = {}
+    for name, prec in (("fp64", adscf_amd.FP64), ("fp32mixed", adscf_amd.FP32)):
+        with adscf_amd.Context(N, N, N, L, L, L, 1.0 / NS, precision=prec) as ctx:
+            ctx.set_interactions(KAPPA, [[0.0, CHIN], [CHIN, 0.0]], [1.0, 1.0])
+            ctx.scf_setup([1, 2], segs, ensemble=1, rho=1.0, mixing_dim=M, lam_mix=LAM)
+            ctx.scf_set_field(w0)
+            t0 = time.perf_counter()
+            if prec == adscf_amd.FP32:
+                st = ctx.scf_iterate(max_iter, tol=1e-5)
+                it32 = st.iters
+                ctx.set_precision(adscf_amd.FP64)
+            st = ctx.scf_iterate(max_iter, tol=1e-8)
+            t = time.perf_counter() - t0
+            en = ctx.scf_energy()
+            res[name] = dict(iters=st.iters, accuracy=st.accuracy, converged=st.converged,
+                             F_per_chain=en["F_per_chain"], seconds=t)
+            if prec == adscf_amd.FP32:
+                res[name]["fp32_iters"] = it32
+    res["dF_per_chain"] = abs(res["fp32mixed"]["F_per_chain"] - res["fp64"]["F_per_chain"])
+    return res
+
+
+def cpu_baseline(fields, w0):
+    """The CPU FP64 oracle (single thread) on one SCF iteration of the same workload."""
+    import oracle
+
+    segs = fields.block_segments([FA, 1 - FA], NS)
+    g = oracle.Grid(N, N, N, L, L, L, 1.0 / NS)
+    fh = oracle.FH(KAPPA, [[0.0, CHIN], [CHIN, 0.0]], [1.0, 1.0])
+    t0 = time.perf_counter()
+    oracle.scf_anderson(g, [1, 2], segs, fh, w0, M, lam=LAM, tol=-1.0, maxit=1)
+    t = time.perf_counter() - t0
+    return dict(value=1.0 / t, unit="SCF iters/s", cores=1, kind="port",
+                sample="1 SCF iteration (2x200 propagator steps, contour integral, mean fields, mixing) of the "
+                       "64^3 workload by the single-threaded C FP64 oracle (oracle/scft_oracle.c)",
+                seconds=t)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-converge", action="store_true")
+    ap.add_argument("--converge-max-iter", type=int, default=4000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the SCFT path on a single MI355X; "
+              "multi-GPU runs are not part of it", file=sys.stderr)
+        sys.exit(2)
+    import adscf_amd
+    from adscf_amd import fields
+
+    w0 = fields.cylinder_seed(N, L, CHIN, sigma=0.2, seed=2007)
+    ng = N ** 3
+    ctx64, r64 = run_gpu(adscf_amd, fields, adscf_amd.FP64, args.steps, args.warmup, w0)
+    ctx64.close()
+    ctx32, r32 = run_gpu(adscf_amd, fields, adscf_amd.FP32, args.steps, args.warmup, w0)
+    ctx32.close()
+    npass = 2 * NS + 1
+    sweep_bytes32 = 2 * NS * 3 * ng * 4
+    sweep_bytes64 = 2 * NS * 3 * ng * 8
+    ach32 = sweep_bytes32 / (r32["ms_sweep"] * 1e-3) / 1e9
+    ach64 = sweep_bytes64 / (r64["ms_sweep"] * 1e-3) / 1e9
+    biter32 = algorithmic_bytes_iter(ng, NS, M, 4)
+    biter64 = algorithmic_bytes_iter(ng, NS, M, 8)
+    line = {
+        "metric": METRIC,
+        "value": r32["iters_per_s"],
+        "unit": "SCF iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_iter"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-mixed (f32 propagators/FFTs; f64 Q, F, phi sums, Anderson)",
+        "data": "synthetic (seeded cylinder fields, MT19937 seed 2007)",
+        "config": {"workload": "AB diblock melt SCF, 64^3 grid, box 4.5 Rg, Ns=200 (60/140), f_A=0.3, chiN=20, "
+                               "kappa=25, Anderson m=20, lam_mix=0.5",
+                   "grid": [N, N, N], "contour_steps": NS, "mixing_dim": M, "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach32 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_line_pass<float> (propagator line pass)",
+                     "bytes_per_launch": sweep_bytes32 / npass,
+                     "us_per_launch": r32["ms_sweep"] * 1e3 / npass},
+        "fp64": {"iters_per_s": r64["iters_per_s"], "ms_per_iter": r64["ms_per_iter"],
+                 "roofline_achieved_GBs": ach64, "roofline_frac": ach64 / HBM_PEAK_GBS,
+                 "us_per_pass": r64["ms_sweep"] * 1e3 / npass,
+                 "iteration_bytes_frac": biter64 * r64["iters_per_s"] / (HBM_PEAK_GBS * 1e9)},
+        "fp32mixed_iteration_bytes_frac": biter32 * r32["iters_per_s"] / (HBM_PEAK_GBS * 1e9),
+        "speedup_fp32mixed_vs_fp64": r32["iters_per_s"] / r64["iters_per_s"],
+        "contour_steps_per_s": r32["iters_per_s"] * 2 * NS,
+        "timing_valid": not (r32["stopped_early"] or r64["stopped_early"]),
+    }
+    if not args.no_converge:
+        line["convergence"] = converge(adscf_amd, fields, w0, args.converge_max_iter)
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(fields, w0)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,145 @@
+/*
+ * adscf.h -- C-ABI of libadscf_hip.so, the MI355X (gfx950) implementation of
+ * ADSCF's Bulk_3D_FFT pseudo-spectral SCFT propagator path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns
+ * an int status (ADSCF_OK == 0).  On failure adscf_last_error() describes the
+ * problem; the Fortran replacement modules print it and stop, which is the
+ * reference's own error convention (print*, then stop).
+ *
+ * Layout conventions follow the reference Fortran exactly:
+ *   - a field is real(8)(nx,ny,nz) column-major (x fastest);
+ *   - a field set is real(8)(nx,ny,nz,k) with k the slowest index
+ *     (monomer type, or block);
+ *   - a half spectrum is complex(8)(nx/2+1,ny,nz) stored as interleaved
+ *     {re,im} doubles (layout-compatible with Fortran double complex);
+ *   - monomer types are 1-based;
+ *   - Anderson history arrays are real(8)(fieldsize, 0:mixing_dim).
+ *
+ * All host arrays are owned by the caller.  The context owns every device
+ * buffer and one HIP stream; calls are synchronous (outputs are valid on
+ * return) and not re-entrant for one context.
+ *
+ * Citations name the reference procedure each entry point replaces, relative
+ * to the reference tree's Bulk_3D_FFT directory.
+ */
+#ifndef ADSCF_H
+#define ADSCF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ADSCF_OK 0
+#define ADSCF_ERR_ARG (-1)      /* invalid argument or size */
+#define ADSCF_ERR_HIP (-2)      /* HIP runtime error (device, allocation, launch) */
+#define ADSCF_ERR_ENSEMBLE (-3) /* invalid ensemble: mod_diblock.f90:164-166 stops here */
+#define ADSCF_ERR_STATE (-4)    /* call order violated (e.g. iterate before set_field) */
+
+#define ADSCF_FP64 0 /* plain FP64 path: every array and FFT in double */
+#define ADSCF_FP32 1 /* FP32-mixed: propagators/FFTs in float; Q, F, phi sums, Anderson in double */
+
+typedef struct adscf_ctx adscf_ctx;
+
+/* ---- lifetime ---------------------------------------------------------- */
+
+/* Initialize (General_Routines/sub_wrap.f90:8-40): Initialize_Fourier
+ * (Geometry_Modules/Bulk/mod_fourier.f90:49-54), Calculate_Laplace (:165-198) and
+ * Calculate_Volume_Parameters (:152-157) for a runtime grid (the reference fixes
+ * gridx/gridy/gridz at compile time in module Global, e.g. Main_Diblock_Melt/mod_global.f90:8-19).
+ * ntypes = monomer_types.  precision = ADSCF_FP64 or ADSCF_FP32. */
+int adscf_init(adscf_ctx** ctx, int nx, int ny, int nz, int ntypes, double lx, double ly, double lz, double ds,
+               int precision);
+/* Finish / End_Fourier (sub_wrap.f90:48-59, mod_fourier.f90:62-67) */
+int adscf_finalize(adscf_ctx* ctx);
+/* Box or contour step changed: Calculate_Laplace + Calculate_Volume_Parameters again */
+int adscf_set_box(adscf_ctx* ctx, double lx, double ly, double lz);
+int adscf_set_ds(adscf_ctx* ctx, double ds);
+int adscf_set_precision(adscf_ctx* ctx, int precision);
+/* volume and dvol (mod_fourier.f90:154-155) */
+int adscf_volume(const adscf_ctx* ctx, double* volume, double* dvol);
+const char* adscf_last_error(const adscf_ctx* ctx);
+
+/* ---- Fourier_fftw3 (mod_fourier.f90) ------------------------------------ */
+
+/* Real2Fourier (mod_fourier.f90:94-105): r2c, divided by nx*ny*nz */
+int adscf_real2fourier(adscf_ctx* ctx, const double* prop, double* propfourier);
+/* Fourier2Real (mod_fourier.f90:75-86): unnormalised c2r */
+int adscf_fourier2real(adscf_ctx* ctx, const double* propfourier, double* prop);
+/* public arrays laplace, explaplace, nabla of Calculate_Laplace (mod_fourier.f90:26-28,165-198);
+ * each output is complex (nx/2+1,ny,nz) [nabla: (nx/2+1,ny,nz,3)]; NULL skips it. */
+int adscf_laplace(adscf_ctx* ctx, double* laplace, double* explaplace, double* nabla);
+
+/* ---- Propagate (General_Routines/mod_propagate.f90) ---------------------- */
+
+/* Calculate_Expfield (mod_propagate.f90:22-32): expfield = exp(-0.5*ds*field), (nx,ny,nz,ntypes) */
+int adscf_expfield(adscf_ctx* ctx, const double* field, double* expfield);
+/* Propagate_Step (mod_propagate.f90:40-68): prop <- E * c2r(K * r2c(E*prop)/Ng) */
+int adscf_propagate_step(adscf_ctx* ctx, double* prop, const double* expfield_type);
+
+/* ---- Polymer_Modules: Propagate_Diblock / _Triblock / _Multiblock -------- */
+
+/* Propagate_Multiblock (Polymer_Modules/mod_multiblock.f90:112-188); the same algorithm as
+ * Propagate_Diblock (mod_diblock.f90:100-174) and Propagate_Triblock (mod_triblock.f90:103-177).
+ * monomer_type[nblocks] (1-based), block_segments[nblocks]; ensemble 1 canonical, 2 grand
+ * canonical.  rho, mu are in/out as in the reference type.  expfield (nx,ny,nz,ntypes);
+ * distribution and density (nx,ny,nz,nblocks); either may be NULL.  qsum, energy: out. */
+int adscf_propagate_chain(adscf_ctx* ctx, int nblocks, const int* monomer_type, const int* block_segments,
+                          int ensemble, double* rho, double* mu, const double* expfield, double* distribution,
+                          double* density, double* qsum, double* energy);
+
+/* ---- Interactions_FloryHuggins (Interaction_Modules/mod_FloryHuggins.f90) - */
+
+/* The values Initialize_Interactions reads from input_interactions (:37-66):
+ * kappa, chi (ntypes x ntypes column-major, symmetric) and monomer volumes (ntypes). */
+int adscf_set_interactions(adscf_ctx* ctx, double kappa, const double* chi, const double* monomer_volume);
+/* Calculate_Mean_Fields (:104-129): density (nx,ny,nz,ntypes) -> field */
+int adscf_mean_fields(adscf_ctx* ctx, const double* density, double* field);
+/* Interaction_Energy (:74-96) */
+int adscf_interaction_energy(adscf_ctx* ctx, const double* density, double* energy);
+/* Check_Volume_Fraction (:139-160): the average total volume fraction it prints */
+int adscf_average_volume_fraction(adscf_ctx* ctx, const double* density, double* average);
+
+/* ---- Iterate (General_Routines/mod_iterate.f90) ------------------------- */
+
+/* field, dfield: (fieldsize, 0:mixing_dim), updated in place as the reference does. */
+int adscf_mixing_simple(adscf_ctx* ctx, double* field, const double* dfield, long fieldsize, int mixing_dim,
+                        double lambda);                                            /* :27-38 */
+int adscf_mixing_lambda(adscf_ctx* ctx, double* field, double* dfield, long fieldsize, int mixing_dim,
+                        double lammax, int iter);                                  /* :46-84 */
+int adscf_mixing_anderson(adscf_ctx* ctx, double* field, double* dfield, long fieldsize, int mixing_dim,
+                          double lam_mix, int iter);                               /* :92-150 */
+/* Host routines with the reference's semantics (column-major n x n): */
+int adscf_ludcmp(double* a, int n, int* indx, double* d);                          /* :232-298 */
+int adscf_lubksb(const double* a, int n, const int* indx, double* b);              /* :302-337 */
+int adscf_solve_lu(const double* a, const double* b, double* x, int n);            /* :203-224 */
+int adscf_solve_cholesky(const double* a, const double* b, double* x, int n);      /* :160-195 */
+
+/* ---- device-resident SCF loop ------------------------------------------ */
+/* The loop of Find_SCF_Saddle_Point (Main_Multiblock/main_multiblock.f90:648-700):
+ * Calculate_Densities (514-540) -> Calculate_Mean_Fields -> dfield = new - field ->
+ * accuracy = maxval(abs(dfield)) (773-783) -> exit if .not.(accuracy > tol) ->
+ * Mix_Fields/Mixing_Anderson with a SAVE'd history (708-765), all kept in HBM.
+ * Iterations are captured once in a HIP graph and replayed. */
+int adscf_scf_setup(adscf_ctx* ctx, int nblocks, const int* monomer_type, const int* block_segments, int ensemble,
+                    double rho, double mu, int mixing_dim, double lam_mix);
+/* load field (nx,ny,nz,ntypes), clear the mixing history and the iteration count */
+int adscf_scf_set_field(adscf_ctx* ctx, const double* field);
+/* run up to niter iterations; stops early when .not.(accuracy > tol).  iters_total is the
+ * reference's loop index so far, accuracy the last max|dfield|, converged 0/1. */
+int adscf_scf_iterate(adscf_ctx* ctx, int niter, double tol, int* iters_total, double* accuracy, int* converged);
+/* recompute densities of the current field (main_multiblock.f90:319) without mixing */
+int adscf_scf_evaluate(adscf_ctx* ctx);
+/* copy the current field and the species densities of the last evaluation; NULL skips */
+int adscf_scf_get(adscf_ctx* ctx, double* field, double* density);
+/* Calculate_Energy (main_multiblock.f90:548-557) for the last evaluation: total F, F per chain
+ * (F / (rho*V)), Qsum, mu, rho.  Any pointer may be NULL. */
+int adscf_scf_energy(adscf_ctx* ctx, double* F, double* F_per_chain, double* qsum, double* mu, double* rho);
+/* mean device time per SCF iteration of the last adscf_scf_iterate call, and of the
+ * propagator sweep alone, in milliseconds (HIP events on the context stream) */
+int adscf_scf_timing(adscf_ctx* ctx, double* ms_per_iter, double* ms_sweep);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
