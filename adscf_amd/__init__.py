@@ -75,6 +75,7 @@ SIGNATURES = {
     "adscf_scf_evaluate": (C.c_int, [_ctxp]),
     "adscf_scf_get": (C.c_int, [_ctxp, _dp, _dp]),
     "adscf_scf_energy": (C.c_int, [_ctxp, _dp, _dp, _dp, _dp, _dp]),
+    "adscf_scf_residuals": (C.c_int, [_ctxp, _dp, C.c_int, _ip]),
     "adscf_scf_timing": (C.c_int, [_ctxp, _dp, _dp]),
 }
 
@@ -330,6 +331,12 @@ class Context:
         self._chk(self._lib.adscf_scf_energy(self._h, C.byref(F), C.byref(Fc), C.byref(q), C.byref(mu),
                                              C.byref(rho)))
         return dict(F=F.value, F_per_chain=Fc.value, Qsum=q.value, mu=mu.value, rho=rho.value)
+
+    def scf_residuals(self, cap=200000):
+        out = np.empty(cap)
+        n = C.c_int()
+        self._chk(self._lib.adscf_scf_residuals(self._h, _p(out), int(cap), C.byref(n)))
+        return out[:min(n.value, cap)].copy()
 
     def scf_timing(self):
         a, b = C.c_double(), C.c_double()

@@ -112,6 +112,7 @@ struct adscf_ctx {
   int graph_prec = -1;
   std::vector<const void*> graph_key;  // buffers baked into the captured iteration
   double last_ms_iter = 0, last_ms_sweep = 0;
+  bool use_reg = true;  // register-resident line kernels where a four-step shape exists
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -240,8 +241,71 @@ PassArgs<T> base_pass(adscf_ctx* c, int axis) {
   return p;
 }
 
+// Four-step shapes with register-resident kernels; other lengths use the
+// generic Stockham kernel.
+bool reg_shape(int n, int& N1, int& N2) {
+  static const int tab[][3] = {{2, 1, 2},   {3, 1, 3},   {4, 1, 4},   {5, 1, 5},   {6, 1, 6},   {8, 1, 8},
+                               {16, 1, 16}, {10, 2, 5},  {12, 4, 3},  {20, 4, 5},  {24, 8, 3},  {32, 4, 8},
+                               {36, 6, 6},  {40, 8, 5},  {48, 8, 6},  {64, 8, 8},  {80, 16, 5}, {96, 16, 6},
+                               {128, 8, 16}, {256, 16, 16}};
+  for (const auto& r : tab)
+    if (r[0] == n) {
+      N1 = r[1];
+      N2 = r[2];
+      return true;
+    }
+  return false;
+}
+
+int reg_tpl_rt(int N1, int N2) { return N1 == 1 ? 1 : std::max(N1, N2); }
+
+template <typename T, int N1, int N2>
+void launch_reg(adscf_ctx* c, PassArgs<T> p) {
+  constexpr int TPL = reg_tpl<N1, N2>();
+  // one wave per block by default; the tile width L sets the lines per block
+  int L = std::max(1, 64 / TPL);
+  if (p.axis != 0) L = std::min(L, c->nx);
+  p.L = L;
+  p.gx = (c->nx + L - 1) / L;
+  int items;
+  if (p.axis == 0) items = (int)(((long)c->ny * c->nz + L - 1) / L);
+  else if (p.axis == 1) items = c->nz * p.gx;
+  else items = c->ny * p.gx;
+  const size_t smem = N1 == 1 ? 0 : (size_t)L * FourStep<N1, N2>::lds_elems * sizeof(C2<T>);
+  hipLaunchKernelGGL((k_line_pass_reg<T, N1, N2>), dim3(items), dim3(TPL * L), smem, c->stream, p);
+  HIPCK(hipGetLastError());
+}
+
+template <typename T>
+bool launch_pass_reg(adscf_ctx* c, const PassArgs<T>& p) {
+  switch (p.plan.n) {
+    case 2: launch_reg<T, 1, 2>(c, p); return true;
+    case 3: launch_reg<T, 1, 3>(c, p); return true;
+    case 4: launch_reg<T, 1, 4>(c, p); return true;
+    case 5: launch_reg<T, 1, 5>(c, p); return true;
+    case 6: launch_reg<T, 1, 6>(c, p); return true;
+    case 8: launch_reg<T, 1, 8>(c, p); return true;
+    case 16: launch_reg<T, 1, 16>(c, p); return true;
+    case 10: launch_reg<T, 2, 5>(c, p); return true;
+    case 12: launch_reg<T, 4, 3>(c, p); return true;
+    case 20: launch_reg<T, 4, 5>(c, p); return true;
+    case 24: launch_reg<T, 8, 3>(c, p); return true;
+    case 32: launch_reg<T, 4, 8>(c, p); return true;
+    case 36: launch_reg<T, 6, 6>(c, p); return true;
+    case 40: launch_reg<T, 8, 5>(c, p); return true;
+    case 48: launch_reg<T, 8, 6>(c, p); return true;
+    case 64: launch_reg<T, 8, 8>(c, p); return true;
+    case 80: launch_reg<T, 16, 5>(c, p); return true;
+    case 96: launch_reg<T, 16, 6>(c, p); return true;
+    case 128: launch_reg<T, 8, 16>(c, p); return true;
+    case 256: launch_reg<T, 16, 16>(c, p); return true;
+    default: return false;
+  }
+}
+
 template <typename T>
 void launch_pass(adscf_ctx* c, const PassArgs<T>& p) {
+  if (c->use_reg && launch_pass_reg<T>(c, p)) return;
   int items;
   if (p.axis == 0) items = (int)(((long)c->ny * c->nz + p.L - 1) / p.L);
   else if (p.axis == 1) items = c->nz * p.gx;
@@ -1477,6 +1541,24 @@ int adscf_scf_energy(adscf_ctx* c, double* F, double* Fc, double* qsum, double* 
     if (qsum) *qsum = st.qsum;
     if (mu) *mu = st.mu;
     if (rho) *rho = rho_v;
+    return ADSCF_OK;
+  });
+}
+
+int adscf_scf_residuals(adscf_ctx* c, double* out, int cap, int* n) {
+  return guarded(c, [&]() -> int {
+    if (!c->field_set) {
+      c->err = "adscf_scf_set_field has not been called";
+      return ADSCF_ERR_STATE;
+    }
+    if (!out || !n || cap < 0) throw ArgError{"bad residual buffer"};
+    ScfState st{};
+    HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    const int avail = std::min<int>(st.iter, (int)c->resid.n);
+    const int k = std::min(avail, cap);
+    if (k > 0) HIPCK(hipMemcpy(out, c->resid.p, k * sizeof(double), hipMemcpyDeviceToHost));
+    *n = avail;
     return ADSCF_OK;
   });
 }

@@ -4,8 +4,11 @@
 // tensor product Lx Ly Lz of three 1-D circulant operators because
 // exp(-ds k^2) factorises over the axes.  A step is therefore applied as
 // E * L_a L_b L_c * E with one "line pass" per axis; each pass loads a tile of
-// lines into LDS, runs a forward Stockham FFT, multiplies by K_axis/n, runs the
-// inverse FFT and writes the lines back.  Steps alternate the axis order
+// lines, runs a forward FFT, multiplies by K_axis/n, runs the inverse FFT and
+// writes the lines back.  Lengths with a four-step shape (fft_reg.hpp) keep
+// each line in the registers of a few threads of one wave and exchange through
+// LDS twice per operator; other lengths use LDS-staged mixed-radix Stockham
+// passes (fft_line.hpp).  Steps alternate the axis order
 // (x,y,z then z,y,x), so the last pass of a step and the first pass of the
 // next act on the same axis and merge into one pass that also multiplies by
 // E, stores q(s) and multiplies by the next step's E.  A contour step thus
@@ -19,6 +22,7 @@
 #include <cstdint>
 
 #include "fft_line.hpp"
+#include "fft_reg.hpp"
 
 namespace adscf {
 
@@ -193,6 +197,105 @@ __global__ void __launch_bounds__(kThreads) k_line_pass(PassArgs<T> a) {
       C2<T> v = r[l * a.ldn + j];
       a.h1[g] = v.x * a.Ea1[g] * sc1;
       if (a.h2) a.h2[g] = v.y * a.Ea2[g] * sc2;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// register-resident line pass for n = N1 * N2 (fft_reg.hpp)
+// ---------------------------------------------------------------------------
+template <typename T, int N1, int N2>
+__device__ __forceinline__ void reg_line_op(C2<T> (&v)[N2], C2<T>* lds, int t, const PassArgs<T>& a) {
+  if constexpr (N1 == 1) {
+    line_op_single<T, N2>(v, a.kf);
+  } else {
+    if (t < N1) fs_phase_a<T, N1, N2>(v, lds, t, a.tw);
+    __syncthreads();
+    if (t < N2) fs_phase_b<T, N1, N2>(lds, t, a.tw, a.kf);
+    __syncthreads();
+    if (t < N1) fs_phase_c<T, N1, N2>(v, lds, t);
+  }
+}
+
+template <int N1, int N2>
+constexpr int reg_tpl() {
+  return N1 == 1 ? 1 : FourStep<N1, N2>::tpl;
+}
+
+// Same contract as k_line_pass; a line lives in registers of reg_tpl threads,
+// each holding the N2 points j1 + N1*j2 of its j1.  blockDim = reg_tpl * L.
+template <typename T, int N1, int N2>
+__global__ void __launch_bounds__(kThreads) k_line_pass_reg(PassArgs<T> a) {
+  using FS = FourStep<N1, N2>;
+  constexpr int TPL = reg_tpl<N1, N2>();
+  if (a.done && *a.done) return;
+  extern __shared__ unsigned char smem_raw[];
+  const int tid = threadIdx.x;
+  int l, t;
+  if (a.axis == 0) {
+    t = tid % TPL;
+    l = tid / TPL;
+  } else {
+    l = tid % a.L;
+    t = tid / a.L;
+  }
+  bool valid;
+  const long base = pass_line_base(a, blockIdx.x, l, valid);
+  C2<T>* lds = reinterpret_cast<C2<T>*>(smem_raw) + l * FS::lds_elems;
+  const bool act = t < N1 && valid;
+  C2<T> v[N2];
+#pragma unroll
+  for (int j2 = 0; j2 < N2; ++j2) {
+    C2<T> x = {T(0), T(0)};
+    if (act) {
+      const long g = base + (long)(t + N1 * j2) * a.stride;
+      if (a.in_kind == IN_TMP) {
+        x = a.tmp_in[g];
+      } else if (a.in_kind == IN_ONES) {
+        x = {T(1), a.h2 ? T(1) : T(0)};
+      } else {
+        x.x = a.q1_in[g];
+        x.y = a.q2_in ? a.q2_in[g] : T(0);
+      }
+      if (a.pre_E) {
+        x.x *= a.Ea1[g];
+        x.y *= a.Ea2[g];
+      }
+    }
+    v[j2] = x;
+  }
+  reg_line_op<T, N1, N2>(v, lds, t, a);
+  if (a.nops == 2) {
+    const T sc1 = exp2i<T>(a.dexp1[a.sidx]);
+    const T sc2 = exp2i<T>(a.dexp2[a.sidx]);
+    if (act) {
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) {
+        const long g = base + (long)(t + N1 * j2) * a.stride;
+        C2<T> x = v[j2];
+        x.x = x.x * a.Ea1[g] * sc1;
+        x.y = x.y * a.Ea2[g] * sc2;
+        a.h1[g] = x.x;
+        a.h2[g] = x.y;
+        x.x *= a.Eb1[g];
+        x.y *= a.Eb2[g];
+        v[j2] = x;
+      }
+    }
+    reg_line_op<T, N1, N2>(v, lds, t, a);
+  }
+  if (!act) return;
+  if (a.out_kind == OUT_TMP) {
+#pragma unroll
+    for (int j2 = 0; j2 < N2; ++j2) a.tmp_out[base + (long)(t + N1 * j2) * a.stride] = v[j2];
+  } else {
+    const T sc1 = a.dexp1 ? exp2i<T>(a.dexp1[a.sidx]) : T(1);
+    const T sc2 = a.dexp2 ? exp2i<T>(a.dexp2[a.sidx]) : T(1);
+#pragma unroll
+    for (int j2 = 0; j2 < N2; ++j2) {
+      const long g = base + (long)(t + N1 * j2) * a.stride;
+      a.h1[g] = v[j2].x * a.Ea1[g] * sc1;
+      if (a.h2) a.h2[g] = v[j2].y * a.Ea2[g] * sc2;
     }
   }
 }

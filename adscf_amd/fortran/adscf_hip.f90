@@ -1,0 +1,188 @@
+!
+! ISO_C_BINDING interface to libadscf_hip.so (include/adscf.h) and the one
+! context the replacement modules share.  The context is created by
+! Initialize_Fourier and destroyed by End_Fourier, matching the lifetime of the
+! FFTW plans it replaces (Geometry_Modules/Bulk/mod_fourier.f90:49-67).
+!
+module adscf_hip
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: ADSCF_OK = 0, ADSCF_FP64 = 0, ADSCF_FP32 = 1
+  type(c_ptr), save :: adscf_context = c_null_ptr
+
+  interface
+    integer(c_int) function adscf_init(ctx, nx, ny, nz, nt, lx, ly, lz, ds, prec) bind(C, name='adscf_init')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr) :: ctx
+      integer(c_int), value :: nx, ny, nz, nt, prec
+      real(c_double), value :: lx, ly, lz, ds
+    end function
+    integer(c_int) function adscf_finalize(ctx) bind(C, name='adscf_finalize')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function adscf_set_box(ctx, lx, ly, lz) bind(C, name='adscf_set_box')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: lx, ly, lz
+    end function
+    integer(c_int) function adscf_set_ds(ctx, ds) bind(C, name='adscf_set_ds')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: ds
+    end function
+    type(c_ptr) function adscf_last_error(ctx) bind(C, name='adscf_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function adscf_real2fourier(ctx, prop, pf) bind(C, name='adscf_real2fourier')
+      import :: c_ptr, c_int, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: prop(*)
+      complex(c_double_complex), intent(out) :: pf(*)
+    end function
+    integer(c_int) function adscf_fourier2real(ctx, pf, prop) bind(C, name='adscf_fourier2real')
+      import :: c_ptr, c_int, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: pf(*)
+      real(c_double), intent(out) :: prop(*)
+    end function
+    integer(c_int) function adscf_laplace(ctx, lap, elap, nab) bind(C, name='adscf_laplace')
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(out) :: lap(*), elap(*), nab(*)
+    end function
+    integer(c_int) function adscf_expfield(ctx, field, expfield) bind(C, name='adscf_expfield')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: field(*)
+      real(c_double), intent(out) :: expfield(*)
+    end function
+    integer(c_int) function adscf_propagate_step(ctx, prop, e) bind(C, name='adscf_propagate_step')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: prop(*)
+      real(c_double), intent(in) :: e(*)
+    end function
+    integer(c_int) function adscf_propagate_chain(ctx, nblocks, monomer_type, block_segments, ensemble, rho, mu, &
+        expfield, distribution, density, qsum, energy) bind(C, name='adscf_propagate_chain')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nblocks, ensemble
+      integer(c_int), intent(in) :: monomer_type(*), block_segments(*)
+      real(c_double), intent(inout) :: rho, mu
+      real(c_double), intent(in) :: expfield(*)
+      real(c_double), intent(out) :: distribution(*), density(*)
+      real(c_double), intent(out) :: qsum, energy
+    end function
+    integer(c_int) function adscf_set_interactions(ctx, kappa, chi, vol) bind(C, name='adscf_set_interactions')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: kappa
+      real(c_double), intent(in) :: chi(*), vol(*)
+    end function
+    integer(c_int) function adscf_mean_fields(ctx, density, field) bind(C, name='adscf_mean_fields')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: density(*)
+      real(c_double), intent(out) :: field(*)
+    end function
+    integer(c_int) function adscf_interaction_energy(ctx, density, energy) bind(C, name='adscf_interaction_energy')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: density(*)
+      real(c_double), intent(out) :: energy
+    end function
+    integer(c_int) function adscf_average_volume_fraction(ctx, density, average) &
+        bind(C, name='adscf_average_volume_fraction')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: density(*)
+      real(c_double), intent(out) :: average
+    end function
+    integer(c_int) function adscf_mixing_simple(ctx, field, dfield, fieldsize, m, lambda) &
+        bind(C, name='adscf_mixing_simple')
+      import :: c_ptr, c_int, c_long, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: field(*)
+      real(c_double), intent(in) :: dfield(*)
+      integer(c_long), value :: fieldsize
+      integer(c_int), value :: m
+      real(c_double), value :: lambda
+    end function
+    integer(c_int) function adscf_mixing_lambda(ctx, field, dfield, fieldsize, m, lammax, iter) &
+        bind(C, name='adscf_mixing_lambda')
+      import :: c_ptr, c_int, c_long, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: field(*), dfield(*)
+      integer(c_long), value :: fieldsize
+      integer(c_int), value :: m, iter
+      real(c_double), value :: lammax
+    end function
+    integer(c_int) function adscf_mixing_anderson(ctx, field, dfield, fieldsize, m, lam_mix, iter) &
+        bind(C, name='adscf_mixing_anderson')
+      import :: c_ptr, c_int, c_long, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: field(*), dfield(*)
+      integer(c_long), value :: fieldsize
+      integer(c_int), value :: m, iter
+      real(c_double), value :: lam_mix
+    end function
+    integer(c_int) function adscf_ludcmp(a, n, indx, d) bind(C, name='adscf_ludcmp')
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: a(*)
+      integer(c_int), value :: n
+      integer(c_int), intent(out) :: indx(*)
+      real(c_double), intent(out) :: d
+    end function
+    integer(c_int) function adscf_lubksb(a, n, indx, b) bind(C, name='adscf_lubksb')
+      import :: c_int, c_double
+      real(c_double), intent(in) :: a(*)
+      integer(c_int), value :: n
+      integer(c_int), intent(in) :: indx(*)
+      real(c_double), intent(inout) :: b(*)
+    end function
+    integer(c_int) function adscf_solve_lu(a, b, x, n) bind(C, name='adscf_solve_lu')
+      import :: c_int, c_double
+      real(c_double), intent(in) :: a(*), b(*)
+      real(c_double), intent(out) :: x(*)
+      integer(c_int), value :: n
+    end function
+    integer(c_int) function adscf_solve_cholesky(a, b, x, n) bind(C, name='adscf_solve_cholesky')
+      import :: c_int, c_double
+      real(c_double), intent(in) :: a(*), b(*)
+      real(c_double), intent(out) :: x(*)
+      integer(c_int), value :: n
+    end function
+    integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
+      import :: c_ptr, c_size_t
+      type(c_ptr), value :: s
+    end function
+  end interface
+
+contains
+
+  ! The reference reports errors with print* and stop; so does the binding.
+  subroutine adscf_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    type(c_ptr) :: p
+    integer :: n, i
+    character(len=512) :: text
+    if (rc == ADSCF_OK) return
+    text = ''
+    p = adscf_last_error(adscf_context)
+    if (c_associated(p)) then
+      n = int(min(c_strlen(p), 512_c_size_t))
+      call c_f_pointer(p, msg, [n])
+      do i = 1, n
+        text(i:i) = msg(i)
+      end do
+    end if
+    print*, 'adscf error in ', where, ' (status', rc, '): ', trim(text)
+    stop 1
+  end subroutine adscf_check
+
+end module adscf_hip

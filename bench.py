@@ -71,8 +71,17 @@ def converge(adscf_amd, fields, w0, max_iter):
             ctx.scf_set_field(w0)
             t0 = time.perf_counter()
             if prec == adscf_amd.FP32:
-                st = ctx.scf_iterate(max_iter, tol=1e-5)
-                it32 = st.iters
+                # FP32 phase until max|dfield| < 1e-5 or it stops improving (FP32 rounding of
+                # phi, amplified by kappa, sets a residual floor), then FP64 refinement
+                best, it32 = float("inf"), 0
+                while it32 < max_iter:
+                    st = ctx.scf_iterate(50, tol=1e-5)
+                    it32 = st.iters
+                    r = ctx.scf_residuals()[-50:]
+                    if st.converged or r.min() > 0.7 * best:
+                        break
+                    best = min(best, r.min())
+                res["fp32_residual_at_switch"] = float(st.accuracy)
                 ctx.set_precision(adscf_amd.FP64)
             st = ctx.scf_iterate(max_iter, tol=1e-8)
             t = time.perf_counter() - t0

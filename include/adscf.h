@@ -135,6 +135,9 @@ int adscf_scf_get(adscf_ctx* ctx, double* field, double* density);
 /* Calculate_Energy (main_multiblock.f90:548-557) for the last evaluation: total F, F per chain
  * (F / (rho*V)), Qsum, mu, rho.  Any pointer may be NULL. */
 int adscf_scf_energy(adscf_ctx* ctx, double* F, double* F_per_chain, double* qsum, double* mu, double* rho);
+/* max|dfield| of every iteration evaluated since adscf_scf_set_field (up to cap values);
+ * n receives the number available */
+int adscf_scf_residuals(adscf_ctx* ctx, double* out, int cap, int* n);
 /* mean device time per SCF iteration of the last adscf_scf_iterate call, and of the
  * propagator sweep alone, in milliseconds (HIP events on the context stream) */
 int adscf_scf_timing(adscf_ctx* ctx, double* ms_per_iter, double* ms_sweep);
