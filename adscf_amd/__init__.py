@@ -77,6 +77,11 @@ SIGNATURES = {
     "adscf_scf_energy": (C.c_int, [_ctxp, _dp, _dp, _dp, _dp, _dp]),
     "adscf_scf_residuals": (C.c_int, [_ctxp, _dp, C.c_int, _ip]),
     "adscf_scf_timing": (C.c_int, [_ctxp, _dp, _dp]),
+    "adscf_autotune": (C.c_int, [_ctxp, C.c_int, C.c_char_p, C.c_int]),
+    "adscf_get_schedule": (C.c_int, [_ctxp, C.c_int, _ip, _ip, _ip, _ip]),
+    "adscf_set_schedule": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "adscf_adjoint": (C.c_int, [_ctxp, C.c_int, _ip, _ip, C.c_double, _dp, C.c_int, _dp, C.c_double, C.c_int,
+                                _dp, _dp, _dp, _dp, _dp]),
 }
 
 
@@ -342,6 +347,40 @@ class Context:
         a, b = C.c_double(), C.c_double()
         self._chk(self._lib.adscf_scf_timing(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+
+    # -- schedules ------------------------------------------------------------
+    def autotune(self, reps=3):
+        buf = C.create_string_buffer(1 << 16)
+        self._chk(self._lib.adscf_autotune(self._h, int(reps), buf, len(buf)))
+        return buf.value.decode()
+
+    def schedule(self, axis):
+        n, a, b, w = (C.c_int() for _ in range(4))
+        self._chk(self._lib.adscf_get_schedule(self._h, int(axis), C.byref(n), C.byref(a), C.byref(b), C.byref(w)))
+        return dict(n=n.value, N1=a.value, N2=b.value, waves=w.value)
+
+    def set_schedule(self, axis, n1, n2, waves=1):
+        self._chk(self._lib.adscf_set_schedule(self._h, int(axis), int(n1), int(n2), int(waves)))
+
+    # -- adjoint ----------------------------------------------------------------
+    def adjoint(self, block_types, block_segments, field, phi_target, lam=0.1, target_type=1, rho=1.0,
+                checkpoint_every=0, gradients=True):
+        """Loss L = sum (phi_target_type - phi_target)^2 + lam*F of one canonical evaluation and,
+        with gradients=True, dL/dw (ntypes, nz, ny, nx) and dL/d(lx, ly, lz).
+        Returns dict(loss, F, phi, grad_w, grad_box)."""
+        bt = np.ascontiguousarray(block_types, dtype=np.int32)
+        bs = np.ascontiguousarray(block_segments, dtype=np.int32)
+        f = _f64(field, (self.ntypes,) + self.shape)
+        pt = _f64(phi_target, self.shape)
+        loss, F = C.c_double(), C.c_double()
+        phi = np.empty((self.ntypes,) + self.shape)
+        gw = np.empty_like(phi) if gradients else None
+        gb = np.empty(3) if gradients else None
+        self._chk(self._lib.adscf_adjoint(self._h, len(bt), _pi(bt), _pi(bs), float(rho), _p(f), int(target_type),
+                                          _p(pt), float(lam), int(checkpoint_every), C.byref(loss), C.byref(F),
+                                          _p(gw), _p(gb), _p(phi)))
+        return dict(loss=loss.value, F=F.value, phi=phi, grad_w=gw, grad_box=gb)
 
 
 # ---- host routines of module Iterate (no GPU needed) -----------------------

@@ -19,6 +19,7 @@
 #include "kernels.hpp"
 #include "lu.hpp"
 #include "scf_kernels.hpp"
+#include "adjoint.hpp"
 
 using namespace adscf;
 
@@ -63,7 +64,10 @@ struct DevBuf {
 struct AxisTables {
   int n = 1;
   LinePlan plan{};
-  int L = 1;
+  int L = 1;      // Stockham tile width
+  int N1 = 0;     // register-resident shape (0: Stockham kernel)
+  int N2 = 0;
+  int waves = 1;  // wavefronts per workgroup of the register kernel
 };
 
 // Per-precision engine: the propagator arrays and FFT tables in T.
@@ -78,6 +82,9 @@ struct Engine {
   DevBuf<T> cq;      // compat single-field propagate
   DevBuf<C2<T>> ch;  // compat half spectrum
   DevBuf<T> cmixE;   // E output of the compat mixing kernel (unused values)
+  // adjoint: contour checkpoints, recompute segments, adjoint states
+  DevBuf<T> ck1, ck2, sa1, sa2, sb1, sb2, mu, nu, y, ones, scr;
+  DevBuf<T> kd[3];   // k_a^2 / n_a multipliers of the D_a line operators
 };
 
 }  // namespace
@@ -112,7 +119,6 @@ struct adscf_ctx {
   int graph_prec = -1;
   std::vector<const void*> graph_key;  // buffers baked into the captured iteration
   double last_ms_iter = 0, last_ms_sweep = 0;
-  bool use_reg = true;  // register-resident line kernels where a four-step shape exists
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -238,32 +244,32 @@ PassArgs<T> base_pass(adscf_ctx* c, int axis) {
   p.tw = E.tw[axis].p;
   p.kf = E.kf[axis].p;
   p.nops = 1;
+  p.post_E = 1;
   return p;
 }
 
-// Four-step shapes with register-resident kernels; other lengths use the
-// generic Stockham kernel.
-bool reg_shape(int n, int& N1, int& N2) {
-  static const int tab[][3] = {{2, 1, 2},   {3, 1, 3},   {4, 1, 4},   {5, 1, 5},   {6, 1, 6},   {8, 1, 8},
-                               {16, 1, 16}, {10, 2, 5},  {12, 4, 3},  {20, 4, 5},  {24, 8, 3},  {32, 4, 8},
-                               {36, 6, 6},  {40, 8, 5},  {48, 8, 6},  {64, 8, 8},  {80, 16, 5}, {96, 16, 6},
-                               {128, 8, 16}, {256, 16, 16}};
-  for (const auto& r : tab)
-    if (r[0] == n) {
-      N1 = r[1];
-      N2 = r[2];
-      return true;
-    }
-  return false;
-}
+// Register-resident schedules: (n, N1, N2).  Several shapes per length are
+// compiled; the autotuner picks one per axis (and the waves per workgroup).
+template <typename T>
+using RegLauncher = int (*)(adscf_ctx*, PassArgs<T>, int);
+
+struct RegShape {
+  int n, N1, N2;
+};
+
+const RegShape kRegShapes[] = {
+    {2, 1, 2},    {3, 1, 3},   {4, 1, 4},   {5, 1, 5},    {6, 1, 6},    {8, 1, 8},   {16, 1, 16},  {10, 2, 5},
+    {12, 4, 3},   {20, 4, 5},  {24, 8, 3},  {24, 4, 6},   {32, 4, 8},   {32, 8, 4},  {32, 2, 16},  {36, 6, 6},
+    {40, 8, 5},   {48, 8, 6},  {48, 6, 8},  {48, 16, 3},  {64, 8, 8},   {64, 4, 16}, {64, 16, 4},  {80, 16, 5},
+    {96, 16, 6},  {128, 8, 16}, {128, 16, 8}, {256, 16, 16}};
 
 int reg_tpl_rt(int N1, int N2) { return N1 == 1 ? 1 : std::max(N1, N2); }
 
 template <typename T, int N1, int N2>
-void launch_reg(adscf_ctx* c, PassArgs<T> p) {
+int launch_reg(adscf_ctx* c, PassArgs<T> p, int waves) {
   constexpr int TPL = reg_tpl<N1, N2>();
-  // one wave per block by default; the tile width L sets the lines per block
-  int L = std::max(1, 64 / TPL);
+  // `waves` wavefronts per block; the tile width L sets the lines per block
+  int L = std::max(1, 64 * waves / TPL);
   if (p.axis != 0) L = std::min(L, c->nx);
   p.L = L;
   p.gx = (c->nx + L - 1) / L;
@@ -274,38 +280,50 @@ void launch_reg(adscf_ctx* c, PassArgs<T> p) {
   const size_t smem = N1 == 1 ? 0 : (size_t)L * FourStep<N1, N2>::lds_elems * sizeof(C2<T>);
   hipLaunchKernelGGL((k_line_pass_reg<T, N1, N2>), dim3(items), dim3(TPL * L), smem, c->stream, p);
   HIPCK(hipGetLastError());
+  return items;
 }
 
 template <typename T>
-bool launch_pass_reg(adscf_ctx* c, const PassArgs<T>& p) {
-  switch (p.plan.n) {
-    case 2: launch_reg<T, 1, 2>(c, p); return true;
-    case 3: launch_reg<T, 1, 3>(c, p); return true;
-    case 4: launch_reg<T, 1, 4>(c, p); return true;
-    case 5: launch_reg<T, 1, 5>(c, p); return true;
-    case 6: launch_reg<T, 1, 6>(c, p); return true;
-    case 8: launch_reg<T, 1, 8>(c, p); return true;
-    case 16: launch_reg<T, 1, 16>(c, p); return true;
-    case 10: launch_reg<T, 2, 5>(c, p); return true;
-    case 12: launch_reg<T, 4, 3>(c, p); return true;
-    case 20: launch_reg<T, 4, 5>(c, p); return true;
-    case 24: launch_reg<T, 8, 3>(c, p); return true;
-    case 32: launch_reg<T, 4, 8>(c, p); return true;
-    case 36: launch_reg<T, 6, 6>(c, p); return true;
-    case 40: launch_reg<T, 8, 5>(c, p); return true;
-    case 48: launch_reg<T, 8, 6>(c, p); return true;
-    case 64: launch_reg<T, 8, 8>(c, p); return true;
-    case 80: launch_reg<T, 16, 5>(c, p); return true;
-    case 96: launch_reg<T, 16, 6>(c, p); return true;
-    case 128: launch_reg<T, 8, 16>(c, p); return true;
-    case 256: launch_reg<T, 16, 16>(c, p); return true;
-    default: return false;
+RegLauncher<T> reg_launcher(int n, int N1, int N2) {
+#define ADSCF_SHAPE(a, b, cc) \
+  if (n == a && N1 == b && N2 == cc) return &launch_reg<T, b, cc>;
+  ADSCF_SHAPE(2, 1, 2) ADSCF_SHAPE(3, 1, 3) ADSCF_SHAPE(4, 1, 4) ADSCF_SHAPE(5, 1, 5) ADSCF_SHAPE(6, 1, 6)
+  ADSCF_SHAPE(8, 1, 8) ADSCF_SHAPE(16, 1, 16) ADSCF_SHAPE(10, 2, 5) ADSCF_SHAPE(12, 4, 3) ADSCF_SHAPE(20, 4, 5)
+  ADSCF_SHAPE(24, 8, 3) ADSCF_SHAPE(24, 4, 6) ADSCF_SHAPE(32, 4, 8) ADSCF_SHAPE(32, 8, 4) ADSCF_SHAPE(32, 2, 16)
+  ADSCF_SHAPE(36, 6, 6) ADSCF_SHAPE(40, 8, 5) ADSCF_SHAPE(48, 8, 6) ADSCF_SHAPE(48, 6, 8) ADSCF_SHAPE(48, 16, 3)
+  ADSCF_SHAPE(64, 8, 8) ADSCF_SHAPE(64, 4, 16) ADSCF_SHAPE(64, 16, 4) ADSCF_SHAPE(80, 16, 5)
+  ADSCF_SHAPE(96, 16, 6) ADSCF_SHAPE(128, 8, 16) ADSCF_SHAPE(128, 16, 8) ADSCF_SHAPE(256, 16, 16)
+#undef ADSCF_SHAPE
+  return nullptr;
+}
+
+// default schedule for an axis of n points: the first registered shape, one wave per block
+void default_schedule(AxisTables& ax) {
+  ax.N1 = ax.N2 = 0;
+  ax.waves = 1;
+  for (const RegShape& s : kRegShapes)
+    if (s.n == ax.n) {
+      ax.N1 = s.N1;
+      ax.N2 = s.N2;
+      return;
+    }
+}
+
+template <typename T>
+int launch_pass_reg(adscf_ctx* c, const PassArgs<T>& p) {
+  const AxisTables& ax = c->ax[p.axis];
+  if (ax.N1 == 0) return 0;
+  RegLauncher<T> f = reg_launcher<T>(ax.n, ax.N1, ax.N2);
+  return f ? f(c, p, ax.waves) : 0;
+}
+
+// launches one line pass; returns the number of workgroups (OUT_DOT writes one partial each)
+template <typename T>
+int launch_pass(adscf_ctx* c, const PassArgs<T>& p) {
+  {
+    const int items = launch_pass_reg<T>(c, p);
+    if (items > 0) return items;
   }
-}
-
-template <typename T>
-void launch_pass(adscf_ctx* c, const PassArgs<T>& p) {
-  if (c->use_reg && launch_pass_reg<T>(c, p)) return;
   int items;
   if (p.axis == 0) items = (int)(((long)c->ny * c->nz + p.L - 1) / p.L);
   else if (p.axis == 1) items = c->nz * p.gx;
@@ -313,6 +331,7 @@ void launch_pass(adscf_ctx* c, const PassArgs<T>& p) {
   const size_t smem = 2 * (size_t)p.L * p.ldn * sizeof(C2<T>);
   hipLaunchKernelGGL(k_line_pass<T>, dim3(items), dim3(kThreads), smem, c->stream, p);
   HIPCK(hipGetLastError());
+  return items;
 }
 
 std::vector<int> active_axes(const adscf_ctx* c) {
@@ -935,6 +954,54 @@ double sweep_timing(adscf_ctx* c, int reps) {
   return ms / reps;
 }
 
+
+// Autotuner: per active axis, time the captured sweep for every registered
+// register-resident shape of that length and 1, 2 or 4 waves per workgroup, and
+// the LDS Stockham kernel; keep the fastest.  Axes are tuned one after another.
+template <typename T>
+std::string autotune_impl(adscf_ctx* c, int reps) {
+  std::string rep;
+  char line[256];
+  for (int a : active_axes(c)) {
+    AxisTables& ax = c->ax[a];
+    if (ax.n < 2) continue;
+    AxisTables best = ax;
+    double best_ms = 1e300;
+    std::vector<std::pair<int, int>> shapes = {{0, 0}};
+    for (const RegShape& s : kRegShapes)
+      if (s.n == ax.n) shapes.push_back({s.N1, s.N2});
+    for (auto sh : shapes) {
+      for (int waves : {1, 2, 4}) {
+        if (sh.first == 0 && waves > 1) break;
+        ax.N1 = sh.first;
+        ax.N2 = sh.second;
+        ax.waves = waves;
+        const double ms = sweep_timing<T>(c, reps);
+        snprintf(line, sizeof line, "axis %d n=%d shape=%s%dx%d waves=%d sweep=%.4f ms\n", a, ax.n,
+                 sh.first ? "" : "stockham:", sh.first, sh.second, waves, ms);
+        rep += line;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = ax;
+        }
+      }
+    }
+    ax = best;
+    snprintf(line, sizeof line, "axis %d chosen: %dx%d waves=%d (%.4f ms)\n", a, best.N1, best.N2, best.waves,
+             best_ms);
+    rep += line;
+  }
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  if (c->graph_src) {
+    (void)hipGraphDestroy(c->graph_src);
+    c->graph_src = nullptr;
+  }
+  return rep;
+}
+
 void reset_axes(adscf_ctx* c) {
   const int d[3] = {c->nx, c->ny, c->nz};
   for (int a = 0; a < 3; ++a) {
@@ -942,6 +1009,7 @@ void reset_axes(adscf_ctx* c) {
     c->ax[a].plan = make_line_plan(d[a]);
     if (c->ax[a].plan.npass < 0) throw ArgError{"grid dimension has more than 12 prime factors"};
     c->ax[a].L = choose_tile(a, d[a], c->nx, c->ny, c->nz);
+    default_schedule(c->ax[a]);
   }
 }
 
@@ -967,6 +1035,321 @@ double interaction_from_parts(const adscf_ctx* c, const std::vector<double>& p) 
   return e;
 }
 
+
+// ---------------------------------------------------------------------------
+// adjoint of one density evaluation (canonical ensemble)
+// ---------------------------------------------------------------------------
+__global__ void k_acc_partials(const double* __restrict__ part, int n, double* __restrict__ acc) {
+  if (threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += part[i];
+  *acc += s;
+}
+
+struct PosWeights {
+  double w1 = 0.0, w2 = 0.0;
+  int b1 = -1, b2 = -1;  // blocks whose trapezoid covers contour point s (two at a junction)
+};
+
+// trapezoid weights of contour point s (mod_multiblock.f90:157-168)
+PosWeights pos_weights(const ChainDesc& cd, double ds, int s) {
+  PosWeights pw;
+  for (int b = 0; b < cd.nblocks; ++b) {
+    const int s0 = cd.start[b], s1 = cd.start[b] + cd.seg[b];
+    if (s < s0 || s > s1) continue;
+    const double w = (s == s0 || s == s1) ? 0.5 * ds : ds;
+    if (pw.b1 < 0) {
+      pw.b1 = b;
+      pw.w1 = w;
+    } else {
+      pw.b2 = b;
+      pw.w2 = w;
+    }
+  }
+  return pw;
+}
+
+Seed seed_of(const ChainDesc& cd, const PosWeights& pw) {
+  Seed sd{};
+  sd.w1 = pw.w1;
+  sd.t1 = cd.type[pw.b1];
+  sd.w2 = pw.w2;
+  sd.t2 = pw.b2 >= 0 ? cd.type[pw.b2] : -1;
+  return sd;
+}
+
+template <typename T>
+void upload_dtables(adscf_ctx* c) {
+  Engine<T>& E = eng<T>(c);
+  const int dims[3] = {c->nx, c->ny, c->nz};
+  const double lens[3] = {c->lx, c->ly, c->lz};
+  for (int a = 0; a < 3; ++a) {
+    const int n = dims[a];
+    std::vector<T> kd(n);
+    for (int t = 0; t < n; ++t) {
+      const double q = wavenumber(t, n, lens[a]);
+      kd[t] = (T)(q * q / (double)n);
+    }
+    E.kd[a].ensure(n);
+    HIPCK(hipMemcpyAsync(E.kd[a].p, kd.data(), sizeof(T) * n, hipMemcpyHostToDevice, c->stream));
+  }
+}
+
+template <typename T>
+struct AdjointRun {
+  adscf_ctx* c;
+  const ChainDesc& cd;
+  Engine<T>& En;
+  long ng;
+  std::vector<int> cks;
+
+  int ck_slot(int s) const {
+    for (size_t i = 0; i < cks.size(); ++i)
+      if (cks[i] == s) return (int)i;
+    return -1;
+  }
+  int ck_floor(int s) const {
+    int best = 0;
+    for (int v : cks)
+      if (v <= s) best = v;
+    return best;
+  }
+  const T* Eof(int chain, int s) const { return En.E.p + (long)step_type(cd, chain, s) * ng; }
+
+  // one packed step: chain 1 at forward step s1, chain 2 at complementary step s2
+  void step(const T* in1, const T* in2, int s1, int s2, T* out1, T* out2, bool post_E) {
+    const std::vector<int> A = active_axes(c);
+    for (size_t i = 0; i < A.size(); ++i) {
+      PassArgs<T> p = base_pass<T>(c, A[i]);
+      p.Ea1 = Eof(0, s1);
+      p.Ea2 = Eof(1, s2);
+      if (i == 0) {
+        p.in_kind = IN_PLANAR;
+        p.q1_in = in1;
+        p.q2_in = in2;
+        p.pre_E = 1;
+      } else {
+        p.in_kind = IN_TMP;
+        p.tmp_in = En.tmp.p;
+      }
+      if (i + 1 == A.size()) {
+        p.out_kind = OUT_PLANAR;
+        p.h1 = out1;
+        p.h2 = out2;
+        p.post_E = post_E ? 1 : 0;
+      } else {
+        p.out_kind = OUT_TMP;
+        p.tmp_out = En.tmp.p;
+      }
+      launch_pass<T>(c, p);
+    }
+  }
+
+  // states of chain 1 from checkpoint a1 for n1 steps and chain 2 from a2 for n2 steps
+  void recompute(int a1, int n1, T* b1, int a2, int n2, T* b2) {
+    HIPCK(hipMemcpyAsync(b1, En.ck1.p + (long)ck_slot(a1) * ng, ng * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    HIPCK(hipMemcpyAsync(b2, En.ck2.p + (long)ck_slot(a2) * ng, ng * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    const int nmax = std::max(n1, n2);
+    for (int i = 1; i <= nmax; ++i)
+      step(b1 + (long)(i - 1) * ng, b2 + (long)(i - 1) * ng, a1 + i, a2 + i, b1 + (long)i * ng, b2 + (long)i * ng,
+           true);
+  }
+};
+
+template <typename T>
+int adjoint_impl(adscf_ctx* c, const ChainDesc& cd, double rho, const double* field, int target_type,
+                 const double* phit, double lam, int ckev, double* loss, double* Fout, double* grad_w,
+                 double* grad_box, double* phi_out) {
+  ensure_tables<T>(c);
+  ensure_partials(c);
+  Engine<T>& En = eng<T>(c);
+  const long ng = c->ng;
+  const int nt = c->nt, ns = cd.ns;
+  for (int b = 0; b < cd.nblocks; ++b)
+    if (cd.seg[b] < 1) throw ArgError{"adjoint: every block needs at least one segment"};
+  const int ck = ckev > 0 ? ckev : std::max(1, (int)lround(sqrt((double)ns)));
+  AdjointRun<T> R{c, cd, En, ng, {}};
+  for (int s = 0; s <= ns; s += ck) R.cks.push_back(s);
+  if (R.cks.back() != ns) R.cks.push_back(ns);
+  const size_t nck = R.cks.size();
+  const size_t seg = 2 * (size_t)ck + 2;
+  En.E.ensure((size_t)nt * ng);
+  En.tmp.ensure(ng);
+  En.ck1.ensure(nck * ng);
+  En.ck2.ensure(nck * ng);
+  En.sa1.ensure(seg * ng);
+  En.sa2.ensure(seg * ng);
+  En.sb1.ensure(seg * ng);
+  En.sb2.ensure(seg * ng);
+  En.mu.ensure(2 * ng);
+  En.nu.ensure(2 * ng);
+  En.y.ensure(2 * ng);
+  En.scr.ensure(4 * ng);
+  const int nb = grid_for(ng);
+  // fields, E, target
+  c->io.ensure((size_t)nt * ng);
+  HIPCK(hipMemcpyAsync(c->io.p, field, (size_t)nt * ng * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_expfield<T>, dim3(nb), dim3(kThreads), 0, c->stream, (const double*)c->io.p, En.E.p, ng, nt,
+                     c->ds, (double*)nullptr, (const int*)nullptr);
+  DevBuf<double> phit_d, dist, phi, g, gE;
+  phit_d.ensure(ng);
+  HIPCK(hipMemcpyAsync(phit_d.p, phit, ng * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  // checkpoint slot 0 holds q(0) = p(0) = 1
+  hipLaunchKernelGGL(k_fill<T>, dim3(nb), dim3(kThreads), 0, c->stream, En.ck1.p, T(1), ng);
+  hipLaunchKernelGGL(k_fill<T>, dim3(nb), dim3(kThreads), 0, c->stream, En.ck2.p, T(1), ng);
+  // ---- forward sweep keeping only the checkpoints
+  {
+    const T* cur1 = En.ck1.p;
+    const T* cur2 = En.ck2.p;
+    for (int s = 1; s <= ns; ++s) {
+      const int k = R.ck_slot(s);
+      T* o1 = k >= 0 ? En.ck1.p + (long)k * ng : En.scr.p + (long)(s % 2) * ng;
+      T* o2 = k >= 0 ? En.ck2.p + (long)k * ng : En.scr.p + (long)(2 + s % 2) * ng;
+      R.step(cur1, cur2, s, s, o1, o2, true);
+      cur1 = o1;
+      cur2 = o2;
+    }
+  }
+  // ---- contour integral by segments: q on [a,b], p on [ns-b, ns-a]
+  dist.ensure((size_t)cd.nblocks * ng);
+  hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(kThreads), 0, c->stream, dist.p, 0.0, (long)cd.nblocks * ng);
+  for (size_t j = 0; j + 1 < nck; ++j) {
+    const int a = R.cks[j], b = R.cks[j + 1];
+    const int pb = R.ck_floor(ns - b);
+    R.recompute(a, b - a, En.sa1.p, pb, ns - a - pb, En.sa2.p);
+    const int last = (b == ns) ? b : b - 1;
+    for (int s = a; s <= last; ++s) {
+      const PosWeights pw = pos_weights(cd, c->ds, s);
+      hipLaunchKernelGGL(k_adj_contour<T>, dim3(nb), dim3(kThreads), 0, c->stream,
+                         (const T*)(En.sa1.p + (long)(s - a) * ng), (const T*)(En.sa2.p + (long)(ns - s - pb) * ng), ng,
+                         pw.w1, pw.b1, pw.w2, pw.b2, dist.p);
+    }
+  }
+  HIPCK(hipGetLastError());
+  // ---- S, phi, F, loss and dL/dphi
+  hipLaunchKernelGGL(k_qsum<T>, dim3(nb), dim3(kThreads), 0, c->stream, (const T*)(En.ck1.p + (long)(nck - 1) * ng),
+                     ng, c->qpart.p, (const int*)nullptr);
+  std::vector<double> part(nb);
+  HIPCK(hipMemcpyAsync(part.data(), c->qpart.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  double S = 0.0;
+  for (double v : part) S += v;
+  const double A = rho * (double)ng / S;  // rho V / Q with Q = dvol S
+  phi.ensure((size_t)nt * ng);
+  hipLaunchKernelGGL(k_adj_phi, dim3(nb), dim3(kThreads), 0, c->stream, (const double*)dist.p, ng, cd, nt, A,
+                     target_type, (const double*)phit_d.p, phi.p, c->qpart.p);
+  HIPCK(hipMemcpyAsync(part.data(), c->qpart.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  double misfit = 0.0;
+  for (double v : part) misfit += v;
+  std::vector<double> ep;
+  host_energy_parts(c, phi.p, c->io.p, ep);
+  const double V = c->volume;
+  const double mu = log(rho * V / (c->dvol * S));
+  const double F = rho * V * (mu - 1.0) - c->dvol * ep[0] + interaction_from_parts(c, ep);
+  if (loss) *loss = misfit + lam * F;
+  if (Fout) *Fout = F;
+  if (phi_out) HIPCK(hipMemcpy(phi_out, phi.p, (size_t)nt * ng * sizeof(double), hipMemcpyDeviceToHost));
+  if (!grad_w && !grad_box) return ADSCF_OK;
+  g.ensure((size_t)nt * ng);
+  hipLaunchKernelGGL(k_adj_g, dim3(nb), dim3(kThreads), 0, c->stream, (const double*)phi.p, (const double*)c->io.p, ng,
+                     c->fh, lam, c->dvol, target_type, (const double*)phit_d.p, g.p, c->qpart.p);
+  HIPCK(hipMemcpyAsync(part.data(), c->qpart.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  double gphi = 0.0;
+  for (double v : part) gphi += v;
+  const double dLdS = -gphi / S - lam * rho * V / S;
+  // ---- backward sweep
+  upload_dtables<T>(c);
+  gE.ensure((size_t)nt * ng);
+  hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(kThreads), 0, c->stream, gE.p, 0.0, (long)nt * ng);
+  DevBuf<double> spart, sacc;
+  spart.ensure((size_t)ng + 64);
+  sacc.ensure(3);
+  HIPCK(hipMemsetAsync(sacc.p, 0, 3 * sizeof(double), c->stream));
+  const Seed sN = seed_of(cd, pos_weights(cd, c->ds, ns));
+  const Seed s0 = seed_of(cd, pos_weights(cd, c->ds, 0));
+  hipLaunchKernelGGL(k_adj_init<T>, dim3(nb), dim3(kThreads), 0, c->stream, (const double*)g.p, ng, A, sN, s0, dLdS,
+                     En.mu.p, En.nu.p);
+  int cur = 0;
+  const std::vector<int> axes = active_axes(c);
+  for (size_t j = nck - 1; j >= 1; --j) {
+    const int lo = R.cks[j - 1], hi = R.cks[j];
+    R.recompute(lo, hi - lo, En.sa1.p, lo, hi - lo, En.sa2.p);
+    const int f0 = R.ck_floor(ns - hi + 1);
+    R.recompute(f0, ns - lo - f0, En.sb1.p, f0, ns - lo - f0, En.sb2.p);
+    for (int s = hi; s > lo; --s) {
+      T* mu_s = En.mu.p + (long)cur * ng;
+      T* nu_s = En.nu.p + (long)cur * ng;
+      const T* q_s = En.sa1.p + (long)(s - lo) * ng;
+      const T* p_s = En.sa2.p + (long)(s - lo) * ng;
+      // y = L(E mu(s)), yp = L(E nu(s))
+      R.step(mu_s, nu_s, s, s, En.y.p, En.y.p + ng, false);
+      // box stress: <E mu(s), D_a q(s)/E> per axis
+      if (grad_box) {
+        for (int a : axes) {
+          if (c->ax[a].n < 2) continue;
+          PassArgs<T> p = base_pass<T>(c, a);
+          p.kf = En.kd[a].p;
+          p.in_kind = IN_PLANAR;
+          p.q1_in = q_s;
+          p.q2_in = p_s;
+          p.pre_E = -1;
+          p.Ea1 = R.Eof(0, s);
+          p.Ea2 = R.Eof(1, s);
+          p.out_kind = OUT_DOT;
+          p.d1 = mu_s;
+          p.d2 = nu_s;
+          p.dot_out = spart.p;
+          const int items = launch_pass<T>(c, p);
+          hipLaunchKernelGGL(k_acc_partials, dim3(1), dim3(64), 0, c->stream, (const double*)spart.p, items,
+                             sacc.p + a);
+        }
+      }
+      AdjStepArgs<T> st{};
+      st.ng = ng;
+      st.A = A;
+      st.seed_q = seed_of(cd, pos_weights(cd, c->ds, s - 1));
+      st.seed_p = seed_of(cd, pos_weights(cd, c->ds, ns - s + 1));
+      st.g = g.p;
+      st.Eq = R.Eof(0, s);
+      st.Ep = R.Eof(1, s);
+      st.y = En.y.p;
+      st.yp = En.y.p + ng;
+      st.mu = mu_s;
+      st.nu = nu_s;
+      st.q_s = q_s;
+      st.q_m = En.sa1.p + (long)(s - 1 - lo) * ng;
+      st.p_s = p_s;
+      st.p_m = En.sa2.p + (long)(s - 1 - lo) * ng;
+      st.p_f = En.sb2.p + (long)(ns - s + 1 - f0) * ng;
+      st.q_f = En.sb1.p + (long)(ns - s + 1 - f0) * ng;
+      st.mu_out = En.mu.p + (long)(1 - cur) * ng;
+      st.nu_out = En.nu.p + (long)(1 - cur) * ng;
+      st.gEq = gE.p + (long)step_type(cd, 0, s) * ng;
+      st.gEp = gE.p + (long)step_type(cd, 1, s) * ng;
+      hipLaunchKernelGGL(k_adj_step<T>, dim3(nb), dim3(kThreads), 0, c->stream, st);
+      HIPCK(hipGetLastError());
+      cur = 1 - cur;
+    }
+  }
+  if (grad_w) {
+    DevBuf<double> gw;
+    gw.ensure((size_t)nt * ng);
+    hipLaunchKernelGGL(k_adj_gradw<T>, dim3(nb), dim3(kThreads), 0, c->stream, (const double*)gE.p,
+                       (const T*)En.E.p, (const double*)phi.p, (long)nt * ng, c->ds, lam, c->dvol, gw.p);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(grad_w, gw.p, (size_t)nt * ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+  }
+  if (grad_box) {
+    double sa[3];
+    HIPCK(hipMemcpy(sa, sacc.p, 3 * sizeof(double), hipMemcpyDeviceToHost));
+    const double lens[3] = {c->lx, c->ly, c->lz};
+    for (int a = 0; a < 3; ++a) grad_box[a] = 2.0 * c->ds / lens[a] * sa[a] + lam * F / lens[a];
+  }
+  return ADSCF_OK;
+}
 }  // namespace
 
 // ===========================================================================
@@ -1574,6 +1957,61 @@ int adscf_scf_timing(adscf_ctx* c, double* ms_per_iter, double* ms_sweep) {
       c->last_ms_sweep = c->prec == ADSCF_FP64 ? sweep_timing<double>(c, 5) : sweep_timing<float>(c, 5);
       *ms_sweep = c->last_ms_sweep;
     }
+    return ADSCF_OK;
+  });
+}
+
+int adscf_adjoint(adscf_ctx* c, int nblocks, const int* monomer_type, const int* block_segments, double rho,
+                  const double* field, int target_type, const double* phi_target, double lam, int checkpoint_every,
+                  double* loss, double* free_energy, double* grad_field, double* grad_box, double* phi) {
+  return guarded(c, [&]() -> int {
+    if (!monomer_type || !block_segments || !field || !phi_target) throw ArgError{"null argument"};
+    if (!c->fh_set) throw ArgError{"adscf_set_interactions has not been called"};
+    if (target_type < 1 || target_type > c->nt) throw ArgError{"target_type out of range 1..ntypes"};
+    ChainDesc cd = make_chain(c, nblocks, monomer_type, block_segments, 1);
+    return c->prec == ADSCF_FP64
+               ? adjoint_impl<double>(c, cd, rho, field, target_type - 1, phi_target, lam, checkpoint_every, loss,
+                                      free_energy, grad_field, grad_box, phi)
+               : adjoint_impl<float>(c, cd, rho, field, target_type - 1, phi_target, lam, checkpoint_every, loss,
+                                     free_energy, grad_field, grad_box, phi);
+  });
+}
+
+int adscf_autotune(adscf_ctx* c, int reps, char* report, int report_cap) {
+  return guarded(c, [&]() -> int {
+    if (!c->field_set) {
+      c->err = "adscf_autotune needs adscf_scf_setup and adscf_scf_set_field";
+      return ADSCF_ERR_STATE;
+    }
+    const std::string r = c->prec == ADSCF_FP64 ? autotune_impl<double>(c, std::max(reps, 1))
+                                                : autotune_impl<float>(c, std::max(reps, 1));
+    if (report && report_cap > 0) {
+      const size_t k = std::min(r.size(), (size_t)report_cap - 1);
+      memcpy(report, r.data(), k);
+      report[k] = 0;
+    }
+    return ADSCF_OK;
+  });
+}
+
+int adscf_get_schedule(const adscf_ctx* c, int axis, int* n, int* n1, int* n2, int* waves) {
+  if (!c || axis < 0 || axis > 2) return ADSCF_ERR_ARG;
+  if (n) *n = c->ax[axis].n;
+  if (n1) *n1 = c->ax[axis].N1;
+  if (n2) *n2 = c->ax[axis].N2;
+  if (waves) *waves = c->ax[axis].waves;
+  return ADSCF_OK;
+}
+
+int adscf_set_schedule(adscf_ctx* c, int axis, int n1, int n2, int waves) {
+  return guarded(c, [&]() -> int {
+    if (axis < 0 || axis > 2 || waves < 1 || waves > 4) throw ArgError{"bad schedule"};
+    AxisTables& ax = c->ax[axis];
+    if (n1 != 0 && !reg_launcher<double>(ax.n, n1, n2)) throw ArgError{"no register kernel for this shape"};
+    ax.N1 = n1;
+    ax.N2 = n1 ? n2 : 0;
+    ax.waves = waves;
+    invalidate_tables(c);
     return ADSCF_OK;
   });
 }

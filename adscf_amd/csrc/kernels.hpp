@@ -35,7 +35,7 @@ constexpr int kMaxHist = 40;    // Anderson history depth
 // line pass
 // ---------------------------------------------------------------------------
 enum InKind : int { IN_TMP = 0, IN_ONES = 1, IN_PLANAR = 2 };
-enum OutKind : int { OUT_TMP = 0, OUT_PLANAR = 1 };
+enum OutKind : int { OUT_TMP = 0, OUT_PLANAR = 1, OUT_DOT = 2 };
 
 template <typename T>
 struct PassArgs {
@@ -50,7 +50,8 @@ struct PassArgs {
   const T* kf;      // n multipliers exp(-ds q_k^2) / n
   int nops;         // 1, or 2 for a merged pass (op, mid store, op)
   int in_kind;
-  int pre_E;        // multiply the loaded lines by (Ea1, Ea2)
+  int pre_E;        // +1: multiply the loaded lines by (Ea1, Ea2); -1: divide by them
+  int post_E;       // OUT_PLANAR: multiply by (Ea1, Ea2) and the rescale before storing
   int out_kind;
   const C2<T>* tmp_in;
   const T* q1_in;
@@ -65,8 +66,39 @@ struct PassArgs {
   const int* dexp2;
   int sidx;
   C2<T>* tmp_out;
+  const T* d1;      // OUT_DOT: sum over points of Re*Ea1*d1 + Im*Ea2*d2 -> dot_out[block]
+  const T* d2;
+  double* dot_out;
   const int* done;  // skip all work once the SCF loop has converged
 };
+
+template <typename T>
+__device__ __forceinline__ void apply_pre_E(const PassArgs<T>& a, long g, C2<T>& v) {
+  if (a.pre_E > 0) {
+    v.x *= a.Ea1[g];
+    v.y *= a.Ea2[g];
+  } else if (a.pre_E < 0) {
+    v.x /= a.Ea1[g];
+    v.y /= a.Ea2[g];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ double dot_term(const PassArgs<T>& a, long g, C2<T> v) {
+  return (double)v.x * (double)a.Ea1[g] * (double)a.d1[g] + (double)v.y * (double)a.Ea2[g] * (double)a.d2[g];
+}
+
+__device__ __forceinline__ void block_dot_out(double acc, double* out) {
+  __shared__ double red[kThreads];
+  const int tid = threadIdx.x;
+  red[tid] = acc;
+  __syncthreads();
+  for (int s = 1; s < blockDim.x; s <<= 1) {
+    if ((tid % (2 * s)) == 0 && tid + s < blockDim.x) red[tid] += red[tid + s];
+    __syncthreads();
+  }
+  if (tid == 0) out[blockIdx.x] = red[0];
+}
 
 template <typename T>
 __device__ __forceinline__ long pass_line_base(const PassArgs<T>& a, int item, int l, bool& valid) {
@@ -143,10 +175,7 @@ __global__ void __launch_bounds__(kThreads) k_line_pass(PassArgs<T> a) {
         v.x = a.q1_in[g];
         v.y = a.q2_in ? a.q2_in[g] : T(0);
       }
-      if (a.pre_E) {
-        v.x *= a.Ea1[g];
-        v.y *= a.Ea2[g];
-      }
+      apply_pre_E(a, g, v);
     }
     b0[l * a.ldn + j] = v;
   }
@@ -184,6 +213,16 @@ __global__ void __launch_bounds__(kThreads) k_line_pass(PassArgs<T> a) {
       const long base = pass_line_base(a, item, l, valid);
       if (valid) a.tmp_out[base + (long)j * a.stride] = r[l * a.ldn + j];
     }
+  } else if (a.out_kind == OUT_DOT) {
+    double acc = 0.0;
+    for (int e = tid; e < tot; e += nthr) {
+      int l, j;
+      pass_elem(a, e, n, l, j);
+      bool valid;
+      const long base = pass_line_base(a, item, l, valid);
+      if (valid) acc += dot_term(a, base + (long)j * a.stride, r[l * a.ldn + j]);
+    }
+    block_dot_out(acc, a.dot_out);
   } else {
     const T sc1 = a.dexp1 ? exp2i<T>(a.dexp1[a.sidx]) : T(1);
     const T sc2 = a.dexp2 ? exp2i<T>(a.dexp2[a.sidx]) : T(1);
@@ -195,8 +234,12 @@ __global__ void __launch_bounds__(kThreads) k_line_pass(PassArgs<T> a) {
       if (!valid) continue;
       const long g = base + (long)j * a.stride;
       C2<T> v = r[l * a.ldn + j];
-      a.h1[g] = v.x * a.Ea1[g] * sc1;
-      if (a.h2) a.h2[g] = v.y * a.Ea2[g] * sc2;
+      if (a.post_E) {
+        v.x = v.x * a.Ea1[g] * sc1;
+        v.y = v.y * a.Ea2[g] * sc2;
+      }
+      a.h1[g] = v.x;
+      if (a.h2) a.h2[g] = v.y;
     }
   }
 }
@@ -257,10 +300,7 @@ __global__ void __launch_bounds__(kThreads) k_line_pass_reg(PassArgs<T> a) {
         x.x = a.q1_in[g];
         x.y = a.q2_in ? a.q2_in[g] : T(0);
       }
-      if (a.pre_E) {
-        x.x *= a.Ea1[g];
-        x.y *= a.Ea2[g];
-      }
+      apply_pre_E(a, g, x);
     }
     v[j2] = x;
   }
@@ -284,6 +324,15 @@ __global__ void __launch_bounds__(kThreads) k_line_pass_reg(PassArgs<T> a) {
     }
     reg_line_op<T, N1, N2>(v, lds, t, a);
   }
+  if (a.out_kind == OUT_DOT) {
+    double acc = 0.0;
+    if (act) {
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) acc += dot_term(a, base + (long)(t + N1 * j2) * a.stride, v[j2]);
+    }
+    block_dot_out(acc, a.dot_out);
+    return;
+  }
   if (!act) return;
   if (a.out_kind == OUT_TMP) {
 #pragma unroll
@@ -294,8 +343,13 @@ __global__ void __launch_bounds__(kThreads) k_line_pass_reg(PassArgs<T> a) {
 #pragma unroll
     for (int j2 = 0; j2 < N2; ++j2) {
       const long g = base + (long)(t + N1 * j2) * a.stride;
-      a.h1[g] = v[j2].x * a.Ea1[g] * sc1;
-      if (a.h2) a.h2[g] = v[j2].y * a.Ea2[g] * sc2;
+      C2<T> x = v[j2];
+      if (a.post_E) {
+        x.x = x.x * a.Ea1[g] * sc1;
+        x.y = x.y * a.Ea2[g] * sc2;
+      }
+      a.h1[g] = x.x;
+      if (a.h2) a.h2[g] = x.y;
     }
   }
 }

@@ -92,3 +92,23 @@ def block_segments(fractions, ns):
     """nint(block_length/ds) with ds = 1/ns (Fortran NINT, mod_multiblock.f90:85)."""
     ds = 1.0 / ns
     return [int(np.floor(f / ds + 0.5)) for f in fractions]
+
+
+def smooth_target(n, sigma=2.0, lo=0.1, hi=0.9, seed=2007):
+    """Seeded N(0,1) noise, Gaussian-filtered with a periodic kernel of sigma grid cells,
+    rescaled to [lo, hi]; shape (n, n, n)."""
+    eta = MTStream(seed).normals(n ** 3).reshape(n, n, n)
+    k = np.fft.fftfreq(n) * 2 * np.pi
+    kz, ky, kx = np.meshgrid(k, k, k, indexing="ij")
+    filt = np.exp(-0.5 * sigma ** 2 * (kx ** 2 + ky ** 2 + kz ** 2))
+    sm = np.fft.ifftn(np.fft.fftn(eta) * filt).real
+    sm = (sm - sm.min()) / (sm.max() - sm.min())
+    return lo + (hi - lo) * sm
+
+
+def unit_directions(count, sizes, seed=2007):
+    """count seeded unit vectors over a concatenated parameter vector of the given sizes."""
+    st = MTStream(seed)
+    n = int(sum(sizes))
+    out = st.normals(count * n).reshape(count, n)
+    return out / np.linalg.norm(out, axis=1, keepdims=True)

@@ -142,6 +142,29 @@ int adscf_scf_residuals(adscf_ctx* ctx, double* out, int cap, int* n);
  * propagator sweep alone, in milliseconds (HIP events on the context stream) */
 int adscf_scf_timing(adscf_ctx* ctx, double* ms_per_iter, double* ms_sweep);
 
+/* ---- schedule autotuner (no counterpart in the reference) --------------------- */
+/* Times the captured propagator sweep of the configured SCF problem for every line-FFT
+ * schedule of each axis (register-resident four-step shapes N1 x N2 with 1, 2 or 4
+ * wavefronts per workgroup, and the LDS Stockham kernel) and keeps the fastest.  Needs
+ * adscf_scf_setup + adscf_scf_set_field.  report (may be NULL) receives the timings. */
+int adscf_autotune(adscf_ctx* ctx, int reps, char* report, int report_cap);
+/* current schedule of an axis (0 x, 1 y, 2 z); n1 = 0 means the Stockham kernel */
+int adscf_get_schedule(const adscf_ctx* ctx, int axis, int* n, int* n1, int* n2, int* waves);
+int adscf_set_schedule(adscf_ctx* ctx, int axis, int n1, int n2, int waves);
+
+/* ---- adjoint (no counterpart in the reference) ---------------------------- */
+/* Reverse-mode derivative of one canonical density evaluation of the chain
+ * (mod_multiblock.f90:112-188 with Calculate_Energy, main_multiblock.f90:548-557) for
+ *   L = sum_r (phi_target_type(r) - phi_target(r))^2 + lam * F.
+ * field (nx,ny,nz,ntypes), phi_target (nx,ny,nz), target_type 1-based.  Propagators are
+ * stored only every checkpoint_every contour steps (0: about sqrt(Ns)) and recomputed
+ * by segments.  Outputs: loss, free_energy (F), grad_field = dL/dw (nx,ny,nz,ntypes),
+ * grad_box = dL/d(lx,ly,lz) (3), phi (nx,ny,nz,ntypes); every output may be NULL, and
+ * with both gradients NULL only the forward part runs. */
+int adscf_adjoint(adscf_ctx* ctx, int nblocks, const int* monomer_type, const int* block_segments, double rho,
+                  const double* field, int target_type, const double* phi_target, double lam, int checkpoint_every,
+                  double* loss, double* free_energy, double* grad_field, double* grad_box, double* phi);
+
 #ifdef __cplusplus
 }
 #endif

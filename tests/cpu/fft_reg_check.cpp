@@ -108,5 +108,6 @@ int main() {
   int bad = 0;
   C1(2) C1(3) C1(4) C1(5) C1(6) C1(8) C1(16)
   C4(4, 8) C4(8, 4) C4(8, 6) C4(6, 8) C4(8, 8) C4(8, 16) C4(16, 8) C4(16, 16) C4(2, 3) C4(4, 4) C4(8, 5)
+  C4(2, 5) C4(4, 3) C4(4, 5) C4(8, 3) C4(4, 6) C4(2, 16) C4(6, 6) C4(16, 3) C4(4, 16) C4(16, 4) C4(16, 5) C4(16, 6)
   return bad;
 }

@@ -222,3 +222,40 @@ def test_scf_converges_to_oracle_saddle_point(amd, orc):
     assert abs(en["F"] - orc.total_energy(g, info_o, fh, den_o, f_gpu)) < FP64_RTOL * abs(F_ref)
     # the two final states differ by the convergence tolerance times the density response
     assert np.max(np.abs(den - den_ref)) < 1e-6
+
+
+@pytest.mark.parametrize("n,shape", [(32, (4, 8)), (32, (8, 4)), (32, (2, 16)), (48, (8, 6)), (48, (6, 8)),
+                                     (48, (16, 3)), (64, (8, 8)), (64, (4, 16)), (64, (16, 4)), (24, (8, 3)),
+                                     (24, (4, 6)), (128, (8, 16)), (128, (16, 8)), (16, (1, 16)), (64, (0, 0))])
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_every_schedule_matches_oracle(amd, orc, n, shape, waves):
+    if shape[0] == 0 and waves > 1:
+        pytest.skip("the Stockham kernel has no wave setting")
+    dims = (n, 6, 10) if n >= 64 else (n, n // 2, 12)
+    print(f"schedule n={n} shape={shape} waves={waves}", flush=True)
+    with amd.Context(*dims, 3.0, 2.5, 3.5, 0.01) as ctx:
+        ctx.set_schedule(0, shape[0], shape[1], waves)
+        g = orc.Grid(*dims, 3.0, 2.5, 3.5, 0.01)
+        w = _fields(g.shape, 7, amp=2.0)
+        E = orc.expfield(g, w)
+        ref = orc.propagate_chain(g, [1, 2], [6, 9], E)
+        got = ctx.propagate_chain([1, 2], [6, 9], E)
+        assert _rel(got.density, ref.density) < FP64_RTOL
+        assert abs(got.Qsum - ref.Qsum) < FP64_RTOL * ref.Qsum
+
+
+def test_autotune_keeps_parity(amd, orc):
+    n, L, ds = 32, 4.0, 0.02
+    g = orc.Grid(n, n, n, L, L, L, ds)
+    w = _fields(g.shape, 8, amp=2.0)
+    with amd.Context(n, n, n, L, L, L, ds) as ctx:
+        ctx.set_interactions(25.0, [[0.0, 15.0], [15.0, 0.0]], [1.0, 1.0])
+        ctx.scf_setup([1, 2], [25, 25], mixing_dim=4)
+        ctx.scf_set_field(w)
+        rep = ctx.autotune(reps=2)
+        print(rep)
+        assert "chosen" in rep
+        E = orc.expfield(g, w)
+        got = ctx.propagate_chain([1, 2], [25, 25], E)
+    ref = orc.propagate_chain(g, [1, 2], [25, 25], E)
+    assert _rel(got.density, ref.density) < FP64_RTOL
