@@ -80,6 +80,17 @@ SIGNATURES = {
     "adscf_autotune": (C.c_int, [_ctxp, C.c_int, C.c_char_p, C.c_int]),
     "adscf_get_schedule": (C.c_int, [_ctxp, C.c_int, _ip, _ip, _ip, _ip]),
     "adscf_set_schedule": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "adscf_batch_create": (C.c_int, [C.POINTER(_ctxp), C.c_int, _ip, _dp, _dp, _ip, _ip, _ip, _dp, _dp, _dp,
+                                     C.c_int, C.c_double, C.c_int, C.c_double, C.c_int]),
+    "adscf_batch_destroy": (C.c_int, [_ctxp]),
+    "adscf_batch_last_error": (C.c_char_p, [_ctxp]),
+    "adscf_batch_set_fields": (C.c_int, [_ctxp, _dp]),
+    "adscf_batch_set_precision": (C.c_int, [_ctxp, C.c_int]),
+    "adscf_batch_iterate": (C.c_int, [_ctxp, C.c_int, C.c_double, _ip, _dp, _ip]),
+    "adscf_batch_energy": (C.c_int, [_ctxp, _dp]),
+    "adscf_batch_get": (C.c_int, [_ctxp, C.c_int, _dp, _dp]),
+    "adscf_batch_timing": (C.c_int, [_ctxp, _dp, _dp]),
+    "adscf_batch_autotune": (C.c_int, [_ctxp, C.c_int, C.c_char_p, C.c_int]),
     "adscf_adjoint": (C.c_int, [_ctxp, C.c_int, _ip, _ip, C.c_double, _dp, C.c_int, _dp, C.c_double, C.c_int,
                                 _dp, _dp, _dp, _dp, _dp]),
 }
@@ -381,6 +392,95 @@ class Context:
                                           _p(pt), float(lam), int(checkpoint_every), C.byref(loss), C.byref(F),
                                           _p(gw), _p(gb), _p(phi)))
         return dict(loss=loss.value, F=F.value, phi=phi, grad_w=gw, grad_box=gb)
+
+
+class Batch:
+    """Independent two-type SCF instances run together (adscf_batch_*).
+
+    instances: list of dicts with keys grid (nx,ny,nz), box (lx,ly,lz), ds, blocks (types),
+    segments, kappa, chi (2x2), vol (2)."""
+
+    def __init__(self, instances, ensemble=1, rho=1.0, mixing_dim=20, lam_mix=0.5, precision=FP32):
+        self._lib = load_library()
+        self.instances = instances
+        n = len(instances)
+        grid = np.ascontiguousarray([v for d in instances for v in d["grid"]], dtype=np.int32)
+        box = _f64([v for d in instances for v in d["box"]])
+        ds = _f64([d["ds"] for d in instances])
+        nb = np.ascontiguousarray([len(d["blocks"]) for d in instances], dtype=np.int32)
+        bt = np.ascontiguousarray([v for d in instances for v in d["blocks"]], dtype=np.int32)
+        bs = np.ascontiguousarray([v for d in instances for v in d["segments"]], dtype=np.int32)
+        kappa = _f64([d["kappa"] for d in instances])
+        chi = _f64([v for d in instances for v in np.asarray(d["chi"], dtype=np.float64).T.ravel()])
+        vol = _f64([v for d in instances for v in d["vol"]])
+        h = _ctxp()
+        rc = self._lib.adscf_batch_create(C.byref(h), n, _pi(grid), _p(box), _p(ds), _pi(nb), _pi(bt), _pi(bs),
+                                          _p(kappa), _p(chi), _p(vol), int(ensemble), float(rho), int(mixing_dim),
+                                          float(lam_mix), int(precision))
+        if rc != OK:
+            raise AdscfError(rc, "adscf_batch_create failed")
+        self._h = h
+        self.n = n
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.adscf_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != OK:
+            raise AdscfError(rc, self._lib.adscf_batch_last_error(self._h).decode())
+
+    def shape(self, i):
+        nx, ny, nz = self.instances[i]["grid"]
+        return (2, nz, ny, nx)
+
+    def set_fields(self, fields):
+        flat = np.concatenate([_f64(f, self.shape(i)).ravel() for i, f in enumerate(fields)])
+        self._chk(self._lib.adscf_batch_set_fields(self._h, _p(flat)))
+
+    def set_precision(self, precision):
+        self._chk(self._lib.adscf_batch_set_precision(self._h, int(precision)))
+
+    def iterate(self, niter, tol=0.0):
+        it = np.zeros(self.n, dtype=np.int32)
+        acc = np.zeros(self.n)
+        conv = np.zeros(self.n, dtype=np.int32)
+        self._chk(self._lib.adscf_batch_iterate(self._h, int(niter), float(tol), _pi(it), _p(acc), _pi(conv)))
+        return it, acc, conv.astype(bool)
+
+    def energy(self):
+        f = np.zeros(self.n)
+        self._chk(self._lib.adscf_batch_energy(self._h, _p(f)))
+        return f
+
+    def get(self, i):
+        f = np.empty(self.shape(i))
+        d = np.empty(self.shape(i))
+        self._chk(self._lib.adscf_batch_get(self._h, int(i), _p(f), _p(d)))
+        return f, d
+
+    def timing(self):
+        a, b = C.c_double(), C.c_double()
+        self._chk(self._lib.adscf_batch_timing(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, reps=3):
+        buf = C.create_string_buffer(1 << 16)
+        self._chk(self._lib.adscf_batch_autotune(self._h, int(reps), buf, len(buf)))
+        return buf.value.decode()
 
 
 # ---- host routines of module Iterate (no GPU needed) -----------------------

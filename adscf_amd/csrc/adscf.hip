@@ -265,10 +265,11 @@ const RegShape kRegShapes[] = {
 
 int reg_tpl_rt(int N1, int N2) { return N1 == 1 ? 1 : std::max(N1, N2); }
 
-template <typename T, int N1, int N2>
-int launch_reg(adscf_ctx* c, PassArgs<T> p, int waves) {
-  constexpr int TPL = reg_tpl<N1, N2>();
-  // `waves` wavefronts per block; the tile width L sets the lines per block
+// Tile geometry of a register-resident pass: `waves` wavefronts per block, L lines
+// per block; fills p.L and p.gx and returns the number of blocks.
+template <typename T>
+int prep_reg(const adscf_ctx* c, PassArgs<T>& p, int N1, int N2, int waves, int* threads, size_t* smem) {
+  const int TPL = N1 == 1 ? 1 : std::max(N1, N2);
   int L = std::max(1, 64 * waves / TPL);
   if (p.axis != 0) L = std::min(L, c->nx);
   p.L = L;
@@ -277,8 +278,17 @@ int launch_reg(adscf_ctx* c, PassArgs<T> p, int waves) {
   if (p.axis == 0) items = (int)(((long)c->ny * c->nz + L - 1) / L);
   else if (p.axis == 1) items = c->nz * p.gx;
   else items = c->ny * p.gx;
-  const size_t smem = N1 == 1 ? 0 : (size_t)L * FourStep<N1, N2>::lds_elems * sizeof(C2<T>);
-  hipLaunchKernelGGL((k_line_pass_reg<T, N1, N2>), dim3(items), dim3(TPL * L), smem, c->stream, p);
+  if (threads) *threads = TPL * L;
+  if (smem) *smem = N1 == 1 ? 0 : (size_t)L * (size_t)N2 * (N1 + 1) * sizeof(C2<T>);
+  return items;
+}
+
+template <typename T, int N1, int N2>
+int launch_reg(adscf_ctx* c, PassArgs<T> p, int waves) {
+  int threads;
+  size_t smem;
+  const int items = prep_reg<T>(c, p, N1, N2, waves, &threads, &smem);
+  hipLaunchKernelGGL((k_line_pass_reg<T, N1, N2>), dim3(items), dim3(threads), smem, c->stream, p);
   HIPCK(hipGetLastError());
   return items;
 }
@@ -484,11 +494,14 @@ void ensure_partials(adscf_ctx* c) {
 
 // sweep + Q + normalisation for the chain described by cd, E already in place
 template <typename T>
-void run_chain(adscf_ctx* c, const ChainDesc& cd, ScfState* st, const int* done, double rho, double mu) {
+void run_chain(adscf_ctx* c, const ChainDesc& cd, ScfState* st, const int* done, double rho, double mu,
+               bool passes_too = true) {
   Engine<T>& En = eng<T>(c);
   const long ng = c->ng;
-  auto passes = build_sweep<T>(c, cd, En.E.p, En.h1.p, En.h2.p, c->d1.p, c->d2.p, done);
-  for (auto& p : passes) launch_pass<T>(c, p);
+  if (passes_too) {
+    auto passes = build_sweep<T>(c, cd, En.E.p, En.h1.p, En.h2.p, c->d1.p, c->d2.p, done);
+    for (auto& p : passes) launch_pass<T>(c, p);
+  }
   const int nb = grid_for(ng);
   hipLaunchKernelGGL(k_qsum<T>, dim3(nb), dim3(kThreads), 0, c->stream, En.h1.p + (long)cd.ns * ng, ng, c->qpart.p,
                      done);
@@ -756,12 +769,12 @@ int c2r_impl(adscf_ctx* c, const double* in, double* prop) {
 // device-resident SCF iteration
 // ---------------------------------------------------------------------------
 template <typename T>
-void enqueue_iteration(adscf_ctx* c) {
+void enqueue_iteration(adscf_ctx* c, bool passes_too = true) {
   Engine<T>& En = eng<T>(c);
   const long ng = c->ng;
   const long nvar = ng * c->nt;
   const int* done = &c->st.p->done;
-  run_chain<T>(c, c->cd, c->st.p, done, c->rho, c->mu);
+  run_chain<T>(c, c->cd, c->st.p, done, c->rho, c->mu, passes_too);
   const int nb = grid_for(ng);
   ContourArgs<T> a = contour_args<T>(c, c->cd, c->st.p);
   a.mode = CONTOUR_SCF;
@@ -1349,6 +1362,210 @@ int adjoint_impl(adscf_ctx* c, const ChainDesc& cd, double rho, const double* fi
     for (int a = 0; a < 3; ++a) grad_box[a] = 2.0 * c->ds / lens[a] * sa[a] + lam * F / lens[a];
   }
   return ADSCF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched instances: one launch per line pass for all instances
+// ---------------------------------------------------------------------------
+template <typename T>
+struct BatchTables {
+  DevBuf<PassArgs<T>> args;
+  DevBuf<int> start, shape;
+  std::vector<size_t> arg_off, start_off;
+  std::vector<int> count, blocks;
+  size_t smem = 0;
+};
+
+int batch_shape_code(const AxisTables& ax) { return ax.N1 * 100 + ax.N2; }
+
+bool batch_shape_ok(int N1, int N2) {
+  for (const auto& s : kBatchShapes)
+    if (s[0] == N1 && s[1] == N2) return true;
+  return false;
+}
+
+// every axis of a batch instance needs a one-wave register schedule the batched kernel knows
+void batch_fix_schedule(adscf_ctx* c) {
+  for (int a = 0; a < 3; ++a) {
+    AxisTables& ax = c->ax[a];
+    if (ax.n < 2) continue;
+    ax.waves = 1;
+    if (ax.N1 && batch_shape_ok(ax.N1, ax.N2)) continue;
+    bool found = false;
+    for (const RegShape& s : kRegShapes)
+      if (s.n == ax.n && batch_shape_ok(s.N1, s.N2)) {
+        ax.N1 = s.N1;
+        ax.N2 = s.N2;
+        found = true;
+        break;
+      }
+    if (!found) throw ArgError{"batched instances need axis lengths 8, 16, 24, 32, 48, 64 or 128"};
+  }
+}
+
+}  // namespace
+
+struct adscf_batch {
+  hipStream_t stream = nullptr;
+  int prec = ADSCF_FP32;
+  std::vector<adscf_ctx*> inst;
+  BatchTables<double> t64;
+  BatchTables<float> t32;
+  bool tables_ok = false;
+  hipGraph_t graph_src = nullptr;
+  hipGraphExec_t graph = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms_iter = 0;
+  std::string err;
+};
+
+namespace {
+
+template <typename T>
+BatchTables<T>& btab(adscf_batch* b);
+template <>
+BatchTables<double>& btab<double>(adscf_batch* b) { return b->t64; }
+template <>
+BatchTables<float>& btab<float>(adscf_batch* b) { return b->t32; }
+
+template <typename T>
+void batch_build_tables(adscf_batch* b) {
+  BatchTables<T>& tb = btab<T>(b);
+  std::vector<std::vector<PassArgs<T>>> per;
+  std::vector<std::vector<int>> items, shapes;
+  size_t npass = 0;
+  tb.smem = 0;
+  for (adscf_ctx* c : b->inst) {
+    ensure_tables<T>(c);
+    Engine<T>& En = eng<T>(c);
+    auto ps = build_sweep<T>(c, c->cd, En.E.p, En.h1.p, En.h2.p, c->d1.p, c->d2.p, &c->st.p->done);
+    std::vector<int> it, sh;
+    for (auto& p : ps) {
+      const AxisTables& ax = c->ax[p.axis];
+      size_t smem;
+      it.push_back(prep_reg<T>(c, p, ax.N1, ax.N2, 1, nullptr, &smem));
+      sh.push_back(batch_shape_code(ax));
+      tb.smem = std::max(tb.smem, smem);
+    }
+    npass = std::max(npass, ps.size());
+    per.push_back(std::move(ps));
+    items.push_back(std::move(it));
+    shapes.push_back(std::move(sh));
+  }
+  std::vector<PassArgs<T>> args;
+  std::vector<int> start, shape;
+  tb.arg_off.clear();
+  tb.start_off.clear();
+  tb.count.clear();
+  tb.blocks.clear();
+  for (size_t p = 0; p < npass; ++p) {
+    tb.arg_off.push_back(args.size());
+    tb.start_off.push_back(start.size());
+    int blocks = 0, cnt = 0;
+    for (size_t i = 0; i < per.size(); ++i) {
+      if (p >= per[i].size()) continue;
+      args.push_back(per[i][p]);
+      start.push_back(blocks);
+      shape.push_back(shapes[i][p]);
+      blocks += items[i][p];
+      ++cnt;
+    }
+    start.push_back(blocks);
+    tb.count.push_back(cnt);
+    tb.blocks.push_back(blocks);
+  }
+  tb.args.ensure(args.size());
+  tb.start.ensure(start.size());
+  tb.shape.ensure(shape.size());
+  HIPCK(hipMemcpy(tb.args.p, args.data(), args.size() * sizeof(PassArgs<T>), hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(tb.start.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(tb.shape.p, shape.data(), shape.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+template <typename T>
+void batch_enqueue_sweep(adscf_batch* b) {
+  BatchTables<T>& tb = btab<T>(b);
+  for (size_t p = 0; p < tb.count.size(); ++p) {
+    BatchPass<T> bp{tb.args.p + tb.arg_off[p], tb.start.p + tb.start_off[p], tb.shape.p + tb.arg_off[p],
+                    tb.count[p]};
+    hipLaunchKernelGGL(k_line_pass_batch<T>, dim3(tb.blocks[p]), dim3(64), tb.smem, b->stream, bp);
+  }
+  HIPCK(hipGetLastError());
+}
+
+void batch_drop_graph(adscf_batch* b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  if (b->graph_src) (void)hipGraphDestroy(b->graph_src);
+  b->graph = nullptr;
+  b->graph_src = nullptr;
+}
+
+template <typename T>
+void batch_build_graph(adscf_batch* b) {
+  if (b->graph) return;
+  batch_build_tables<T>(b);
+  HIPCK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    batch_enqueue_sweep<T>(b);
+    for (adscf_ctx* c : b->inst) enqueue_iteration<T>(c, false);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(b->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  HIPCK(hipStreamEndCapture(b->stream, &b->graph_src));
+  HIPCK(hipGraphInstantiate(&b->graph, b->graph_src, nullptr, nullptr, 0));
+}
+
+template <typename T>
+double batch_sweep_timing(adscf_batch* b, int reps) {
+  batch_build_tables<T>(b);
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  HIPCK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  batch_enqueue_sweep<T>(b);
+  HIPCK(hipStreamEndCapture(b->stream, &g));
+  HIPCK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  HIPCK(hipGraphLaunch(ge, b->stream));
+  HIPCK(hipEventRecord(b->ev0, b->stream));
+  for (int r = 0; r < reps; ++r) HIPCK(hipGraphLaunch(ge, b->stream));
+  HIPCK(hipEventRecord(b->ev1, b->stream));
+  HIPCK(hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  HIPCK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  (void)hipGraphExecDestroy(ge);
+  (void)hipGraphDestroy(g);
+  return ms / reps;
+}
+
+template <typename F>
+int bguard(adscf_batch* b, F&& f) {
+  if (!b) return ADSCF_ERR_ARG;
+  try {
+    return f();
+  } catch (const HipError& e) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "HIP error %d (%s) in %s [adscf.hip:%d]", (int)e.e, hipGetErrorString(e.e), e.what,
+             e.line);
+    b->err = buf;
+    return ADSCF_ERR_HIP;
+  } catch (const ArgError& e) {
+    b->err = e.msg;
+    return ADSCF_ERR_ARG;
+  } catch (const std::bad_alloc&) {
+    b->err = "host allocation failed";
+    return ADSCF_ERR_HIP;
+  }
+}
+
+// the first failing instance's message becomes the batch's
+int bcheck(adscf_batch* b, adscf_ctx* c, int rc) {
+  if (rc != ADSCF_OK) {
+    b->err = c->err;
+    throw ArgError{c->err};
+  }
+  return rc;
 }
 }  // namespace
 
@@ -2012,6 +2229,208 @@ int adscf_set_schedule(adscf_ctx* c, int axis, int n1, int n2, int waves) {
     ax.N2 = n1 ? n2 : 0;
     ax.waves = waves;
     invalidate_tables(c);
+    return ADSCF_OK;
+  });
+}
+
+int adscf_batch_create(adscf_batch** out, int ninst, const int* grid, const double* box, const double* ds,
+                       const int* nblocks, const int* monomer_type, const int* block_segments, const double* kappa,
+                       const double* chi, const double* vol, int ensemble, double rho, int mixing_dim,
+                       double lam_mix, int precision) {
+  if (!out || ninst < 1 || !grid || !box || !ds || !nblocks || !monomer_type || !block_segments || !kappa || !chi ||
+      !vol)
+    return ADSCF_ERR_ARG;
+  *out = nullptr;
+  auto* b = new (std::nothrow) adscf_batch();
+  if (!b) return ADSCF_ERR_HIP;
+  b->prec = precision;
+  int rc = bguard(b, [&]() -> int {
+    HIPCK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HIPCK(hipEventCreate(&b->ev0));
+    HIPCK(hipEventCreate(&b->ev1));
+    int boff = 0;
+    for (int i = 0; i < ninst; ++i) {
+      adscf_ctx* c = nullptr;
+      int r = adscf_init(&c, grid[3 * i], grid[3 * i + 1], grid[3 * i + 2], 2, box[3 * i], box[3 * i + 1],
+                         box[3 * i + 2], ds[i], precision);
+      if (r != ADSCF_OK) throw ArgError{"batch instance init failed"};
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipStreamDestroy(c->stream);
+      c->stream = b->stream;  // every instance runs on the batch stream
+      b->inst.push_back(c);
+      bcheck(b, c, adscf_set_interactions(c, kappa[i], chi + 4 * i, vol + 2 * i));
+      bcheck(b, c, adscf_scf_setup(c, nblocks[i], monomer_type + boff, block_segments + boff, ensemble, rho, 0.0,
+                                   mixing_dim, lam_mix));
+      boff += nblocks[i];
+      batch_fix_schedule(c);
+    }
+    return ADSCF_OK;
+  });
+  if (rc != ADSCF_OK) {
+    fprintf(stderr, "adscf_batch_create: %s\n", b->err.c_str());
+    adscf_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return ADSCF_OK;
+}
+
+int adscf_batch_destroy(adscf_batch* b) {
+  if (!b) return ADSCF_ERR_ARG;
+  batch_drop_graph(b);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (adscf_ctx* c : b->inst) {
+    c->stream = nullptr;
+    adscf_finalize(c);
+  }
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return ADSCF_OK;
+}
+
+const char* adscf_batch_last_error(const adscf_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int adscf_batch_set_fields(adscf_batch* b, const double* fields) {
+  return bguard(b, [&]() -> int {
+    if (!fields) throw ArgError{"null fields"};
+    size_t off = 0;
+    for (adscf_ctx* c : b->inst) {
+      bcheck(b, c, adscf_scf_set_field(c, fields + off));
+      off += (size_t)c->nt * c->ng;
+    }
+    batch_drop_graph(b);
+    return ADSCF_OK;
+  });
+}
+
+int adscf_batch_set_precision(adscf_batch* b, int precision) {
+  return bguard(b, [&]() -> int {
+    for (adscf_ctx* c : b->inst) {
+      bcheck(b, c, adscf_set_precision(c, precision));
+      batch_fix_schedule(c);
+    }
+    b->prec = precision;
+    batch_drop_graph(b);
+    return ADSCF_OK;
+  });
+}
+
+int adscf_batch_iterate(adscf_batch* b, int niter, double tol, int* iters, double* accuracy, int* converged) {
+  return bguard(b, [&]() -> int {
+    for (adscf_ctx* c : b->inst)
+      if (!c->field_set) throw ArgError{"adscf_batch_set_fields has not been called"};
+    if (b->prec == ADSCF_FP64) batch_build_graph<double>(b);
+    else batch_build_graph<float>(b);
+    for (adscf_ctx* c : b->inst)
+      HIPCK(hipMemcpyAsync(&c->st.p->tol, &tol, sizeof(double), hipMemcpyHostToDevice, b->stream));
+    const int n = (int)b->inst.size();
+    std::vector<ScfState> st(n);
+    int done_iters = 0;
+    float total = 0.f;
+    while (done_iters < niter) {
+      const int chunk = std::min(niter - done_iters, tol > 0 ? 25 : niter);
+      HIPCK(hipEventRecord(b->ev0, b->stream));
+      for (int i = 0; i < chunk; ++i) HIPCK(hipGraphLaunch(b->graph, b->stream));
+      HIPCK(hipEventRecord(b->ev1, b->stream));
+      for (int i = 0; i < n; ++i)
+        HIPCK(hipMemcpyAsync(&st[i], b->inst[i]->st.p, sizeof(ScfState), hipMemcpyDeviceToHost, b->stream));
+      HIPCK(hipStreamSynchronize(b->stream));
+      float ms = 0.f;
+      HIPCK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+      total += ms;
+      done_iters += chunk;
+      bool all = true;
+      for (auto& s : st) all = all && s.done;
+      if (all) break;
+    }
+    b->last_ms_iter = done_iters ? total / done_iters : 0.0;
+    for (int i = 0; i < n; ++i) {
+      b->inst[i]->phi_fresh = st[i].done != 0;
+      if (iters) iters[i] = st[i].iter;
+      if (accuracy) accuracy[i] = st[i].accuracy;
+      if (converged) converged[i] = st[i].done;
+    }
+    return ADSCF_OK;
+  });
+}
+
+int adscf_batch_energy(adscf_batch* b, double* F_per_chain) {
+  return bguard(b, [&]() -> int {
+    for (size_t i = 0; i < b->inst.size(); ++i) {
+      double fc = 0.0;
+      bcheck(b, b->inst[i], adscf_scf_energy(b->inst[i], nullptr, &fc, nullptr, nullptr, nullptr));
+      if (F_per_chain) F_per_chain[i] = fc;
+    }
+    return ADSCF_OK;
+  });
+}
+
+int adscf_batch_get(adscf_batch* b, int i, double* field, double* density) {
+  return bguard(b, [&]() -> int {
+    if (i < 0 || i >= (int)b->inst.size()) throw ArgError{"instance index out of range"};
+    return bcheck(b, b->inst[i], adscf_scf_get(b->inst[i], field, density));
+  });
+}
+
+int adscf_batch_timing(adscf_batch* b, double* ms_per_iter, double* ms_sweep) {
+  return bguard(b, [&]() -> int {
+    if (ms_per_iter) *ms_per_iter = b->last_ms_iter;
+    if (ms_sweep) *ms_sweep = b->prec == ADSCF_FP64 ? batch_sweep_timing<double>(b, 3) : batch_sweep_timing<float>(b, 3);
+    return ADSCF_OK;
+  });
+}
+
+// Autotuner for the batch: for every distinct axis length, try each one-wave
+// register shape on all instances together and keep the fastest batched sweep.
+int adscf_batch_autotune(adscf_batch* b, int reps, char* report, int report_cap) {
+  return bguard(b, [&]() -> int {
+    std::string rep;
+    char line[256];
+    std::vector<int> lengths;
+    for (adscf_ctx* c : b->inst)
+      for (int a = 0; a < 3; ++a)
+        if (c->ax[a].n > 1 && std::find(lengths.begin(), lengths.end(), c->ax[a].n) == lengths.end())
+          lengths.push_back(c->ax[a].n);
+    for (int n : lengths) {
+      std::vector<std::pair<int, int>> cands;
+      for (const RegShape& s : kRegShapes)
+        if (s.n == n && batch_shape_ok(s.N1, s.N2)) cands.push_back({s.N1, s.N2});
+      double best = 1e300;
+      std::pair<int, int> bestsh = cands.empty() ? std::make_pair(0, 0) : cands[0];
+      for (auto sh : cands) {
+        for (adscf_ctx* c : b->inst)
+          for (int a = 0; a < 3; ++a)
+            if (c->ax[a].n == n) {
+              c->ax[a].N1 = sh.first;
+              c->ax[a].N2 = sh.second;
+              c->ax[a].waves = 1;
+            }
+        const double ms = b->prec == ADSCF_FP64 ? batch_sweep_timing<double>(b, std::max(reps, 1))
+                                                : batch_sweep_timing<float>(b, std::max(reps, 1));
+        snprintf(line, sizeof line, "n=%d shape=%dx%d batched sweep=%.4f ms\n", n, sh.first, sh.second, ms);
+        rep += line;
+        if (ms < best) {
+          best = ms;
+          bestsh = sh;
+        }
+      }
+      for (adscf_ctx* c : b->inst)
+        for (int a = 0; a < 3; ++a)
+          if (c->ax[a].n == n) {
+            c->ax[a].N1 = bestsh.first;
+            c->ax[a].N2 = bestsh.second;
+          }
+      snprintf(line, sizeof line, "n=%d chosen %dx%d (%.4f ms)\n", n, bestsh.first, bestsh.second, best);
+      rep += line;
+    }
+    batch_drop_graph(b);
+    if (report && report_cap > 0) {
+      const size_t k = std::min(rep.size(), (size_t)report_cap - 1);
+      memcpy(report, rep.data(), k);
+      report[k] = 0;
+    }
     return ADSCF_OK;
   });
 }

@@ -268,11 +268,9 @@ constexpr int reg_tpl() {
 // Same contract as k_line_pass; a line lives in registers of reg_tpl threads,
 // each holding the N2 points j1 + N1*j2 of its j1.  blockDim = reg_tpl * L.
 template <typename T, int N1, int N2>
-__global__ void __launch_bounds__(kThreads) k_line_pass_reg(PassArgs<T> a) {
+__device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, unsigned char* smem_raw) {
   using FS = FourStep<N1, N2>;
   constexpr int TPL = reg_tpl<N1, N2>();
-  if (a.done && *a.done) return;
-  extern __shared__ unsigned char smem_raw[];
   const int tid = threadIdx.x;
   int l, t;
   if (a.axis == 0) {
@@ -283,7 +281,7 @@ __global__ void __launch_bounds__(kThreads) k_line_pass_reg(PassArgs<T> a) {
     t = tid / a.L;
   }
   bool valid;
-  const long base = pass_line_base(a, blockIdx.x, l, valid);
+  const long base = pass_line_base(a, item, l, valid);
   C2<T>* lds = reinterpret_cast<C2<T>*>(smem_raw) + l * FS::lds_elems;
   const bool act = t < N1 && valid;
   C2<T> v[N2];
@@ -353,6 +351,59 @@ __global__ void __launch_bounds__(kThreads) k_line_pass_reg(PassArgs<T> a) {
     }
   }
 }
+
+template <typename T, int N1, int N2>
+__global__ void __launch_bounds__(kThreads) k_line_pass_reg(PassArgs<T> a) {
+  if (a.done && *a.done) return;
+  extern __shared__ unsigned char smem_raw[];
+  reg_pass_body<T, N1, N2>(a, blockIdx.x, smem_raw);
+}
+
+// One pass of many independent instances in one launch: block b belongs to the
+// instance i with start[i] <= b < start[i+1]; every instance carries its own
+// arguments (grid, buffers, step types) and four-step shape.  Blocks are one
+// wavefront for every shape.
+template <typename T>
+struct BatchPass {
+  const PassArgs<T>* args;
+  const int* start;  // n + 1 block offsets
+  const int* shape;  // n codes N1 * 100 + N2
+  int n;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_line_pass_batch(BatchPass<T> tb) {
+  int lo = 0, hi = tb.n - 1;
+  const int b = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (tb.start[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const PassArgs<T> a = tb.args[lo];
+  if (a.done && *a.done) return;
+  extern __shared__ unsigned char smem_raw[];
+  const int item = b - tb.start[lo];
+  switch (tb.shape[lo]) {
+    case 408: reg_pass_body<T, 4, 8>(a, item, smem_raw); break;
+    case 804: reg_pass_body<T, 8, 4>(a, item, smem_raw); break;
+    case 216: reg_pass_body<T, 2, 16>(a, item, smem_raw); break;
+    case 806: reg_pass_body<T, 8, 6>(a, item, smem_raw); break;
+    case 608: reg_pass_body<T, 6, 8>(a, item, smem_raw); break;
+    case 1603: reg_pass_body<T, 16, 3>(a, item, smem_raw); break;
+    case 808: reg_pass_body<T, 8, 8>(a, item, smem_raw); break;
+    case 416: reg_pass_body<T, 4, 16>(a, item, smem_raw); break;
+    case 1604: reg_pass_body<T, 16, 4>(a, item, smem_raw); break;
+    case 816: reg_pass_body<T, 8, 16>(a, item, smem_raw); break;
+    case 1608: reg_pass_body<T, 16, 8>(a, item, smem_raw); break;
+    case 116: reg_pass_body<T, 1, 16>(a, item, smem_raw); break;
+    case 108: reg_pass_body<T, 1, 8>(a, item, smem_raw); break;
+    default: break;
+  }
+}
+
+constexpr int kBatchShapes[][2] = {{4, 8}, {8, 4}, {2, 16}, {8, 6}, {6, 8}, {16, 3}, {8, 8},
+                                   {4, 16}, {16, 4}, {8, 16}, {16, 8}, {1, 16}, {1, 8}};
 
 // ---------------------------------------------------------------------------
 // generic line FFT along one axis of a complex array (compat r2c / c2r)

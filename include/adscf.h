@@ -40,6 +40,7 @@ extern "C" {
 #define ADSCF_FP32 1 /* FP32-mixed: propagators/FFTs in float; Q, F, phi sums, Anderson in double */
 
 typedef struct adscf_ctx adscf_ctx;
+typedef struct adscf_batch adscf_batch;
 
 /* ---- lifetime ---------------------------------------------------------- */
 
@@ -151,6 +152,30 @@ int adscf_autotune(adscf_ctx* ctx, int reps, char* report, int report_cap);
 /* current schedule of an axis (0 x, 1 y, 2 z); n1 = 0 means the Stockham kernel */
 int adscf_get_schedule(const adscf_ctx* ctx, int axis, int* n, int* n1, int* n2, int* waves);
 int adscf_set_schedule(adscf_ctx* ctx, int axis, int n1, int n2, int waves);
+
+/* ---- batched independent instances (no counterpart in the reference) ---------- */
+/* Many independent two-type SCF problems (parameter sweeps) of uneven grid and contour
+ * sizes, run together: each line pass of the propagator sweep is one launch over all
+ * instances, and one HIP graph replay is one SCF iteration of every instance.
+ * Per instance i: grid[3i..3i+2], box[3i..3i+2], ds[i], nblocks[i] blocks whose types and
+ * segments follow in monomer_type/block_segments (concatenated), kappa[i], chi[4i..4i+3]
+ * (2x2 column-major), vol[2i..2i+1].  Axis lengths must be 8, 16, 24, 32, 48, 64 or 128. */
+int adscf_batch_create(adscf_batch** batch, int ninst, const int* grid, const double* box, const double* ds,
+                       const int* nblocks, const int* monomer_type, const int* block_segments, const double* kappa,
+                       const double* chi, const double* vol, int ensemble, double rho, int mixing_dim,
+                       double lam_mix, int precision);
+int adscf_batch_destroy(adscf_batch* batch);
+const char* adscf_batch_last_error(const adscf_batch* batch);
+/* fields of all instances concatenated, each (nx,ny,nz,2) column-major */
+int adscf_batch_set_fields(adscf_batch* batch, const double* fields);
+int adscf_batch_set_precision(adscf_batch* batch, int precision);
+/* per-instance outputs (arrays of ninst); stops when every instance has converged */
+int adscf_batch_iterate(adscf_batch* batch, int niter, double tol, int* iters, double* accuracy, int* converged);
+int adscf_batch_energy(adscf_batch* batch, double* F_per_chain);
+int adscf_batch_get(adscf_batch* batch, int instance, double* field, double* density);
+int adscf_batch_timing(adscf_batch* batch, double* ms_per_iter, double* ms_sweep);
+/* picks the register-resident shape per distinct axis length by timing the batched sweep */
+int adscf_batch_autotune(adscf_batch* batch, int reps, char* report, int report_cap);
 
 /* ---- adjoint (no counterpart in the reference) ---------------------------- */
 /* Reverse-mode derivative of one canonical density evaluation of the chain
