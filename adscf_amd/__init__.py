@@ -91,6 +91,7 @@ SIGNATURES = {
     "adscf_batch_get": (C.c_int, [_ctxp, C.c_int, _dp, _dp]),
     "adscf_batch_timing": (C.c_int, [_ctxp, _dp, _dp]),
     "adscf_batch_autotune": (C.c_int, [_ctxp, C.c_int, C.c_char_p, C.c_int]),
+    "adscf_hipfft_step_time": (C.c_int, [_ctxp, C.c_int, _dp]),
     "adscf_adjoint": (C.c_int, [_ctxp, C.c_int, _ip, _ip, C.c_double, _dp, C.c_int, _dp, C.c_double, C.c_int,
                                 _dp, _dp, _dp, _dp, _dp]),
 }
@@ -359,6 +360,12 @@ class Context:
         self._chk(self._lib.adscf_scf_timing(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+
+    def hipfft_step_us(self, reps=200):
+        """Time of one single-chain step built from hipFFT r2c/c2r (comparator only)."""
+        v = C.c_double()
+        self._chk(self._lib.adscf_hipfft_step_time(self._h, int(reps), C.byref(v)))
+        return v.value
 
     # -- schedules ------------------------------------------------------------
     def autotune(self, reps=3):

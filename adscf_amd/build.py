@@ -23,7 +23,7 @@ FLAGS = [
 def build(verbose: bool = False) -> str:
     """Compile the library; returns its path.  Raises CalledProcessError on failure."""
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, SRC]
+    cmd = [HIPCC, *FLAGS, "-o", tmp, SRC, "-lhipfft"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     out = "\n".join(l for l in (res.stdout + res.stderr).splitlines() if "packed-fp32-ops" not in l)
     if res.returncode != 0:

@@ -5,6 +5,7 @@
 // copy out; the device-resident SCF loop keeps fields, propagators and the
 // Anderson history in HBM and replays one captured HIP graph per iteration.
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 
 #include <algorithm>
 #include <cmath>
@@ -1013,6 +1014,72 @@ std::string autotune_impl(adscf_ctx* c, int reps) {
     c->graph_src = nullptr;
   }
   return rep;
+}
+
+
+// ---------------------------------------------------------------------------
+// hipFFT comparator: the reference's step structure (mod_propagate.f90:56-64)
+// with vendor r2c/c2r transforms; used only to time against the line passes.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_cmp_mulE(T* __restrict__ q, const T* __restrict__ E, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) q[i] *= E[i];
+}
+
+template <typename T>
+__global__ void k_cmp_mulK(C2<T>* __restrict__ s, const T* __restrict__ kx, const T* __restrict__ ky,
+                           const T* __restrict__ kz, int nh, int ny, int nz) {
+  const long n = (long)nh * ny * nz;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(i % nh), y = (int)((i / nh) % ny), z = (int)(i / ((long)nh * ny));
+    const T f = kx[x] * ky[y] * kz[z];
+    s[i].x *= f;
+    s[i].y *= f;
+  }
+}
+
+template <typename T>
+double hipfft_step_us(adscf_ctx* c, int reps) {
+  ensure_tables<T>(c);
+  Engine<T>& En = eng<T>(c);
+  const long ng = c->ng;
+  const int nh = c->nx / 2 + 1;
+  const long nk = (long)nh * c->ny * c->nz;
+  DevBuf<T> q, E;
+  DevBuf<C2<T>> s;
+  q.ensure(ng);
+  E.ensure(ng);
+  s.ensure(nk);
+  hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(ng)), dim3(kThreads), 0, c->stream, q.p, T(1), ng);
+  hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(ng)), dim3(kThreads), 0, c->stream, E.p, T(0.999), ng);
+  hipfftHandle fwd, inv;
+  const bool dbl = std::is_same<T, double>::value;
+  if (hipfftPlan3d(&fwd, c->nz, c->ny, c->nx, dbl ? HIPFFT_D2Z : HIPFFT_R2C) != HIPFFT_SUCCESS ||
+      hipfftPlan3d(&inv, c->nz, c->ny, c->nx, dbl ? HIPFFT_Z2D : HIPFFT_C2R) != HIPFFT_SUCCESS)
+    throw ArgError{"hipfftPlan3d failed"};
+  hipfftSetStream(fwd, c->stream);
+  hipfftSetStream(inv, c->stream);
+  // K/n per axis: the product over axes is exp(-ds k^2)/Ng
+  auto step = [&]() {
+    hipLaunchKernelGGL(k_cmp_mulE<T>, dim3(grid_for(ng)), dim3(kThreads), 0, c->stream, q.p, (const T*)E.p, ng);
+    if (dbl) hipfftExecD2Z(fwd, (hipfftDoubleReal*)q.p, (hipfftDoubleComplex*)s.p);
+    else hipfftExecR2C(fwd, (hipfftReal*)q.p, (hipfftComplex*)s.p);
+    hipLaunchKernelGGL(k_cmp_mulK<T>, dim3(grid_for(nk)), dim3(kThreads), 0, c->stream, s.p,
+                       (const T*)En.kf[0].p, (const T*)En.kf[1].p, (const T*)En.kf[2].p, nh, c->ny, c->nz);
+    if (dbl) hipfftExecZ2D(inv, (hipfftDoubleComplex*)s.p, (hipfftDoubleReal*)q.p);
+    else hipfftExecC2R(inv, (hipfftComplex*)s.p, (hipfftReal*)q.p);
+    hipLaunchKernelGGL(k_cmp_mulE<T>, dim3(grid_for(ng)), dim3(kThreads), 0, c->stream, q.p, (const T*)E.p, ng);
+  };
+  for (int r = 0; r < 5; ++r) step();
+  HIPCK(hipEventRecord(c->ev0, c->stream));
+  for (int r = 0; r < reps; ++r) step();
+  HIPCK(hipEventRecord(c->ev1, c->stream));
+  HIPCK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  hipfftDestroy(fwd);
+  hipfftDestroy(inv);
+  return 1000.0 * ms / reps;
 }
 
 void reset_axes(adscf_ctx* c) {
@@ -2431,6 +2498,14 @@ int adscf_batch_autotune(adscf_batch* b, int reps, char* report, int report_cap)
       memcpy(report, rep.data(), k);
       report[k] = 0;
     }
+    return ADSCF_OK;
+  });
+}
+
+int adscf_hipfft_step_time(adscf_ctx* c, int reps, double* us_per_step) {
+  return guarded(c, [&]() -> int {
+    if (!us_per_step || reps < 1) throw ArgError{"bad arguments"};
+    *us_per_step = c->prec == ADSCF_FP64 ? hipfft_step_us<double>(c, reps) : hipfft_step_us<float>(c, reps);
     return ADSCF_OK;
   });
 }

@@ -112,3 +112,25 @@ def unit_directions(count, sizes, seed=2007):
     n = int(sum(sizes))
     out = st.normals(count * n).reshape(count, n)
     return out / np.linalg.norm(out, axis=1, keepdims=True)
+
+
+def batched_instances(seed=2007):
+    """The 64-instance parameter sweep: instances 0-15 on 32^3, 16-47 on 48^3, 48-63 on 64^3
+    grids.  From one MT19937 stream seeded with `seed`, per instance in index order:
+    Ns = [100, 150, 200][floor(3u)], chiN = 12 + 18u, f_A = 0.2 + 0.3u, box = 3.5 + 1.5u
+    (four draws).  Fields: lamellar seed with noise N(0, 0.1^2) from seed + 1 + i.
+    Returns (instances, fields) for adscf_amd.Batch."""
+    st = MTStream(seed)
+    inst, w = [], []
+    for i in range(64):
+        u = st.grnd(4)
+        ns = [100, 150, 200][min(int(3 * u[0]), 2)]
+        chiN = 12.0 + 18.0 * u[1]
+        fa = 0.2 + 0.3 * u[2]
+        L = 3.5 + 1.5 * u[3]
+        n = 32 if i < 16 else (48 if i < 48 else 64)
+        segs = block_segments([fa, 1.0 - fa], ns)
+        inst.append(dict(grid=(n, n, n), box=(L, L, L), ds=1.0 / ns, blocks=[1, 2], segments=segs, kappa=25.0,
+                         chi=[[0.0, chiN], [chiN, 0.0]], vol=[1.0, 1.0]))
+        w.append(lamellar_seed(n, L, chiN, sigma=0.1, seed=seed + 1 + i))
+    return inst, w

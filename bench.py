@@ -54,9 +54,10 @@ def run_gpu(adscf_amd, fields, precision, steps, warmup, w0):
     st = ctx.scf_iterate(steps, tol=-1.0)  # synchronises the library stream on return
     t1 = time.perf_counter()
     ms_iter_ev, ms_sweep = ctx.scf_timing()
+    hipfft_us = ctx.hipfft_step_us(200)
     wall = t1 - t0
     out = dict(iters_per_s=steps / wall, ms_per_iter=1000.0 * wall / steps, ms_per_iter_events=ms_iter_ev,
-               ms_sweep=ms_sweep, iterations_run=st.iters, stopped_early=st.converged)
+               ms_sweep=ms_sweep, iterations_run=st.iters, stopped_early=st.converged, hipfft_step_us=hipfft_us)
     return ctx, out
 
 
@@ -94,6 +95,31 @@ def converge(adscf_amd, fields, w0, max_iter):
     return res
 
 
+def run_batched(adscf_amd, fields, steps, warmup):
+    """The 64-instance parameter sweep (32^3/48^3/64^3 grids, Ns in {100,150,200}), FP32-mixed,
+    one graph replay per SCF iteration of all instances, schedules from the batch autotuner."""
+    inst, w = fields.batched_instances(seed=2007)
+    with adscf_amd.Batch(inst, mixing_dim=M, lam_mix=LAM, precision=adscf_amd.FP32) as b:
+        b.set_fields(w)
+        report = b.autotune(reps=2)
+        b.set_fields(w)
+        b.iterate(warmup, tol=-1.0)
+        t0 = time.perf_counter()
+        it, acc, conv = b.iterate(steps, tol=-1.0)
+        wall = time.perf_counter() - t0
+        ms_iter_ev, ms_sweep = b.timing()
+    ips = steps / wall
+    bytes_iter = sum(algorithmic_bytes_iter(d["grid"][0] ** 3, sum(d["segments"]), M, 4) for d in inst)
+    sweep_bytes = sum(2 * sum(d["segments"]) * 3 * d["grid"][0] ** 3 * 4 for d in inst)
+    return dict(instances=len(inst), batch_iters_per_s=ips, instance_iters_per_s=ips * len(inst),
+                ms_per_batch_iter=1000.0 * wall / steps,
+                contour_steps_per_s=ips * sum(2 * sum(d["segments"]) for d in inst),
+                iteration_bytes_frac=bytes_iter * ips / (HBM_PEAK_GBS * 1e9),
+                sweep_roofline_frac=sweep_bytes / (ms_sweep * 1e-3) / (HBM_PEAK_GBS * 1e9),
+                timing_valid=not bool(conv.any()),
+                schedules=[l for l in report.splitlines() if "chosen" in l])
+
+
 def cpu_baseline(fields, w0):
     """The CPU FP64 oracle (single thread) on one SCF iteration of the same workload."""
     import oracle
@@ -118,6 +144,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--converge-max-iter", type=int, default=4000)
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the SCFT path on a single MI355X; "
@@ -169,6 +196,8 @@ def main():
         "contour_steps_per_s": r32["iters_per_s"] * 2 * NS,
         "timing_valid": not (r32["stopped_early"] or r64["stopped_early"]),
     }
+    if not args.no_batched:
+        line["batched"] = run_batched(adscf_amd, fields, max(5, args.steps // 5), 2)
     if not args.no_converge:
         line["convergence"] = converge(adscf_amd, fields, w0, args.converge_max_iter)
     if not args.no_cpu_baseline:

@@ -177,6 +177,12 @@ int adscf_batch_timing(adscf_batch* batch, double* ms_per_iter, double* ms_sweep
 /* picks the register-resident shape per distinct axis length by timing the batched sweep */
 int adscf_batch_autotune(adscf_batch* batch, int reps, char* report, int report_cap);
 
+/* ---- comparator (no counterpart in the reference) ---------------------------- */
+/* Mean time in microseconds of one single-chain contour step built like the reference's
+ * Propagate_Step (E*q, r2c, K/Ng, c2r, E*q) with hipFFT plans on this context's grid and
+ * precision.  For timing comparisons only; the library's own path does not use hipFFT. */
+int adscf_hipfft_step_time(adscf_ctx* ctx, int reps, double* us_per_step);
+
 /* ---- adjoint (no counterpart in the reference) ---------------------------- */
 /* Reverse-mode derivative of one canonical density evaluation of the chain
  * (mod_multiblock.f90:112-188 with Calculate_Energy, main_multiblock.f90:548-557) for

@@ -1,0 +1,69 @@
+"""Metric-size (64^3 x 200) checks through size-independent properties, plus a
+64^3 oracle comparison with a short contour.  Workload: box 4.5, ds = 0.005,
+blocks 60/140, seeded cylinder fields with chiN = 20."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N, L, NS = 64, 4.5, 200
+SEGS = [60, 140]
+
+
+@pytest.fixture(scope="module")
+def setup():
+    import adscf_amd
+    from adscf_amd import fields
+
+    w = fields.cylinder_seed(N, L, 20.0, sigma=0.2, seed=2007)
+    return adscf_amd, w
+
+
+def _E(w, ds):
+    return np.exp(-0.5 * ds * w)
+
+
+def test_mass_reversal_translation_fp64(setup):
+    amd, w = setup
+    ds = 1.0 / NS
+    with amd.Context(N, N, N, L, L, L, ds) as ctx:
+        vol, dvol = ctx.volume()
+        E = _E(w, ds)
+        a = ctx.propagate_chain([1, 2], SEGS, E, ensemble=1, rho=1.0)
+        # canonical mass: dvol * sum(density) = rho V ds Ns (trapezoid weights sum to ds*Ns)
+        assert abs(dvol * a.density.sum() - vol) < 1e-11 * vol
+        # reversing the block sequence exchanges q and q+
+        b = ctx.propagate_chain([2, 1], SEGS[::-1], E, ensemble=1, rho=1.0)
+        assert abs(a.Qsum - b.Qsum) < 1e-12 * a.Qsum
+        assert np.max(np.abs(a.density - b.density[::-1])) < 1e-11 * np.max(a.density)
+        # translating the fields translates the densities
+        sh = (3, 5, 7)
+        c = ctx.propagate_chain([1, 2], SEGS, np.roll(E, sh, axis=(1, 2, 3)), ensemble=1, rho=1.0)
+        assert abs(a.Qsum - c.Qsum) < 1e-12 * a.Qsum
+        assert np.max(np.abs(np.roll(a.density, sh, axis=(1, 2, 3)) - c.density)) < 1e-11 * np.max(a.density)
+
+
+def test_metric_grid_against_oracle_short_contour(setup, orc):
+    amd, w = setup
+    ds = 1.0 / NS
+    g = orc.Grid(N, N, N, L, L, L, ds)
+    E = _E(w, ds)
+    ref = orc.propagate_chain(g, [1, 2], [3, 7], E)
+    with amd.Context(N, N, N, L, L, L, ds) as ctx:
+        got = ctx.propagate_chain([1, 2], [3, 7], E)
+    assert abs(got.Qsum - ref.Qsum) < 1e-10 * ref.Qsum
+    assert np.max(np.abs(got.density - ref.density)) < 1e-10 * np.max(ref.density)
+
+
+def test_metric_fp32_error(setup):
+    amd, w = setup
+    ds = 1.0 / NS
+    E = _E(w, ds)
+    with amd.Context(N, N, N, L, L, L, ds) as ctx:
+        r64 = ctx.propagate_chain([1, 2], SEGS, E)
+    with amd.Context(N, N, N, L, L, L, ds, precision=amd.FP32) as ctx:
+        r32 = ctx.propagate_chain([1, 2], SEGS, E)
+    eq = abs(r32.Qsum - r64.Qsum) / r64.Qsum
+    ed = np.max(np.abs(r32.density - r64.density)) / np.max(r64.density)
+    print(f"64^3 x 200 FP32-mixed vs FP64: Q {eq:.2e}, phi {ed:.2e}")
+    assert eq < 1e-5 and ed < 1e-4
