@@ -17,6 +17,9 @@ FLAGS = [
     "-shared",
     # packed FP32 arithmetic is kept off for the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # fuse a*b+c only inside one source expression: every instantiation of a pass
+    # (single-instance, fixed-variant and batched kernels) then rounds identically
+    "-ffp-contract=on",
 ]
 
 

@@ -106,7 +106,7 @@ struct adscf_ctx {
   FHDesc fh{};
   bool fh_set = false;
   DevBuf<double> wring, dring, phi, dens_b, dist_b, io, io2;
-  DevBuf<double> qpart, rpart, spart, e2part, gpart, epart, resid;
+  DevBuf<double> qpart, rpart, spart, e2part, gpart, gsum, epart, resid;
   DevBuf<int> e1, e2, d1, d2;
   DevBuf<ScfState> st, cst;
   // SCF configuration
@@ -154,6 +154,34 @@ int guarded(adscf_ctx* c, F&& f) {
 int grid_for(long n) {
   long b = (n + kThreads - 1) / kThreads;
   return (int)std::max<long>(1, std::min<long>(b, kRedBlocks));
+}
+
+// contour kernel: 16-byte point packs when ng allows; returns the blocks launched
+// (one rpart/spart slot each)
+template <typename T>
+int launch_contour(hipStream_t stream, const ContourArgs<T>& a) {
+  constexpr int VP = 16 / sizeof(T);
+  if (a.ng % VP == 0) {
+    const int nb = grid_for(a.ng / VP);
+    hipLaunchKernelGGL((k_contour<T, VP>), dim3(nb), dim3(kThreads), 0, stream, a);
+    return nb;
+  }
+  const int nb = grid_for(a.ng);
+  hipLaunchKernelGGL((k_contour<T, 1>), dim3(nb), dim3(kThreads), 0, stream, a);
+  return nb;
+}
+
+// Anderson Gram sums of a history ring (m+1 slots of nvar doubles): partials over up to
+// kGramBlocks blocks, then one block per pair; gsum receives the (m+1)(m+2)/2 sums.
+void launch_gram(hipStream_t stream, const double* ring, long nvar, int m, const ScfState* st, int force,
+                 double* part, double* gsum, const int* done) {
+  const long ntile = (nvar + kGramTile - 1) / kGramTile;
+  const int nblk = (int)std::max<long>(1, std::min<long>(ntile, kGramBlocks));
+  const int np = (m + 1) * (m + 2) / 2;
+  const size_t gsm = (size_t)(m + 1) * (kGramTile + 1) * sizeof(double);
+  hipLaunchKernelGGL(k_gram, dim3(nblk), dim3(kThreads), gsm, stream, ring, nvar, m, st, force, part, done);
+  hipLaunchKernelGGL(k_gram_reduce, dim3(np), dim3(kThreads), 0, stream, (const double*)part, nblk, m, st, force,
+                     gsum, done);
 }
 
 // Wave number of FFT bin k on an axis of n points and length len, folded as
@@ -289,7 +317,16 @@ int launch_reg(adscf_ctx* c, PassArgs<T> p, int waves) {
   int threads;
   size_t smem;
   const int items = prep_reg<T>(c, p, N1, N2, waves, &threads, &smem);
-  hipLaunchKernelGGL((k_line_pass_reg<T, N1, N2>), dim3(items), dim3(threads), smem, c->stream, p);
+  const dim3 grid(items), block(threads);
+  switch (pass_mode(p)) {
+    case PM_FIRST: hipLaunchKernelGGL((k_line_pass_reg<T, N1, N2, PM_FIRST>), grid, block, smem, c->stream, p); break;
+    case PM_PLAIN: hipLaunchKernelGGL((k_line_pass_reg<T, N1, N2, PM_PLAIN>), grid, block, smem, c->stream, p); break;
+    case PM_MERGED:
+      hipLaunchKernelGGL((k_line_pass_reg<T, N1, N2, PM_MERGED>), grid, block, smem, c->stream, p);
+      break;
+    case PM_LAST: hipLaunchKernelGGL((k_line_pass_reg<T, N1, N2, PM_LAST>), grid, block, smem, c->stream, p); break;
+    default: hipLaunchKernelGGL((k_line_pass_reg<T, N1, N2, PM_GENERIC>), grid, block, smem, c->stream, p);
+  }
   HIPCK(hipGetLastError());
   return items;
 }
@@ -556,7 +593,7 @@ int propagate_chain_impl(adscf_ctx* c, const ChainDesc& cd, double* rho, double*
   hipLaunchKernelGGL(k_convert<T>, dim3(grid_for((long)nE)), dim3(kThreads), 0, c->stream, c->io.p, En.E.p, (long)nE);
   const int nb = grid_for(ng);
   hipLaunchKernelGGL(k_e2<T>, dim3(nb), dim3(kThreads), 0, c->stream, En.E.p, ng, c->nt, c->e2part.p);
-  hipLaunchKernelGGL(k_schedule, dim3(1), dim3(64), 0, c->stream, cd, c->e2part.p, nb, ng, c->nt, c->e1.p, c->e2.p,
+  hipLaunchKernelGGL(k_schedule, dim3(1), dim3(kThreads), 0, c->stream, cd, c->e2part.p, nb, ng, c->nt, c->e1.p, c->e2.p,
                      c->d1.p, c->d2.p, (ScfState*)nullptr, 0, 0, (const int*)nullptr);
   fill_ones_slot0<T>(c);
   ScfState init{};
@@ -570,12 +607,12 @@ int propagate_chain_impl(adscf_ctx* c, const ChainDesc& cd, double* rho, double*
   a.dist = c->dist_b.p;
   a.dens = c->dens_b.p;
   a.rpart = nullptr;
-  hipLaunchKernelGGL(k_contour<T>, dim3(nb), dim3(kThreads), 0, c->stream, a);
+  const int nbc = launch_contour<T>(c->stream, a);
   HIPCK(hipGetLastError());
   ScfState st{};
-  std::vector<double> sp(nb);
+  std::vector<double> sp(nbc);
   HIPCK(hipMemcpyAsync(&st, c->cst.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
-  HIPCK(hipMemcpyAsync(sp.data(), c->spart.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipMemcpyAsync(sp.data(), c->spart.p, nbc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if (distribution)
     HIPCK(hipMemcpyAsync(distribution, c->dist_b.p, nbl * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if (density) HIPCK(hipMemcpyAsync(density, c->dens_b.p, nbl * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -586,7 +623,7 @@ int propagate_chain_impl(adscf_ctx* c, const ChainDesc& cd, double* rho, double*
     *energy = st.energy;
   } else {
     double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += sp[b];
+    for (int b = 0; b < nbc; ++b) s += sp[b];
     *rho = s * c->dvol / c->volume / (c->ds * cd.ns);
     *energy = -*rho * c->volume;
   }
@@ -783,22 +820,17 @@ void enqueue_iteration(adscf_ctx* c, bool passes_too = true) {
   a.dring = c->dring.p;
   a.done = done;
   a.w0 = c->wring.p;  // ring base; the kernel reads slot `head` from the device state
-  hipLaunchKernelGGL(k_contour<T>, dim3(nb), dim3(kThreads), 0, c->stream, a);
+  const int nbc = launch_contour<T>(c->stream, a);
   HIPCK(hipGetLastError());
   const int m = c->m;
-  const int np = (m + 1) * (m + 2) / 2;
-  const int ngram = grid_for(nvar / kGramTile + 1);
-  const size_t gsm = (size_t)(m + 1) * (kGramTile + 1) * sizeof(double);
-  hipLaunchKernelGGL(k_gram, dim3(ngram), dim3(kThreads), gsm, c->stream, c->dring.p, nvar, m,
-                     (const ScfState*)c->st.p, 0, c->gpart.p, done);
+  launch_gram(c->stream, c->dring.p, nvar, m, (const ScfState*)c->st.p, 0, c->gpart.p, c->gsum.p, done);
   SolveArgs s{};
   s.m = m;
   s.lam_mix = c->lam_mix;
-  s.nblk_contour = nb;
+  s.nblk_contour = nbc;
   s.rpart = c->rpart.p;
   s.spart = c->spart.p;
-  s.nblk_gram = ngram;
-  s.gpart = c->gpart.p;
+  s.gsum = c->gsum.p;
   s.resid_hist = c->resid.p;
   s.resid_cap = (int)c->resid.n;
   s.dvol = c->dvol;
@@ -810,10 +842,9 @@ void enqueue_iteration(adscf_ctx* c, bool passes_too = true) {
   hipLaunchKernelGGL(k_solve, dim3(1), dim3(kThreads), 0, c->stream, s);
   hipLaunchKernelGGL(k_mix<T>, dim3(nb), dim3(kThreads), 0, c->stream, c->wring.p, (const double*)c->dring.p, ng,
                      c->nt, m, (const ScfState*)c->st.p, c->ds, En.E.p, c->e2part.p);
-  hipLaunchKernelGGL(k_schedule, dim3(1), dim3(64), 0, c->stream, c->cd, (const double*)c->e2part.p, nb, ng, c->nt,
+  hipLaunchKernelGGL(k_schedule, dim3(1), dim3(kThreads), 0, c->stream, c->cd, (const double*)c->e2part.p, nb, ng, c->nt,
                      c->e1.p, c->e2.p, c->d1.p, c->d2.p, c->st.p, 1, m, done);
   HIPCK(hipGetLastError());
-  (void)np;
 }
 
 template <typename T>
@@ -826,7 +857,7 @@ void enqueue_evaluate(adscf_ctx* c) {
   a.phi = c->phi.p;
   a.dring = nullptr;
   a.done = nullptr;
-  hipLaunchKernelGGL(k_contour<T>, dim3(grid_for(ng)), dim3(kThreads), 0, c->stream, a);
+  launch_contour<T>(c->stream, a);
   HIPCK(hipGetLastError());
 }
 
@@ -841,7 +872,8 @@ void scf_set_field_impl(adscf_ctx* c, const double* field) {
   c->wring.ensure((size_t)(m + 1) * nvar);
   c->dring.ensure((size_t)(m + 1) * nvar);
   c->phi.ensure(nvar);
-  c->gpart.ensure((size_t)kRedBlocks * (m + 1) * (m + 2) / 2);
+  c->gpart.ensure((size_t)kGramBlocks * (m + 1) * (m + 2) / 2);
+  c->gsum.ensure((size_t)(m + 1) * (m + 2) / 2);
   c->resid.ensure(200000);
   HIPCK(hipMemsetAsync(c->wring.p, 0, sizeof(double) * (m + 1) * nvar, c->stream));
   HIPCK(hipMemsetAsync(c->dring.p, 0, sizeof(double) * (m + 1) * nvar, c->stream));
@@ -853,7 +885,7 @@ void scf_set_field_impl(adscf_ctx* c, const double* field) {
   const int nb = grid_for(ng);
   hipLaunchKernelGGL(k_expfield<T>, dim3(nb), dim3(kThreads), 0, c->stream, (const double*)c->wring.p, En.E.p, ng,
                      c->nt, c->ds, c->e2part.p, (const int*)nullptr);
-  hipLaunchKernelGGL(k_schedule, dim3(1), dim3(64), 0, c->stream, c->cd, (const double*)c->e2part.p, nb, ng, c->nt,
+  hipLaunchKernelGGL(k_schedule, dim3(1), dim3(kThreads), 0, c->stream, c->cd, (const double*)c->e2part.p, nb, ng, c->nt,
                      c->e1.p, c->e2.p, c->d1.p, c->d2.p, (ScfState*)nullptr, 0, m, (const int*)nullptr);
   HIPCK(hipGetLastError());
   HIPCK(hipStreamSynchronize(c->stream));
@@ -864,7 +896,7 @@ std::vector<const void*> graph_key(adscf_ctx* c) {
   Engine<T>& En = eng<T>(c);
   return {En.E.p, En.tmp.p, En.h1.p, En.h2.p, En.tw[0].p, En.tw[1].p, En.tw[2].p, En.kf[0].p, En.kf[1].p,
           En.kf[2].p, c->wring.p, c->dring.p, c->phi.p, c->qpart.p, c->rpart.p, c->spart.p, c->e2part.p,
-          c->gpart.p, c->resid.p, c->e1.p, c->e2.p, c->d1.p, c->d2.p, c->st.p};
+          c->gpart.p, c->gsum.p, c->resid.p, c->e1.p, c->e2.p, c->d1.p, c->d2.p, c->st.p};
 }
 
 // E and the rescale schedule of the current field in precision T (the history is kept)
@@ -883,7 +915,7 @@ void refresh_E(adscf_ctx* c) {
   hipLaunchKernelGGL(k_expfield<T>, dim3(nb), dim3(kThreads), 0, c->stream,
                      (const double*)(c->wring.p + (size_t)st.head * ng * c->nt), En.E.p, ng, c->nt, c->ds,
                      c->e2part.p, (const int*)nullptr);
-  hipLaunchKernelGGL(k_schedule, dim3(1), dim3(64), 0, c->stream, c->cd, (const double*)c->e2part.p, nb, ng, c->nt,
+  hipLaunchKernelGGL(k_schedule, dim3(1), dim3(kThreads), 0, c->stream, c->cd, (const double*)c->e2part.p, nb, ng, c->nt,
                      c->e1.p, c->e2.p, c->d1.p, c->d2.p, (ScfState*)nullptr, 0, c->m, (const int*)nullptr);
   HIPCK(hipGetLastError());
   HIPCK(hipStreamSynchronize(c->stream));
@@ -1437,11 +1469,14 @@ int adjoint_impl(adscf_ctx* c, const ChainDesc& cd, double rho, const double* fi
 template <typename T>
 struct BatchTables {
   DevBuf<PassArgs<T>> args;
-  DevBuf<int> start, shape;
-  std::vector<size_t> arg_off, start_off;
+  DevBuf<int> blk, shape;
+  std::vector<size_t> arg_off, blk_off;
   std::vector<int> count, blocks;
   size_t smem = 0;
 };
+
+// instance slots are packed above kBatchItemBits in a signed 32-bit block code
+constexpr int kMaxBatchInstances = 1 << (31 - kBatchItemBits);
 
 int batch_shape_code(const AxisTables& ax) { return ax.N1 * 100 + ax.N2; }
 
@@ -1520,32 +1555,32 @@ void batch_build_tables(adscf_batch* b) {
     shapes.push_back(std::move(sh));
   }
   std::vector<PassArgs<T>> args;
-  std::vector<int> start, shape;
+  std::vector<int> blk, shape;
   tb.arg_off.clear();
-  tb.start_off.clear();
+  tb.blk_off.clear();
   tb.count.clear();
   tb.blocks.clear();
   for (size_t p = 0; p < npass; ++p) {
     tb.arg_off.push_back(args.size());
-    tb.start_off.push_back(start.size());
+    tb.blk_off.push_back(blk.size());
     int blocks = 0, cnt = 0;
     for (size_t i = 0; i < per.size(); ++i) {
       if (p >= per[i].size()) continue;
+      if (items[i][p] >= (1 << kBatchItemBits)) throw ArgError{"batched pass has too many blocks per instance"};
       args.push_back(per[i][p]);
-      start.push_back(blocks);
       shape.push_back(shapes[i][p]);
+      for (int it = 0; it < items[i][p]; ++it) blk.push_back((cnt << kBatchItemBits) | it);
       blocks += items[i][p];
       ++cnt;
     }
-    start.push_back(blocks);
     tb.count.push_back(cnt);
     tb.blocks.push_back(blocks);
   }
   tb.args.ensure(args.size());
-  tb.start.ensure(start.size());
+  tb.blk.ensure(blk.size());
   tb.shape.ensure(shape.size());
   HIPCK(hipMemcpy(tb.args.p, args.data(), args.size() * sizeof(PassArgs<T>), hipMemcpyHostToDevice));
-  HIPCK(hipMemcpy(tb.start.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(tb.blk.p, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(tb.shape.p, shape.data(), shape.size() * sizeof(int), hipMemcpyHostToDevice));
 }
 
@@ -1553,7 +1588,7 @@ template <typename T>
 void batch_enqueue_sweep(adscf_batch* b) {
   BatchTables<T>& tb = btab<T>(b);
   for (size_t p = 0; p < tb.count.size(); ++p) {
-    BatchPass<T> bp{tb.args.p + tb.arg_off[p], tb.start.p + tb.start_off[p], tb.shape.p + tb.arg_off[p],
+    BatchPass<T> bp{tb.args.p + tb.arg_off[p], tb.blk.p + tb.blk_off[p], tb.shape.p + tb.arg_off[p],
                     tb.count[p]};
     hipLaunchKernelGGL(k_line_pass_batch<T>, dim3(tb.blocks[p]), dim3(64), tb.smem, b->stream, bp);
   }
@@ -1946,24 +1981,19 @@ int adscf_mixing_lambda(adscf_ctx* c, double* field, double* dfield, long fields
       HIPCK(hipMemcpyAsync(c->io2.p, dfield, 2 * fieldsize * sizeof(double), hipMemcpyHostToDevice, c->stream));
       // the m = 1 Gram kernel over the two-slot rings gives <D0-D1, D0-D1>; the field
       // differences use the same kernel on the field ring
-      c->gpart.ensure((size_t)kRedBlocks * 3);
+      c->gpart.ensure((size_t)kGramBlocks * 3);
+      c->gsum.ensure(3);
       ScfState st{};
       HIPCK(hipMemcpyAsync(c->cst.p, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
-      const int ng2 = grid_for(fieldsize / kGramTile + 1);
-      const size_t gsm = 2 * (kGramTile + 1) * sizeof(double);
       double sums[2];
       for (int which = 0; which < 2; ++which) {
-        hipLaunchKernelGGL(k_gram, dim3(ng2), dim3(kThreads), gsm, c->stream,
-                           (const double*)(which == 0 ? c->io.p : c->io2.p), fieldsize, 1,
-                           (const ScfState*)c->cst.p, 1, c->gpart.p, (const int*)nullptr);
+        launch_gram(c->stream, (const double*)(which == 0 ? c->io.p : c->io2.p), fieldsize, 1,
+                    (const ScfState*)c->cst.p, 1, c->gpart.p, c->gsum.p, nullptr);
         HIPCK(hipGetLastError());
-        std::vector<double> part((size_t)ng2 * 3);
-        HIPCK(hipMemcpyAsync(part.data(), c->gpart.p, part.size() * sizeof(double), hipMemcpyDeviceToHost,
-                             c->stream));
+        double g3[3];
+        HIPCK(hipMemcpyAsync(g3, c->gsum.p, sizeof g3, hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
-        double s = 0.0;
-        for (int b = 0; b < ng2; ++b) s += part[(size_t)b * 3 + 2];  // pair (1,1)
-        sums[which] = s;
+        sums[which] = g3[2];  // pair (1,1)
       }
       const double num = sums[0], nom = sums[1];
       if (nom > 1.0e-50) {
@@ -2007,19 +2037,16 @@ int adscf_mixing_anderson(adscf_ctx* c, double* field, double* dfield, long fiel
     ScfState st{};
     HIPCK(hipMemcpyAsync(c->cst.p, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
     const int np = (m + 1) * (m + 2) / 2;
-    const int ngram = grid_for(fieldsize / kGramTile + 1);
-    c->gpart.ensure((size_t)kRedBlocks * np);
+    c->gpart.ensure((size_t)kGramBlocks * np);
+    c->gsum.ensure(np);
     const int status = iter < 1000 ? iter : 1000;
-    if (status > m) {
-      const size_t gsm = (size_t)(m + 1) * (kGramTile + 1) * sizeof(double);
-      hipLaunchKernelGGL(k_gram, dim3(ngram), dim3(kThreads), gsm, c->stream, (const double*)c->io2.p, fieldsize, m,
-                         (const ScfState*)c->cst.p, 1, c->gpart.p, (const int*)nullptr);
-    }
+    if (status > m)
+      launch_gram(c->stream, (const double*)c->io2.p, fieldsize, m, (const ScfState*)c->cst.p, 1, c->gpart.p,
+                  c->gsum.p, nullptr);
     SolveArgs s{};
     s.m = m;
     s.lam_mix = lam_mix;
-    s.nblk_gram = ngram;
-    s.gpart = c->gpart.p;
+    s.gsum = c->gsum.p;
     s.iter_override = iter;
     s.st = c->cst.p;
     hipLaunchKernelGGL(k_solve, dim3(1), dim3(kThreads), 0, c->stream, s);
@@ -2304,8 +2331,8 @@ int adscf_batch_create(adscf_batch** out, int ninst, const int* grid, const doub
                        const int* nblocks, const int* monomer_type, const int* block_segments, const double* kappa,
                        const double* chi, const double* vol, int ensemble, double rho, int mixing_dim,
                        double lam_mix, int precision) {
-  if (!out || ninst < 1 || !grid || !box || !ds || !nblocks || !monomer_type || !block_segments || !kappa || !chi ||
-      !vol)
+  if (!out || ninst < 1 || ninst > kMaxBatchInstances || !grid || !box || !ds || !nblocks || !monomer_type ||
+      !block_segments || !kappa || !chi || !vol)
     return ADSCF_ERR_ARG;
   *out = nullptr;
   auto* b = new (std::nothrow) adscf_batch();

@@ -120,34 +120,60 @@ struct FourStep {
   static constexpr int lds_elems = N2 * n1p;     // per line
 };
 
+// Per-thread twiddles and multipliers of one line operator, loaded once per
+// pass so that their memory latency overlaps the loads of the data.
+//   ta[k2] = W_n^(t*k2)       phase A, thread t < N1
+//   tb[j1] = W_n^(j1*t)       phase B, thread t < N2
+//   k[k1]  = K[t + N2*k1]     phase B; with N1 == 1, k[k] = K[k] for the whole line
+template <typename T, int N1, int N2>
+struct RegTables {
+  static constexpr int NK = N1 > 1 ? N1 : N2;
+  C2<T> ta[N1 > 1 ? N2 : 1];
+  C2<T> tb[N1];
+  T k[NK];
+  ADSCF_HD void load(int t, const C2<T>* tw, const T* kf) {
+    if constexpr (N1 == 1) {
+#pragma unroll
+      for (int i = 0; i < N2; ++i) k[i] = kf[i];
+    } else {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) ta[k2] = t < N1 ? tw[t * k2] : C2<T>{T(1), T(0)};
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) tb[j1] = t < N2 ? tw[j1 * t] : C2<T>{T(1), T(0)};
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) k[k1] = t < N2 ? kf[t + N2 * k1] : T(0);
+    }
+  }
+};
+
 // phase A: thread t < N1 holds v[j2] = x[t + N1*j2]
 template <typename T, int N1, int N2>
-ADSCF_HD void fs_phase_a(C2<T>* v, C2<T>* lds, int t, const C2<T>* tw) {
+ADSCF_HD void fs_phase_a(C2<T>* v, C2<T>* lds, int t, const RegTables<T, N1, N2>& R) {
   using FS = FourStep<N1, N2>;
   dft_reg<T, N2, -1>(v);
 #pragma unroll
   for (int k2 = 0; k2 < N2; ++k2) {
     C2<T> y = v[k2];
-    if (k2 > 0 && t > 0) y = cmul(y, tw[t * k2]);
+    if (k2 > 0 && t > 0) y = cmul(y, R.ta[k2]);
     lds[t + FS::n1p * k2] = y;
   }
 }
 
 // phase B: thread t < N2 (as k2)
 template <typename T, int N1, int N2>
-ADSCF_HD void fs_phase_b(C2<T>* lds, int t, const C2<T>* tw, const T* kf) {
+ADSCF_HD void fs_phase_b(C2<T>* lds, int t, const RegTables<T, N1, N2>& R) {
   using FS = FourStep<N1, N2>;
   C2<T> u[N1];
 #pragma unroll
   for (int j1 = 0; j1 < N1; ++j1) u[j1] = lds[j1 + FS::n1p * t];
   dft_reg<T, N1, -1>(u);
 #pragma unroll
-  for (int k1 = 0; k1 < N1; ++k1) u[k1] = cscale(u[k1], kf[t + N2 * k1]);
+  for (int k1 = 0; k1 < N1; ++k1) u[k1] = cscale(u[k1], R.k[k1]);
   dft_reg<T, N1, +1>(u);
 #pragma unroll
   for (int j1 = 0; j1 < N1; ++j1) {
     C2<T> z = u[j1];
-    if (j1 > 0 && t > 0) z = cmulc(z, tw[j1 * t]);
+    if (j1 > 0 && t > 0) z = cmulc(z, R.tb[j1]);
     lds[j1 + FS::n1p * t] = z;
   }
 }
@@ -163,10 +189,10 @@ ADSCF_HD void fs_phase_c(C2<T>* v, const C2<T>* lds, int t) {
 
 // whole line in one thread (N1 == 1): forward DFT, K, inverse DFT
 template <typename T, int N>
-ADSCF_HD void line_op_single(C2<T>* v, const T* kf) {
+ADSCF_HD void line_op_single(C2<T>* v, const RegTables<T, 1, N>& R) {
   dft_reg<T, N, -1>(v);
 #pragma unroll
-  for (int k = 0; k < N; ++k) v[k] = cscale(v[k], kf[k]);
+  for (int k = 0; k < N; ++k) v[k] = cscale(v[k], R.k[k]);
   dft_reg<T, N, +1>(v);
 }
 

@@ -100,6 +100,9 @@ __device__ __forceinline__ void block_dot_out(double acc, double* out) {
   if (tid == 0) out[blockIdx.x] = red[0];
 }
 
+// read in place of PassArgs::done when a pass has no flag
+__device__ const int g_no_stop = 0;
+
 template <typename T>
 __device__ __forceinline__ long pass_line_base(const PassArgs<T>& a, int item, int l, bool& valid) {
   if (a.axis == 0) {
@@ -248,13 +251,13 @@ __global__ void __launch_bounds__(kThreads) k_line_pass(PassArgs<T> a) {
 // register-resident line pass for n = N1 * N2 (fft_reg.hpp)
 // ---------------------------------------------------------------------------
 template <typename T, int N1, int N2>
-__device__ __forceinline__ void reg_line_op(C2<T> (&v)[N2], C2<T>* lds, int t, const PassArgs<T>& a) {
+__device__ __forceinline__ void reg_line_op(C2<T> (&v)[N2], C2<T>* lds, int t, const RegTables<T, N1, N2>& R) {
   if constexpr (N1 == 1) {
-    line_op_single<T, N2>(v, a.kf);
+    line_op_single<T, N2>(v, R);
   } else {
-    if (t < N1) fs_phase_a<T, N1, N2>(v, lds, t, a.tw);
+    if (t < N1) fs_phase_a<T, N1, N2>(v, lds, t, R);
     __syncthreads();
-    if (t < N2) fs_phase_b<T, N1, N2>(lds, t, a.tw, a.kf);
+    if (t < N2) fs_phase_b<T, N1, N2>(lds, t, R);
     __syncthreads();
     if (t < N1) fs_phase_c<T, N1, N2>(v, lds, t);
   }
@@ -265,11 +268,43 @@ constexpr int reg_tpl() {
   return N1 == 1 ? 1 : FourStep<N1, N2>::tpl;
 }
 
+// Pass variants fixed at compile time.  The four shapes of a propagator sweep
+// (first pass from q = 1, plain single-operator pass, merged pass with the mid
+// store, last pass with planar output) get kernels without runtime branches on
+// the pass arguments, so that every argument load is issued at entry;
+// PM_GENERIC reads the kind fields at run time (adjoint, compat, batched).
+enum PassMode : int { PM_GENERIC = 0, PM_FIRST = 1, PM_PLAIN = 2, PM_MERGED = 3, PM_LAST = 4 };
+
+// which fixed variant a pass matches (host side); PM_GENERIC if none
+template <typename T>
+inline int pass_mode(const PassArgs<T>& p) {
+  const bool both = p.h2 != nullptr;
+  if (p.in_kind == IN_ONES && both && p.pre_E == 1 && p.nops == 1 && p.out_kind == OUT_TMP) return PM_FIRST;
+  if (p.in_kind == IN_TMP && p.pre_E == 0 && p.nops == 1 && p.out_kind == OUT_TMP) return PM_PLAIN;
+  if (p.in_kind == IN_TMP && p.pre_E == 0 && p.nops == 2 && p.out_kind == OUT_TMP && both && p.dexp1 && p.dexp2)
+    return PM_MERGED;
+  if (p.in_kind == IN_TMP && p.pre_E == 0 && p.nops == 1 && p.out_kind == OUT_PLANAR && p.post_E && both &&
+      p.dexp1 && p.dexp2)
+    return PM_LAST;
+  return PM_GENERIC;
+}
+
 // Same contract as k_line_pass; a line lives in registers of reg_tpl threads,
 // each holding the N2 points j1 + N1*j2 of its j1.  blockDim = reg_tpl * L.
-template <typename T, int N1, int N2>
+// At the small grids of the SCF loop a pass has few waves per SIMD, so its
+// time is mostly memory latency: every global load of the pass (the stop flag,
+// the lines, the E factors of both steps, twiddles, multipliers and rescale
+// exponents) is issued before the first use, and the pass pays one latency.
+template <typename T, int N1, int N2, int MODE = PM_GENERIC>
 __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, unsigned char* smem_raw) {
   using FS = FourStep<N1, N2>;
+  constexpr bool G = MODE == PM_GENERIC;
+  const int in_kind = G ? a.in_kind : (MODE == PM_FIRST ? (int)IN_ONES : (int)IN_TMP);
+  const int nops = G ? a.nops : (MODE == PM_MERGED ? 2 : 1);
+  const int out_kind = G ? a.out_kind : (MODE == PM_LAST ? (int)OUT_PLANAR : (int)OUT_TMP);
+  const int pre_E = G ? a.pre_E : (MODE == PM_FIRST ? 1 : 0);
+  const int post_E = G ? a.post_E : (MODE == PM_LAST ? 1 : 0);
+  const bool two = G ? a.h2 != nullptr : true;  // second chain present
   constexpr int TPL = reg_tpl<N1, N2>();
   const int tid = threadIdx.x;
   int l, t;
@@ -284,45 +319,76 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
   const long base = pass_line_base(a, item, l, valid);
   C2<T>* lds = reinterpret_cast<C2<T>*>(smem_raw) + l * FS::lds_elems;
   const bool act = t < N1 && valid;
+  const int stop = *(a.done ? a.done : &g_no_stop);
+  const bool useEa = pre_E != 0 || nops == 2 || out_kind == OUT_DOT || (out_kind == OUT_PLANAR && post_E);
+  const bool useEb = nops == 2;
   C2<T> v[N2];
+  T ea1[N2], ea2[N2], eb1[N2], eb2[N2];
 #pragma unroll
   for (int j2 = 0; j2 < N2; ++j2) {
     C2<T> x = {T(0), T(0)};
+    ea1[j2] = ea2[j2] = eb1[j2] = eb2[j2] = T(1);
     if (act) {
       const long g = base + (long)(t + N1 * j2) * a.stride;
-      if (a.in_kind == IN_TMP) {
+      if (in_kind == IN_TMP) {
         x = a.tmp_in[g];
-      } else if (a.in_kind == IN_ONES) {
-        x = {T(1), a.h2 ? T(1) : T(0)};
+      } else if (in_kind == IN_ONES) {
+        x = {T(1), two ? T(1) : T(0)};
       } else {
         x.x = a.q1_in[g];
         x.y = a.q2_in ? a.q2_in[g] : T(0);
       }
-      apply_pre_E(a, g, x);
+      if (useEa) {
+        ea1[j2] = a.Ea1[g];
+        ea2[j2] = a.Ea2[g];
+      }
+      if (useEb) {
+        eb1[j2] = a.Eb1[g];
+        eb2[j2] = a.Eb2[g];
+      }
     }
     v[j2] = x;
   }
-  reg_line_op<T, N1, N2>(v, lds, t, a);
-  if (a.nops == 2) {
-    const T sc1 = exp2i<T>(a.dexp1[a.sidx]);
-    const T sc2 = exp2i<T>(a.dexp2[a.sidx]);
+  RegTables<T, N1, N2> R;
+  R.load(t, a.tw, a.kf);
+  const bool scaled = G ? a.dexp1 != nullptr : (MODE == PM_MERGED || MODE == PM_LAST);
+  const T sc1 = scaled ? exp2i<T>(a.dexp1[a.sidx]) : T(1);
+  const T sc2 = (G ? a.dexp2 != nullptr : scaled) ? exp2i<T>(a.dexp2[a.sidx]) : T(1);
+  if (pre_E > 0) {
+#pragma unroll
+    for (int j2 = 0; j2 < N2; ++j2) {
+      v[j2].x *= ea1[j2];
+      v[j2].y *= ea2[j2];
+    }
+  } else if (pre_E < 0) {
+#pragma unroll
+    for (int j2 = 0; j2 < N2; ++j2) {
+      v[j2].x /= ea1[j2];
+      v[j2].y /= ea2[j2];
+    }
+  }
+  reg_line_op<T, N1, N2>(v, lds, t, R);
+  // The stop flag is tested only here, after the first operator: its barriers keep
+  // every load above in flight together, and nothing has been stored yet.
+  if (stop) return;  // uniform over the block
+  if (nops == 2) {
     if (act) {
 #pragma unroll
       for (int j2 = 0; j2 < N2; ++j2) {
         const long g = base + (long)(t + N1 * j2) * a.stride;
         C2<T> x = v[j2];
-        x.x = x.x * a.Ea1[g] * sc1;
-        x.y = x.y * a.Ea2[g] * sc2;
+        x.x = x.x * ea1[j2] * sc1;
+        x.y = x.y * ea2[j2] * sc2;
         a.h1[g] = x.x;
         a.h2[g] = x.y;
-        x.x *= a.Eb1[g];
-        x.y *= a.Eb2[g];
+        x.x *= eb1[j2];
+        x.y *= eb2[j2];
         v[j2] = x;
       }
     }
-    reg_line_op<T, N1, N2>(v, lds, t, a);
+    reg_line_op<T, N1, N2>(v, lds, t, R);
   }
-  if (a.out_kind == OUT_DOT) {
+  if (out_kind == OUT_DOT) {
     double acc = 0.0;
     if (act) {
 #pragma unroll
@@ -332,59 +398,64 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
     return;
   }
   if (!act) return;
-  if (a.out_kind == OUT_TMP) {
+  if (out_kind == OUT_TMP) {
 #pragma unroll
     for (int j2 = 0; j2 < N2; ++j2) a.tmp_out[base + (long)(t + N1 * j2) * a.stride] = v[j2];
   } else {
-    const T sc1 = a.dexp1 ? exp2i<T>(a.dexp1[a.sidx]) : T(1);
-    const T sc2 = a.dexp2 ? exp2i<T>(a.dexp2[a.sidx]) : T(1);
 #pragma unroll
     for (int j2 = 0; j2 < N2; ++j2) {
       const long g = base + (long)(t + N1 * j2) * a.stride;
       C2<T> x = v[j2];
-      if (a.post_E) {
-        x.x = x.x * a.Ea1[g] * sc1;
-        x.y = x.y * a.Ea2[g] * sc2;
+      if (post_E) {
+        x.x = x.x * ea1[j2] * sc1;
+        x.y = x.y * ea2[j2] * sc2;
       }
       a.h1[g] = x.x;
-      if (a.h2) a.h2[g] = x.y;
+      if (two) a.h2[g] = x.y;
     }
   }
 }
 
-template <typename T, int N1, int N2>
-__global__ void __launch_bounds__(kThreads) k_line_pass_reg(PassArgs<T> a) {
-  if (a.done && *a.done) return;
-  extern __shared__ unsigned char smem_raw[];
-  reg_pass_body<T, N1, N2>(a, blockIdx.x, smem_raw);
+// Makes every kernel-argument field the pass reads live in scalar registers at
+// entry, so the compiler issues their constant-cache loads as one batch rather
+// than one batch per branch of the pass body.  (The empty asm emits nothing.)
+template <typename T>
+__device__ __forceinline__ void pin_pass_args(const PassArgs<T>& a) {
+  asm volatile("" ::"s"(a.nx), "s"(a.ny), "s"(a.nz), "s"(a.axis), "s"(a.L), "s"(a.gx), "s"(a.stride), "s"(a.tw),
+               "s"(a.kf), "s"(a.tmp_in), "s"(a.Ea1), "s"(a.Ea2), "s"(a.Eb1), "s"(a.Eb2), "s"(a.h1), "s"(a.h2),
+               "s"(a.dexp1), "s"(a.dexp2), "s"(a.sidx), "s"(a.tmp_out), "s"(a.done));
 }
 
-// One pass of many independent instances in one launch: block b belongs to the
-// instance i with start[i] <= b < start[i+1]; every instance carries its own
-// arguments (grid, buffers, step types) and four-step shape.  Blocks are one
-// wavefront for every shape.
+template <typename T, int N1, int N2, int MODE>
+__global__ void __launch_bounds__(kThreads) k_line_pass_reg(PassArgs<T> a) {
+  extern __shared__ unsigned char smem_raw[];
+  if constexpr (MODE != PM_GENERIC) pin_pass_args(a);
+  reg_pass_body<T, N1, N2, MODE>(a, blockIdx.x, smem_raw);
+}
+
+// One pass of many independent instances in one launch: blk[b] packs the
+// instance slot of block b (bits 20 and up) and the block's item within that
+// instance (low 20 bits); every instance carries its own arguments (grid,
+// buffers, step types) and four-step shape.  Blocks are one wavefront for
+// every shape.
 template <typename T>
 struct BatchPass {
   const PassArgs<T>* args;
-  const int* start;  // n + 1 block offsets
+  const int* blk;    // one entry per block
   const int* shape;  // n codes N1 * 100 + N2
   int n;
 };
 
+constexpr int kBatchItemBits = 20;
+
 template <typename T>
 __global__ void __launch_bounds__(64) k_line_pass_batch(BatchPass<T> tb) {
-  int lo = 0, hi = tb.n - 1;
-  const int b = blockIdx.x;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) / 2;
-    if (tb.start[mid] <= b) lo = mid;
-    else hi = mid - 1;
-  }
-  const PassArgs<T> a = tb.args[lo];
-  if (a.done && *a.done) return;
+  const int code = tb.blk[blockIdx.x];
+  const int i = code >> kBatchItemBits;
+  const int item = code & ((1 << kBatchItemBits) - 1);
+  const PassArgs<T> a = tb.args[i];
   extern __shared__ unsigned char smem_raw[];
-  const int item = b - tb.start[lo];
-  switch (tb.shape[lo]) {
+  switch (tb.shape[i]) {
     case 408: reg_pass_body<T, 4, 8>(a, item, smem_raw); break;
     case 804: reg_pass_body<T, 8, 4>(a, item, smem_raw); break;
     case 216: reg_pass_body<T, 2, 16>(a, item, smem_raw); break;

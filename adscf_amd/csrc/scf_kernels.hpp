@@ -86,42 +86,59 @@ __global__ void __launch_bounds__(kThreads) k_e2(const T* __restrict__ E, long n
 // the stored q'(s) = q(s) * 2^e_c(s) near one.  Powers of two scale every
 // rounding step exactly, so the FP64 path computes the same bits as an
 // unscaled one; in FP32 the two packed chains stay balanced.
-__global__ void k_schedule(ChainDesc cd, const double* __restrict__ e2part, int nblk, long ng, int nt,
-                           int* __restrict__ e1, int* __restrict__ e2, int* __restrict__ d1, int* __restrict__ d2,
-                           ScfState* st, int rotate, int m, const int* done) {
+// One block of kThreads: the per-type means come from the E^2 partials, then
+// every thread fills contour indices s in closed form (the log2 sum is
+// piecewise linear in s within a block).
+__global__ void __launch_bounds__(kThreads) k_schedule(ChainDesc cd, const double* __restrict__ e2part, int nblk,
+                                                       long ng, int nt, int* __restrict__ e1, int* __restrict__ e2,
+                                                       int* __restrict__ d1, int* __restrict__ d2, ScfState* st,
+                                                       int rotate, int m, const int* done) {
   if (done && *done) return;
-  if (threadIdx.x != 0) return;
-  double lg[kMaxTypes];
+  __shared__ double sh[kThreads];
+  __shared__ double lg[kMaxTypes];
+  __shared__ double sig0[2][kMaxBlocks];  // log2 sum at the start of each block, chain order
+  __shared__ int s0c[2][kMaxBlocks];      // first contour index of each block, chain order
   for (int t = 0; t < nt; ++t) {
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += e2part[b * kMaxTypes + t];
-    const double mean = s / (double)ng;
-    lg[t] = (mean > 0.0 && mean < 1e300) ? log2(mean) : 0.0;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc += e2part[b * kMaxTypes + t];
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) {
+      const double mean = acc / (double)ng;
+      lg[t] = (mean > 0.0 && mean < 1e300) ? log2(mean) : 0.0;
+    }
   }
-  for (int c = 0; c < 2; ++c) {
-    int* ee = c == 0 ? e1 : e2;
-    int* dd = c == 0 ? d1 : d2;
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const int c = threadIdx.x;
     double sig = 0.0;
-    int prev = 0;
-    ee[0] = 0;
-    dd[0] = 0;
     int s = 0;
     for (int bb = 0; bb < cd.nblocks; ++bb) {
       const int b = c == 0 ? bb : cd.nblocks - 1 - bb;
-      for (int k = 0; k < cd.seg[b]; ++k) {
-        ++s;
-        sig += lg[cd.type[b]];
-        double r = -rint(sig);
-        if (r > 900.0) r = 900.0;
-        if (r < -900.0) r = -900.0;
-        const int e = (int)r;
-        ee[s] = e;
-        dd[s] = e - prev;
-        prev = e;
-      }
+      sig0[c][bb] = sig;
+      s0c[c][bb] = s;
+      sig += cd.seg[b] * lg[cd.type[b]];
+      s += cd.seg[b];
     }
   }
-  if (rotate && st) st->head = (st->head + m) % (m + 1);
+  __syncthreads();
+  auto expo = [&](int c, int s) -> int {
+    if (s == 0) return 0;
+    int bb = cd.nblocks - 1;
+    while (bb > 0 && s0c[c][bb] >= s) --bb;
+    const int b = c == 0 ? bb : cd.nblocks - 1 - bb;
+    double r = -rint(sig0[c][bb] + (s - s0c[c][bb]) * lg[cd.type[b]]);
+    if (r > 900.0) r = 900.0;
+    if (r < -900.0) r = -900.0;
+    return (int)r;
+  };
+  for (int s = threadIdx.x; s <= cd.ns; s += blockDim.x) {
+    const int a1 = expo(0, s), a2 = expo(1, s);
+    e1[s] = a1;
+    e2[s] = a2;
+    d1[s] = s == 0 ? 0 : a1 - expo(0, s - 1);
+    d2[s] = s == 0 ? 0 : a2 - expo(1, s - 1);
+  }
+  if (rotate && st && threadIdx.x == 0) st->head = (st->head + m) % (m + 1);
 }
 
 // partial sums of q'(Ns) for Qsum (mod_multiblock.f90:155)
@@ -186,68 +203,112 @@ struct ContourArgs {
   const int* done;
 };
 
+template <typename T, int VP>
+struct alignas(sizeof(T) * VP) PackT {
+  T v[VP];
+};
+
 // Trapezoid contour integral (mod_multiblock.f90:157-168) in the reference's
 // summation order, then the ensemble normalisation (173-180).  In SCF mode the
 // species densities (main_multiblock.f90:530-534), the mean fields
 // (mod_FloryHuggins.f90:104-129), dfield = new - field and max|dfield|
 // (main_multiblock.f90:674-676, 780) follow in the same pass.
-template <typename T>
+// Each thread owns VP consecutive points (16-byte loads; ng % VP == 0) and
+// loads kUnroll contour slices ahead of the sum, which keeps the stream of
+// slice reads deep enough to run at HBM rate.
+constexpr int kContourUnroll = 8;
+
+template <typename T, int VP>
 __global__ void __launch_bounds__(kThreads) k_contour(ContourArgs<T> a) {
   if (a.done && *a.done) return;
+  using P = PackT<T, VP>;
+  constexpr int U = kContourUnroll;
   __shared__ double sh[kThreads];
   const long ng = a.ng;
   const int ns = a.cd.ns;
   const double ds = a.ds, hds = 0.5 * a.ds;
   const double norm = a.st->norm;
   double rmax = 0.0, dsum = 0.0;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ng; i += (long)gridDim.x * blockDim.x) {
-    double phi[kMaxTypes];
-    for (int t = 0; t < a.fh.nt; ++t) phi[t] = 0.0;
+  auto ld = [&](const T* base, int s, long i0) { return *reinterpret_cast<const P*>(base + (long)s * ng + i0); };
+  auto scl = [&](int s) { return ldexp(1.0, -(a.e1[s] + a.e2[ns - s])); };
+  const long npk = ng / VP;
+  for (long ip = blockIdx.x * (long)blockDim.x + threadIdx.x; ip < npk; ip += (long)gridDim.x * blockDim.x) {
+    const long i0 = ip * VP;
+    double phi[kMaxTypes][VP];
+    for (int t = 0; t < a.fh.nt; ++t)
+#pragma unroll
+      for (int v = 0; v < VP; ++v) phi[t][v] = 0.0;
     for (int b = 0; b < a.cd.nblocks; ++b) {
       const int s0 = a.cd.start[b], nb = a.cd.seg[b];
-      double acc;
+      double acc[VP];
       {
-        const int s = s0;
-        acc = ((hds * (double)a.h1[(long)s * ng + i]) * (double)a.h2[(long)(ns - s) * ng + i]) *
-              ldexp(1.0, -(a.e1[s] + a.e2[ns - s]));
+        const P x1 = ld(a.h1, s0, i0), x2 = ld(a.h2, ns - s0, i0);
+        const double sc = scl(s0);
+#pragma unroll
+        for (int v = 0; v < VP; ++v) acc[v] = ((hds * (double)x1.v[v]) * (double)x2.v[v]) * sc;
       }
-      for (int s = s0 + 1; s <= s0 + nb - 1; ++s) {
-        const double t = ((ds * (double)a.h1[(long)s * ng + i]) * (double)a.h2[(long)(ns - s) * ng + i]) *
-                         ldexp(1.0, -(a.e1[s] + a.e2[ns - s]));
-        acc = acc + t;
+      int s = s0 + 1;
+      for (; s + U <= s0 + nb; s += U) {
+        P x1[U], x2[U];
+        double sc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          x1[u] = ld(a.h1, s + u, i0);
+          x2[u] = ld(a.h2, ns - s - u, i0);
+          sc[u] = scl(s + u);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int v = 0; v < VP; ++v) acc[v] = acc[v] + ((ds * (double)x1[u].v[v]) * (double)x2[u].v[v]) * sc[u];
+      }
+      for (; s <= s0 + nb - 1; ++s) {
+        const P x1 = ld(a.h1, s, i0), x2 = ld(a.h2, ns - s, i0);
+        const double sc = scl(s);
+#pragma unroll
+        for (int v = 0; v < VP; ++v) acc[v] = acc[v] + ((ds * (double)x1.v[v]) * (double)x2.v[v]) * sc;
       }
       {
-        const int s = s0 + nb;
-        const double t = ((hds * (double)a.h1[(long)s * ng + i]) * (double)a.h2[(long)(ns - s) * ng + i]) *
-                         ldexp(1.0, -(a.e1[s] + a.e2[ns - s]));
-        acc = acc + t;
+        const int se = s0 + nb;
+        const P x1 = ld(a.h1, se, i0), x2 = ld(a.h2, ns - se, i0);
+        const double sc = scl(se);
+#pragma unroll
+        for (int v = 0; v < VP; ++v) acc[v] = acc[v] + ((hds * (double)x1.v[v]) * (double)x2.v[v]) * sc;
       }
-      const double den = norm * acc;
-      dsum += den;
-      if (a.mode == CONTOUR_BLOCKS) {
-        if (a.dist) a.dist[(long)b * ng + i] = acc;
-        if (a.dens) a.dens[(long)b * ng + i] = den;
-      } else {
-        phi[a.cd.type[b]] += den;
+#pragma unroll
+      for (int v = 0; v < VP; ++v) {
+        const long i = i0 + v;
+        const double den = norm * acc[v];
+        dsum += den;
+        if (a.mode == CONTOUR_BLOCKS) {
+          if (a.dist) a.dist[(long)b * ng + i] = acc[v];
+          if (a.dens) a.dens[(long)b * ng + i] = den;
+        } else {
+          phi[a.cd.type[b]][v] += den;
+        }
       }
     }
     if (a.mode == CONTOUR_SCF) {
       const int nt = a.fh.nt;
-      double vf[kMaxTypes], tot = 0.0;
-      for (int t = 0; t < nt; ++t) {
-        a.phi[(long)t * ng + i] = phi[t];
-        vf[t] = phi[t] * a.fh.vol[t];
-      }
-      for (int t = 0; t < nt; ++t) tot += vf[t];
       double* d0 = a.dring ? a.dring + (long)a.st->head * nt * ng : nullptr;
       const double* w0 = a.w0 + (long)a.st->head * nt * ng;  // current field: ring slot `head`
-      for (int t = 0; t < nt; ++t) {
-        double f = a.fh.kappa * (tot - 1.0);
-        for (int t2 = 0; t2 < nt; ++t2) f = f + a.fh.chi[t + t2 * nt] * vf[t2];
-        f = f * a.fh.vol[t];
-        const double df = f - w0[(long)t * ng + i];
-        if (d0) d0[(long)t * ng + i] = df;
-        rmax = nanmax(rmax, fabs(df));
+#pragma unroll
+      for (int v = 0; v < VP; ++v) {
+        const long i = i0 + v;
+        double vf[kMaxTypes], tot = 0.0;
+        for (int t = 0; t < nt; ++t) {
+          a.phi[(long)t * ng + i] = phi[t][v];
+          vf[t] = phi[t][v] * a.fh.vol[t];
+        }
+        for (int t = 0; t < nt; ++t) tot += vf[t];
+        for (int t = 0; t < nt; ++t) {
+          double f = a.fh.kappa * (tot - 1.0);
+          for (int t2 = 0; t2 < nt; ++t2) f = f + a.fh.chi[t + t2 * nt] * vf[t2];
+          f = f * a.fh.vol[t];
+          const double df = f - w0[(long)t * ng + i];
+          if (d0) d0[(long)t * ng + i] = df;
+          rmax = nanmax(rmax, fabs(df));
+        }
       }
     }
   }
@@ -263,7 +324,11 @@ __global__ void __launch_bounds__(kThreads) k_contour(ContourArgs<T> a) {
 // y_0 = D_0, y_i = D_0 - D_i (i = 1..m); every pair (i <= j) of
 // <y_i, y_j> is accumulated, giving v_i = <y_i, y_0> and U_ij = <y_i, y_j>.
 constexpr int kGramTile = 64;
+constexpr int kGramBlocks = 1024;  // partial-sum slots of k_gram
 
+// Partial Gram sums: block b covers tiles b, b + gridDim.x, ... and writes its
+// np pair sums to part[p * gridDim.x + b] (pair-major, so k_gram_reduce reads
+// them contiguously).
 __global__ void __launch_bounds__(kThreads) k_gram(const double* __restrict__ dring, long nvar, int m,
                                                    const ScfState* st, int force, double* __restrict__ part,
                                                    const int* done) {
@@ -290,19 +355,18 @@ __global__ void __launch_bounds__(kThreads) k_gram(const double* __restrict__ dr
     }
   }
   const long ntile = (nvar + kGramTile - 1) / kGramTile;
+  const double* d0 = dring + (long)head * nvar;
   for (long tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
     const long k0 = tile * kGramTile;
-    const double* d0 = dring + (long)head * nvar;
-    for (int e = tid; e < kGramTile; e += blockDim.x) {
-      const long k = k0 + e;
-      ys[e] = k < nvar ? d0[k] : 0.0;
-    }
-    __syncthreads();
-    for (int e = tid; e < m * kGramTile; e += blockDim.x) {
-      const int i = 1 + e / kGramTile, c = e % kGramTile;
+    for (int e = tid; e < (m + 1) * kGramTile; e += blockDim.x) {
+      const int i = e / kGramTile, c = e % kGramTile;
       const long k = k0 + c;
-      const double* di = dring + (long)((head + i) % (m + 1)) * nvar;
-      ys[i * ld + c] = k < nvar ? ys[c] - di[k] : 0.0;
+      double y = 0.0;
+      if (k < nvar) {
+        const double x0 = d0[k];
+        y = i == 0 ? x0 : x0 - dring[(long)((head + i) % (m + 1)) * nvar + k];
+      }
+      ys[i * ld + c] = y;
     }
     __syncthreads();
     for (int r = 0; r < 4; ++r) {
@@ -317,8 +381,25 @@ __global__ void __launch_bounds__(kThreads) k_gram(const double* __restrict__ dr
   }
   for (int r = 0; r < 4; ++r) {
     const int p = tid + r * blockDim.x;
-    if (p < np) part[(long)blockIdx.x * np + p] = acc[r];
+    if (p < np) part[(long)p * gridDim.x + blockIdx.x] = acc[r];
   }
+}
+
+// One block per pair: gsum[p] = sum over the k_gram blocks, fixed tree order.
+__global__ void __launch_bounds__(kThreads) k_gram_reduce(const double* __restrict__ part, int nblk, int m,
+                                                          const ScfState* st, int force, double* __restrict__ gsum,
+                                                          const int* done) {
+  if (done && *done) return;
+  if (!force) {
+    const int it = st->iter + 1;
+    if ((it < 1000 ? it : 1000) <= m) return;
+  }
+  __shared__ double sh[kThreads];
+  const int p = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(long)p * nblk + b];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) gsum[p] = s;
 }
 
 struct SolveArgs {
@@ -327,8 +408,7 @@ struct SolveArgs {
   int nblk_contour;
   const double* rpart;
   const double* spart;
-  int nblk_gram;
-  const double* gpart;
+  const double* gsum;  // np reduced Gram sums
   double* resid_hist;  // residual per iteration, or null
   int resid_cap;
   double dvol, volume, ds;
@@ -386,8 +466,7 @@ __global__ void k_solve(SolveArgs a) {
   }
   const int np = (m + 1) * (m + 2) / 2;
   for (int p = tid; p < np; p += blockDim.x) {
-    double s = 0.0;
-    for (int b = 0; b < a.nblk_gram; ++b) s += a.gpart[(long)b * np + p];
+    const double s = a.gsum[p];
     int i = 0, rem = p;
     while (rem >= m + 1 - i) { rem -= m + 1 - i; ++i; }
     const int j = i + rem;

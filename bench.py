@@ -42,7 +42,7 @@ def algorithmic_bytes_iter(ng, ns, m, p):
             + (3 * m + 6) * 2 * ng * 8)
 
 
-def run_gpu(adscf_amd, fields, precision, steps, warmup, w0):
+def run_gpu(adscf_amd, fields, precision, steps, warmup, w0, hipfft=True):
     segs = fields.block_segments([FA, 1 - FA], NS)
     ctx = adscf_amd.Context(N, N, N, L, L, L, 1.0 / NS, precision=precision)
     chi = [[0.0, CHIN], [CHIN, 0.0]]
@@ -54,7 +54,7 @@ def run_gpu(adscf_amd, fields, precision, steps, warmup, w0):
     st = ctx.scf_iterate(steps, tol=-1.0)  # synchronises the library stream on return
     t1 = time.perf_counter()
     ms_iter_ev, ms_sweep = ctx.scf_timing()
-    hipfft_us = ctx.hipfft_step_us(200)
+    hipfft_us = ctx.hipfft_step_us(200) if hipfft else None
     wall = t1 - t0
     out = dict(iters_per_s=steps / wall, ms_per_iter=1000.0 * wall / steps, ms_per_iter_events=ms_iter_ev,
                ms_sweep=ms_sweep, iterations_run=st.iters, stopped_early=st.converged, hipfft_step_us=hipfft_us)
@@ -120,6 +120,19 @@ def run_batched(adscf_amd, fields, steps, warmup):
                 schedules=[l for l in report.splitlines() if "chosen" in l])
 
 
+def copy_probe(mib=256, reps=3):
+    """A plain device copy of known size: FETCH_SIZE and WRITE_SIZE of its kernel calibrate
+    the counters against bytes (each should read mib MiB per dispatch)."""
+    import torch
+
+    x = torch.ones(mib * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
+
+
 def cpu_baseline(fields, w0):
     """The CPU FP64 oracle (single thread) on one SCF iteration of the same workload."""
     import oracle
@@ -139,12 +152,15 @@ def cpu_baseline(fields, w0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--converge-max-iter", type=int, default=4000)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-hipfft", action="store_true", help="skip the hipFFT comparator")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 256 MiB device-to-device copy (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the SCFT path on a single MI355X; "
@@ -155,9 +171,11 @@ def main():
 
     w0 = fields.cylinder_seed(N, L, CHIN, sigma=0.2, seed=2007)
     ng = N ** 3
-    ctx64, r64 = run_gpu(adscf_amd, fields, adscf_amd.FP64, args.steps, args.warmup, w0)
+    if args.copy_probe:
+        copy_probe()
+    ctx64, r64 = run_gpu(adscf_amd, fields, adscf_amd.FP64, args.steps, args.warmup, w0, not args.no_hipfft)
     ctx64.close()
-    ctx32, r32 = run_gpu(adscf_amd, fields, adscf_amd.FP32, args.steps, args.warmup, w0)
+    ctx32, r32 = run_gpu(adscf_amd, fields, adscf_amd.FP32, args.steps, args.warmup, w0, not args.no_hipfft)
     ctx32.close()
     npass = 2 * NS + 1
     sweep_bytes32 = 2 * NS * 3 * ng * 4
@@ -196,6 +214,12 @@ def main():
         "contour_steps_per_s": r32["iters_per_s"] * 2 * NS,
         "timing_valid": not (r32["stopped_early"] or r64["stopped_early"]),
     }
+    if not args.no_hipfft:
+        # one propagator step of one chain: ours = sweep time / (2 chains x Ns steps);
+        # hipFFT = E * c2r(K * r2c(E * q)) with hipFFT plans and two pointwise kernels
+        line["hipfft_comparator"] = {
+            "us_per_chain_step": {"fp32": {"adscf": r32["ms_sweep"] * 1e3 / (2 * NS), "hipfft": r32["hipfft_step_us"]},
+                                  "fp64": {"adscf": r64["ms_sweep"] * 1e3 / (2 * NS), "hipfft": r64["hipfft_step_us"]}}}
     if not args.no_batched:
         line["batched"] = run_batched(adscf_amd, fields, max(5, args.steps // 5), 2)
     if not args.no_converge:

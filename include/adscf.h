@@ -159,7 +159,8 @@ int adscf_set_schedule(adscf_ctx* ctx, int axis, int n1, int n2, int waves);
  * instances, and one HIP graph replay is one SCF iteration of every instance.
  * Per instance i: grid[3i..3i+2], box[3i..3i+2], ds[i], nblocks[i] blocks whose types and
  * segments follow in monomer_type/block_segments (concatenated), kappa[i], chi[4i..4i+3]
- * (2x2 column-major), vol[2i..2i+1].  Axis lengths must be 8, 16, 24, 32, 48, 64 or 128. */
+ * (2x2 column-major), vol[2i..2i+1].  Axis lengths must be 8, 16, 24, 32, 48, 64 or 128;
+ * at most 2048 instances. */
 int adscf_batch_create(adscf_batch** batch, int ninst, const int* grid, const double* box, const double* ds,
                        const int* nblocks, const int* monomer_type, const int* block_segments, const double* kappa,
                        const double* chi, const double* vol, int ensemble, double rho, int mixing_dim,

@@ -56,8 +56,11 @@ static double check4() {
   C2<T> v[N1][N2];
   for (int t = 0; t < N1; ++t)
     for (int j2 = 0; j2 < N2; ++j2) v[t][j2] = x[t + N1 * j2];
-  for (int t = 0; t < N1; ++t) fs_phase_a<T, N1, N2>(v[t], lds.data(), t, tw.data());
-  for (int t = 0; t < N2; ++t) fs_phase_b<T, N1, N2>(lds.data(), t, tw.data(), kf.data());
+  const int tpl = N1 > N2 ? N1 : N2;
+  std::vector<RegTables<T, N1, N2>> R(tpl);
+  for (int t = 0; t < tpl; ++t) R[t].load(t, tw.data(), kf.data());
+  for (int t = 0; t < N1; ++t) fs_phase_a<T, N1, N2>(v[t], lds.data(), t, R[t]);
+  for (int t = 0; t < N2; ++t) fs_phase_b<T, N1, N2>(lds.data(), t, R[t]);
   for (int t = 0; t < N1; ++t) fs_phase_c<T, N1, N2>(v[t], lds.data(), t);
   double worst = 0;
   for (int t = 0; t < N1; ++t)
@@ -82,7 +85,9 @@ static double check1() {
   reference<T>(N, x, kf, re, im);
   C2<T> v[N];
   for (int j = 0; j < N; ++j) v[j] = x[j];
-  line_op_single<T, N>(v, kf.data());
+  RegTables<T, 1, N> R;
+  R.load(0, nullptr, kf.data());
+  line_op_single<T, N>(v, R);
   double worst = 0;
   for (int j = 0; j < N; ++j)
     worst = std::max(worst, std::fabs((double)(v[j].x - re[j])) + std::fabs((double)(v[j].y - im[j])));

@@ -117,3 +117,27 @@ def test_torch_autograd_wrapper(setup):
         loss, gw, gb = H.loss_and_grads(ctx, ([1, 2], [50, 50]), w, [L, L, L], phit, LAM, CKPT)
     assert loss == direct["loss"]
     assert np.array_equal(gw, direct["grad_w"]) and np.array_equal(gb, direct["grad_box"])
+
+
+@pytest.mark.parametrize("dims,bt,bs", [((8, 6, 5), [1, 2, 1], [3, 4, 2]), ((16, 8, 12), [1, 2], [9, 6])])
+def test_adjoint_matches_numpy_restatement(dims, bt, bs):
+    """Exact comparison (no finite-difference noise) against oracle/np_adjoint.py."""
+    import adscf_amd
+    from oracle import np_adjoint
+
+    nx, ny, nz = dims
+    rng = np.random.default_rng(3)
+    w = rng.standard_normal((2, nz, ny, nx))
+    phit = rng.uniform(0.1, 0.9, (nz, ny, nx))
+    Lb = (2.0, 1.7, 1.5)
+    ds = 0.05
+    chi = [[0.0, 12.0], [12.0, 0.0]]
+    loss, F, gw, gb = np_adjoint.adjoint(w, np.array(Lb), bt, bs, ds, 25.0, chi, [1.0, 0.8], phit, 0.1)
+    with adscf_amd.Context(nx, ny, nz, *Lb, ds) as ctx:
+        ctx.set_interactions(25.0, chi, [1.0, 0.8])
+        for ck in (1, 3, 0):
+            out = ctx.adjoint(bt, bs, w, phit, lam=0.1, checkpoint_every=ck)
+            assert abs(out["loss"] - loss) < 1e-11 * abs(loss)
+            assert abs(out["F"] - F) < 1e-11 * abs(F)
+            assert np.max(np.abs(out["grad_w"] - gw)) < 1e-10 * np.max(np.abs(gw))
+            assert np.max(np.abs(out["grad_box"] - gb)) < 1e-10 * np.max(np.abs(gb))
