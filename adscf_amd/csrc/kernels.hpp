@@ -426,11 +426,25 @@ __device__ __forceinline__ void pin_pass_args(const PassArgs<T>& a) {
                "s"(a.dexp1), "s"(a.dexp2), "s"(a.sidx), "s"(a.tmp_out), "s"(a.done));
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// Giving XCD k a contiguous range of tiles keeps tiles that share 128-byte lines
+// (neighbouring x tiles of the y and z passes) in one L2, and gives the x and y
+// passes of the same z-planes the same XCD, so a line fetched or written by one
+// pass is found in that XCD's L2 by the next.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  if (nb < kXcds) return b;
+  const int q = nb / kXcds, r = nb % kXcds;
+  const int k = b % kXcds, i = b / kXcds;
+  return k * q + (k < r ? k : r) + i;
+}
+
 template <typename T, int N1, int N2, int MODE>
 __global__ void __launch_bounds__(kThreads) k_line_pass_reg(PassArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
   if constexpr (MODE != PM_GENERIC) pin_pass_args(a);
-  reg_pass_body<T, N1, N2, MODE>(a, blockIdx.x, smem_raw);
+  reg_pass_body<T, N1, N2, MODE>(a, xcd_tile(blockIdx.x, gridDim.x), smem_raw);
 }
 
 // One pass of many independent instances in one launch: blk[b] packs the

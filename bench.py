@@ -42,13 +42,17 @@ def algorithmic_bytes_iter(ng, ns, m, p):
             + (3 * m + 6) * 2 * ng * 8)
 
 
-def run_gpu(adscf_amd, fields, precision, steps, warmup, w0, hipfft=True):
+def run_gpu(adscf_amd, fields, precision, steps, warmup, w0, hipfft=True, tune=True):
     segs = fields.block_segments([FA, 1 - FA], NS)
     ctx = adscf_amd.Context(N, N, N, L, L, L, 1.0 / NS, precision=precision)
     chi = [[0.0, CHIN], [CHIN, 0.0]]
     ctx.set_interactions(KAPPA, chi, [1.0, 1.0])
     ctx.scf_setup([1, 2], segs, ensemble=1, rho=1.0, mixing_dim=M, lam_mix=LAM)
     ctx.scf_set_field(w0)
+    # per-axis line-pass schedule (shape, waves per workgroup)
+    report = ctx.autotune(reps=3) if tune else "".join(
+        "axis %d chosen: %dx%d waves=%d (default)\n" % ((a,) + tuple(ctx.schedule(a)[k] for k in ("N1", "N2", "waves")))
+        for a in range(3))
     ctx.scf_iterate(warmup, tol=-1.0)
     t0 = time.perf_counter()
     st = ctx.scf_iterate(steps, tol=-1.0)  # synchronises the library stream on return
@@ -57,7 +61,8 @@ def run_gpu(adscf_amd, fields, precision, steps, warmup, w0, hipfft=True):
     hipfft_us = ctx.hipfft_step_us(200) if hipfft else None
     wall = t1 - t0
     out = dict(iters_per_s=steps / wall, ms_per_iter=1000.0 * wall / steps, ms_per_iter_events=ms_iter_ev,
-               ms_sweep=ms_sweep, iterations_run=st.iters, stopped_early=st.converged, hipfft_step_us=hipfft_us)
+               ms_sweep=ms_sweep, iterations_run=st.iters, stopped_early=st.converged, hipfft_step_us=hipfft_us,
+               schedule=[l for l in report.splitlines() if "chosen" in l])
     return ctx, out
 
 
@@ -121,16 +126,19 @@ def run_batched(adscf_amd, fields, steps, warmup):
 
 
 def copy_probe(mib=256, reps=3):
-    """A plain device copy of known size: FETCH_SIZE and WRITE_SIZE of its kernel calibrate
-    the counters against bytes (each should read mib MiB per dispatch)."""
+    """Plain device copies of known size for calibrating FETCH_SIZE / WRITE_SIZE against bytes:
+    a contiguous copy of mib MiB (hipMemcpy's kernel, wide requests) and a copy that reads
+    mib/2 MiB in 64-byte pieces (every other 64 B; the segment size of the line passes)."""
     import torch
 
     x = torch.ones(mib * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
     for _ in range(reps):
         y.copy_(x)
+    for _ in range(reps):
+        z = x.view(-1, 32)[:, :16].contiguous()
     torch.cuda.synchronize()
-    del x, y
+    del x, y, z
 
 
 def cpu_baseline(fields, w0):
@@ -159,6 +167,7 @@ def main():
     ap.add_argument("--converge-max-iter", type=int, default=4000)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-hipfft", action="store_true", help="skip the hipFFT comparator")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default line-pass schedule")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 256 MiB device-to-device copy (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -173,9 +182,11 @@ def main():
     ng = N ** 3
     if args.copy_probe:
         copy_probe()
-    ctx64, r64 = run_gpu(adscf_amd, fields, adscf_amd.FP64, args.steps, args.warmup, w0, not args.no_hipfft)
+    ctx64, r64 = run_gpu(adscf_amd, fields, adscf_amd.FP64, args.steps, args.warmup, w0, not args.no_hipfft,
+                         not args.no_autotune)
     ctx64.close()
-    ctx32, r32 = run_gpu(adscf_amd, fields, adscf_amd.FP32, args.steps, args.warmup, w0, not args.no_hipfft)
+    ctx32, r32 = run_gpu(adscf_amd, fields, adscf_amd.FP32, args.steps, args.warmup, w0, not args.no_hipfft,
+                         not args.no_autotune)
     ctx32.close()
     npass = 2 * NS + 1
     sweep_bytes32 = 2 * NS * 3 * ng * 4
@@ -205,7 +216,8 @@ def main():
                      "kernel": "k_line_pass<float> (propagator line pass)",
                      "bytes_per_launch": sweep_bytes32 / npass,
                      "us_per_launch": r32["ms_sweep"] * 1e3 / npass},
-        "fp64": {"iters_per_s": r64["iters_per_s"], "ms_per_iter": r64["ms_per_iter"],
+        "schedule": r32["schedule"],
+        "fp64": {"iters_per_s": r64["iters_per_s"], "ms_per_iter": r64["ms_per_iter"], "schedule": r64["schedule"],
                  "roofline_achieved_GBs": ach64, "roofline_frac": ach64 / HBM_PEAK_GBS,
                  "us_per_pass": r64["ms_sweep"] * 1e3 / npass,
                  "iteration_bytes_frac": biter64 * r64["iters_per_s"] / (HBM_PEAK_GBS * 1e9)},
