@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -27,6 +28,7 @@ using namespace adscf;
 namespace {
 
 constexpr int kRedBlocks = 512;  // blocks of the grid-stride reduction kernels
+constexpr long kMaxGridPoints = 1L << 27;  // 512^3; a complex FP64 grid array stays below 2 GiB
 
 struct HipError {
   hipError_t e;
@@ -116,6 +118,7 @@ struct adscf_ctx {
   double lam_mix = 0.5;
   double rho = 1.0, mu = 0.0;
   hipGraphExec_t graph = nullptr;
+  bool use_graph = true;  // ADSCF_GRAPH=0 in the environment: stream launches (profiling aid)
   hipGraph_t graph_src = nullptr;
   int graph_prec = -1;
   std::vector<const void*> graph_key;  // buffers baked into the captured iteration
@@ -178,7 +181,7 @@ void launch_gram(hipStream_t stream, const double* ring, long nvar, int m, const
   const long ntile = (nvar + kGramTile - 1) / kGramTile;
   const int nblk = (int)std::max<long>(1, std::min<long>(ntile, kGramBlocks));
   const int np = (m + 1) * (m + 2) / 2;
-  const size_t gsm = (size_t)(m + 1) * (kGramTile + 1) * sizeof(double);
+  const size_t gsm = (size_t)gram_lds_doubles(m) * sizeof(double);
   hipLaunchKernelGGL(k_gram, dim3(nblk), dim3(kThreads), gsm, stream, ring, nvar, m, st, force, part, done);
   hipLaunchKernelGGL(k_gram_reduce, dim3(np), dim3(kThreads), 0, stream, (const double*)part, nblk, m, st, force,
                      gsum, done);
@@ -951,7 +954,7 @@ void build_graph(adscf_ctx* c) {
 
 template <typename T>
 int scf_iterate_impl(adscf_ctx* c, int niter, double tol, int* iters, double* acc, int* conv) {
-  build_graph<T>(c);
+  if (c->use_graph) build_graph<T>(c);
   HIPCK(hipMemcpyAsync(&c->st.p->tol, &tol, sizeof(double), hipMemcpyHostToDevice, c->stream));
   ScfState st{};
   int done_iters = 0;
@@ -959,7 +962,10 @@ int scf_iterate_impl(adscf_ctx* c, int niter, double tol, int* iters, double* ac
   while (done_iters < niter) {
     const int chunk = std::min(niter - done_iters, tol > 0 ? 25 : niter);
     HIPCK(hipEventRecord(c->ev0, c->stream));
-    for (int i = 0; i < chunk; ++i) HIPCK(hipGraphLaunch(c->graph, c->stream));
+    for (int i = 0; i < chunk; ++i) {
+      if (c->use_graph) HIPCK(hipGraphLaunch(c->graph, c->stream));
+      else enqueue_iteration<T>(c);
+    }
     HIPCK(hipEventRecord(c->ev1, c->stream));
     HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
@@ -1683,6 +1689,8 @@ int adscf_init(adscf_ctx** out, int nx, int ny, int nz, int ntypes, double lx, d
   if (nx < 1 || ny < 1 || nz < 1 || ntypes < 1 || ntypes > kMaxTypes || !(lx > 0) || !(ly > 0) || !(lz > 0) ||
       !(ds > 0) || (precision != ADSCF_FP64 && precision != ADSCF_FP32))
     return ADSCF_ERR_ARG;
+  // the line passes address one grid-sized array with 32-bit byte offsets
+  if ((long)nx * ny * nz > kMaxGridPoints) return ADSCF_ERR_ARG;
   auto* c = new (std::nothrow) adscf_ctx();
   if (!c) return ADSCF_ERR_HIP;
   int rc = guarded(c, [&] {
@@ -1690,6 +1698,7 @@ int adscf_init(adscf_ctx** out, int nx, int ny, int nz, int ntypes, double lx, d
     HIPCK(hipGetDeviceCount(&ndev));
     if (ndev < 1) throw ArgError{"no HIP device"};
     HIPCK(hipGetDevice(&c->dev));
+    if (const char* g = getenv("ADSCF_GRAPH")) c->use_graph = strcmp(g, "0") != 0;
     HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCK(hipEventCreate(&c->ev0));
     HIPCK(hipEventCreate(&c->ev1));

@@ -295,6 +295,19 @@ inline int pass_mode(const PassArgs<T>& p) {
 // time is mostly memory latency: every global load of the pass (the stop flag,
 // the lines, the E factors of both steps, twiddles, multipliers and rescale
 // exponents) is issued before the first use, and the pass pays one latency.
+// Loads and stores at a 32-bit byte offset from a uniform base pointer: the
+// compiler then addresses with the scalar base plus one vector offset register
+// instead of 64-bit vector address arithmetic per access.  Every array a pass
+// touches is below 2 GiB (the context limits the grid size).
+template <typename V>
+__device__ __forceinline__ V ldo(const V* base, unsigned off) {
+  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + off);
+}
+template <typename V>
+__device__ __forceinline__ void sto(V* base, unsigned off, V x) {
+  *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + off) = x;
+}
+
 template <typename T, int N1, int N2, int MODE = PM_GENERIC>
 __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, unsigned char* smem_raw) {
   using FS = FourStep<N1, N2>;
@@ -317,6 +330,11 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
   }
   bool valid;
   const long base = pass_line_base(a, item, l, valid);
+  // byte offsets of this thread's points j2 in a real (fo) and a complex (fc) array
+  const unsigned f0 = (unsigned)(base + (long)t * a.stride) * (unsigned)sizeof(T);
+  const unsigned fstep = (unsigned)(N1 * a.stride) * (unsigned)sizeof(T);
+  auto fo = [&](int j2) { return f0 + (unsigned)j2 * fstep; };
+  auto fc = [&](int j2) { return 2u * (f0 + (unsigned)j2 * fstep); };
   C2<T>* lds = reinterpret_cast<C2<T>*>(smem_raw) + l * FS::lds_elems;
   const bool act = t < N1 && valid;
   const int stop = *(a.done ? a.done : &g_no_stop);
@@ -329,22 +347,21 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
     C2<T> x = {T(0), T(0)};
     ea1[j2] = ea2[j2] = eb1[j2] = eb2[j2] = T(1);
     if (act) {
-      const long g = base + (long)(t + N1 * j2) * a.stride;
       if (in_kind == IN_TMP) {
-        x = a.tmp_in[g];
+        x = ldo(a.tmp_in, fc(j2));
       } else if (in_kind == IN_ONES) {
         x = {T(1), two ? T(1) : T(0)};
       } else {
-        x.x = a.q1_in[g];
-        x.y = a.q2_in ? a.q2_in[g] : T(0);
+        x.x = ldo(a.q1_in, fo(j2));
+        x.y = a.q2_in ? ldo(a.q2_in, fo(j2)) : T(0);
       }
       if (useEa) {
-        ea1[j2] = a.Ea1[g];
-        ea2[j2] = a.Ea2[g];
+        ea1[j2] = ldo(a.Ea1, fo(j2));
+        ea2[j2] = ldo(a.Ea2, fo(j2));
       }
       if (useEb) {
-        eb1[j2] = a.Eb1[g];
-        eb2[j2] = a.Eb2[g];
+        eb1[j2] = ldo(a.Eb1, fo(j2));
+        eb2[j2] = ldo(a.Eb2, fo(j2));
       }
     }
     v[j2] = x;
@@ -375,12 +392,11 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
     if (act) {
 #pragma unroll
       for (int j2 = 0; j2 < N2; ++j2) {
-        const long g = base + (long)(t + N1 * j2) * a.stride;
         C2<T> x = v[j2];
         x.x = x.x * ea1[j2] * sc1;
         x.y = x.y * ea2[j2] * sc2;
-        a.h1[g] = x.x;
-        a.h2[g] = x.y;
+        sto(a.h1, fo(j2), x.x);
+        sto(a.h2, fo(j2), x.y);
         x.x *= eb1[j2];
         x.y *= eb2[j2];
         v[j2] = x;
@@ -400,18 +416,17 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
   if (!act) return;
   if (out_kind == OUT_TMP) {
 #pragma unroll
-    for (int j2 = 0; j2 < N2; ++j2) a.tmp_out[base + (long)(t + N1 * j2) * a.stride] = v[j2];
+    for (int j2 = 0; j2 < N2; ++j2) sto(a.tmp_out, fc(j2), v[j2]);
   } else {
 #pragma unroll
     for (int j2 = 0; j2 < N2; ++j2) {
-      const long g = base + (long)(t + N1 * j2) * a.stride;
       C2<T> x = v[j2];
       if (post_E) {
         x.x = x.x * ea1[j2] * sc1;
         x.y = x.y * ea2[j2] * sc2;
       }
-      a.h1[g] = x.x;
-      if (two) a.h2[g] = x.y;
+      sto(a.h1, fo(j2), x.x);
+      if (two) sto(a.h2, fo(j2), x.y);
     }
   }
 }

@@ -328,7 +328,25 @@ constexpr int kGramBlocks = 1024;  // partial-sum slots of k_gram
 
 // Partial Gram sums: block b covers tiles b, b + gridDim.x, ... and writes its
 // np pair sums to part[p * gridDim.x + b] (pair-major, so k_gram_reduce reads
-// them contiguously).
+// them contiguously).  Pairs are register-blocked 3 x 3: a thread owns one block
+// (bi <= bj) of the (m+1) x (m+1) pair matrix and one of R sub-ranges of the
+// tile's elements, so each LDS value it reads feeds three products.
+constexpr int kGramBlk = 3;
+
+__host__ __device__ constexpr int gram_pair_blocks(int m) {
+  return ((m + 1 + kGramBlk - 1) / kGramBlk) * ((m + 1 + kGramBlk - 1) / kGramBlk + 1) / 2;
+}
+// sub-ranges per tile: the largest power of two with R * blocks <= kThreads and R <= kGramTile
+__host__ __device__ constexpr int gram_subranges(int m) {
+  int r = 1;
+  while (2 * r * gram_pair_blocks(m) <= kThreads && 2 * r <= kGramTile) r *= 2;
+  return r;
+}
+// dynamic LDS of k_gram in doubles: the tile, then the per-thread block sums
+__host__ __device__ constexpr int gram_lds_doubles(int m) {
+  return (m + 1) * (kGramTile + 1) + gram_pair_blocks(m) * gram_subranges(m) * kGramBlk * kGramBlk;
+}
+
 __global__ void __launch_bounds__(kThreads) k_gram(const double* __restrict__ dring, long nvar, int m,
                                                    const ScfState* st, int force, double* __restrict__ part,
                                                    const int* done) {
@@ -337,51 +355,82 @@ __global__ void __launch_bounds__(kThreads) k_gram(const double* __restrict__ dr
     const int it = st->iter + 1;
     if ((it < 1000 ? it : 1000) <= m) return;  // this iteration mixes without the history
   }
-  extern __shared__ double ys[];  // (m+1) x (kGramTile+1)
+  extern __shared__ double ys[];  // (m+1) x (kGramTile+1), then block sums
   const int ld = kGramTile + 1;
+  const int nv = m + 1;
+  const int nb3 = (nv + kGramBlk - 1) / kGramBlk;
+  const int npb = gram_pair_blocks(m);
+  const int R = gram_subranges(m);
+  const int sub = kGramTile / R;
+  double* red = ys + nv * ld;
   const int head = st ? st->head : 0;
-  const int np = (m + 1) * (m + 2) / 2;
   const int tid = threadIdx.x;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  int pi[4], pj[4];
-  for (int r = 0; r < 4; ++r) {
-    const int p = tid + r * blockDim.x;
-    pi[r] = pj[r] = -1;
-    if (p < np) {
-      int i = 0, rem = p;
-      while (rem >= m + 1 - i) { rem -= m + 1 - i; ++i; }
-      pi[r] = i;
-      pj[r] = i + rem;
+  const bool worker = tid < npb * R;
+  int bi = 0, bj = 0, r = 0;
+  if (worker) {
+    int rem = tid / R;
+    r = tid % R;
+    while (rem >= nb3 - bi) {
+      rem -= nb3 - bi;
+      ++bi;
     }
+    bj = bi + rem;
   }
+  double acc[kGramBlk * kGramBlk];
+  for (int q = 0; q < kGramBlk * kGramBlk; ++q) acc[q] = 0.0;
+  __shared__ const double* slot[kMaxHist + 1];  // history entry i = ring slot (head + i) mod (m + 1)
+  for (int i = tid; i < nv; i += blockDim.x) slot[i] = dring + (long)((head + i) % nv) * nvar;
+  __syncthreads();
   const long ntile = (nvar + kGramTile - 1) / kGramTile;
   const double* d0 = dring + (long)head * nvar;
   for (long tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
     const long k0 = tile * kGramTile;
-    for (int e = tid; e < (m + 1) * kGramTile; e += blockDim.x) {
+    for (int e = tid; e < nv * kGramTile; e += blockDim.x) {
       const int i = e / kGramTile, c = e % kGramTile;
       const long k = k0 + c;
       double y = 0.0;
       if (k < nvar) {
         const double x0 = d0[k];
-        y = i == 0 ? x0 : x0 - dring[(long)((head + i) % (m + 1)) * nvar + k];
+        y = i == 0 ? x0 : x0 - slot[i][k];
       }
       ys[i * ld + c] = y;
     }
     __syncthreads();
-    for (int r = 0; r < 4; ++r) {
-      if (pi[r] < 0) continue;
-      const double* a = ys + pi[r] * ld;
-      const double* b = ys + pj[r] * ld;
-      double s = acc[r];
-      for (int c = 0; c < kGramTile; ++c) s += a[c] * b[c];
-      acc[r] = s;
+    if (worker) {
+      for (int c = r * sub; c < (r + 1) * sub; ++c) {
+        double yi[kGramBlk], yj[kGramBlk];
+#pragma unroll
+        for (int u = 0; u < kGramBlk; ++u) {
+          const int i = kGramBlk * bi + u, j = kGramBlk * bj + u;
+          yi[u] = i < nv ? ys[i * ld + c] : 0.0;
+          yj[u] = j < nv ? ys[j * ld + c] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kGramBlk; ++u)
+#pragma unroll
+          for (int v = 0; v < kGramBlk; ++v) acc[u * kGramBlk + v] += yi[u] * yj[v];
+      }
     }
     __syncthreads();
   }
-  for (int r = 0; r < 4; ++r) {
-    const int p = tid + r * blockDim.x;
-    if (p < np) part[(long)p * gridDim.x + blockIdx.x] = acc[r];
+  if (worker)
+    for (int q = 0; q < kGramBlk * kGramBlk; ++q) red[(long)tid * kGramBlk * kGramBlk + q] = acc[q];
+  __syncthreads();
+  // unique pairs (i <= j): the R sub-range sums of its block entry, in order
+  const int np = nv * (nv + 1) / 2;
+  for (int p = tid; p < np; p += blockDim.x) {
+    int i = 0, rem = p;
+    while (rem >= nv - i) {
+      rem -= nv - i;
+      ++i;
+    }
+    const int j = i + rem;
+    const int pi = i / kGramBlk, pj = j / kGramBlk;
+    const int pb = pi * nb3 - pi * (pi - 1) / 2 + (pj - pi);
+    const int q = (i % kGramBlk) * kGramBlk + (j % kGramBlk);
+    double s = 0.0;
+    for (int rr = 0; rr < R; ++rr) s += red[(long)(pb * R + rr) * kGramBlk * kGramBlk + q];
+    part[(long)p * gridDim.x + blockIdx.x] = s;
   }
 }
 
@@ -400,6 +449,123 @@ __global__ void __launch_bounds__(kThreads) k_gram_reduce(const double* __restri
   for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(long)p * nblk + b];
   s = block_sum(s, sh);
   if (threadIdx.x == 0) gsum[p] = s;
+}
+
+// ludcmp + lubksb (mod_iterate.f90:232-337) by one wavefront, lanes = rows.
+// Every matrix element sees the operations of lu_decompose / lu_backsubstitute
+// (lu.hpp) in the same order: a left-looking sum s - a(i,1)a(1,j) - a(i,2)a(2,j)
+// - ... becomes a sequence of right-looking updates over k = 1, 2, ..., which
+// is the same sequence for that element, while all rows update in parallel.
+// The backward substitution keeps the reference's increasing-j order per row,
+// so it stays sequential in i with broadcast operands.  a: column-major m x m
+// in LDS; b: right-hand side in LDS, overwritten by the solution; indx gets the
+// 1-based pivots.  Returns 1 if a row of a is entirely zero.
+__device__ __forceinline__ double lane_bcast(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ int lu_solve_wave(double* a, double* b, int m, int* indx) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const bool row = lane < m;
+  double vv = 0.0;
+  int zero_row = 0;
+  if (row) {
+    double aamax = 0.0;
+    for (int j = 0; j < m; ++j) {
+      const double v = fabs(a[lane + j * m]);
+      if (v > aamax) aamax = v;
+    }
+    zero_row = aamax == 0.0;
+    vv = 1.0 / aamax;
+  }
+  const int singular = __ballot(zero_row) != 0;
+  int imax = 0;
+  for (int j = 0; j < m; ++j) {
+    double c = row ? a[lane + j * m] : 0.0;
+    double nxt = (row && j > 0) ? a[lane] : 0.0;  // a(lane, k) one step ahead
+    for (int k = 0; k < j; ++k) {
+      const double aik = nxt;
+      if (row && k + 1 < j) nxt = a[lane + (k + 1) * m];
+      const double akj = lane_bcast(c, k);  // final: lane k has applied every k' < k
+      if (row && lane > k) c = c - aik * akj;
+    }
+    // pivot search of ludcmp (:263-270): the scan over i = j..n in order, '.ge.'
+    const double dum_l = row ? vv * fabs(c) : 0.0;
+    double aamax = 0.0;
+    for (int i = j; i < m; ++i) {
+      const double dum = lane_bcast(dum_l, i);
+      if (dum >= aamax) {
+        imax = i;
+        aamax = dum;
+      }
+    }
+    if (row) a[lane + j * m] = c;
+    wave_lds_sync();
+    if (j != imax) {
+      if (lane < m) {
+        const double t = a[imax + lane * m];
+        a[imax + lane * m] = a[j + lane * m];
+        a[j + lane * m] = t;
+      }
+      const double vj = lane_bcast(vv, j);
+      if (lane == imax) vv = vj;
+      wave_lds_sync();
+    }
+    if (lane == 0) {
+      indx[j] = imax + 1;
+      if (a[j + j * m] == 0.0) a[j + j * m] = 1.0e-20;
+    }
+    wave_lds_sync();
+    if (j != m - 1) {
+      const double dinv = 1.0 / a[j + j * m];
+      if (row && lane > j) a[lane + j * m] = a[lane + j * m] * dinv;
+    }
+    wave_lds_sync();
+  }
+  // forward substitution: the interchanges first (each touches entries >= i only,
+  // so applying them up front gives every step the values lubksb reads), then
+  // y_i -= a(i,j) y_j for j = ii, ii+1, ... with ii the first nonzero entry
+  if (lane == 0) {
+    for (int i = 0; i < m; ++i) {
+      const int ll = indx[i] - 1;
+      const double t = b[ll];
+      b[ll] = b[i];
+      b[i] = t;
+    }
+  }
+  wave_lds_sync();
+  double y = row ? b[lane] : 0.0;
+  int first = m;
+  for (int i = 0; i < m; ++i)
+    if (lane_bcast(y, i) != 0.0) {
+      first = i;
+      break;
+    }
+  for (int j = first; j < m - 1; ++j) {
+    const double yj = lane_bcast(y, j);
+    if (row && lane > j) y = y - a[lane + j * m] * yj;
+  }
+  // backward substitution (:329-335): the products a(i,j) x_j in parallel (lane j),
+  // then the reference's increasing-j chain of subtractions
+  double x = 0.0;
+  for (int i = m - 1; i >= 0; --i) {
+    const double t = (lane > i && row) ? a[i + lane * m] * x : 0.0;
+    double sacc = lane_bcast(y, i);
+    for (int j = i + 1; j < m; ++j) sacc = sacc - lane_bcast(t, j);
+    const double xi = sacc / a[i + i * m];
+    if (lane == i) x = xi;
+  }
+  if (row) b[lane] = x;
+  wave_lds_sync();
+  return singular;
 }
 
 struct SolveArgs {
@@ -427,7 +593,6 @@ __global__ void k_solve(SolveArgs a) {
   __shared__ double U[kMaxHist * kMaxHist];
   __shared__ double V[kMaxHist];
   __shared__ int indx[kMaxHist];
-  __shared__ double vv[kMaxHist + 1];
   __shared__ int done_now;
   const int tid = threadIdx.x;
   const int m = a.m;
@@ -478,13 +643,14 @@ __global__ void k_solve(SolveArgs a) {
     }
   }
   __syncthreads();
-  if (tid == 0) {
-    double d;
-    st->singular = lu_decompose(U, m, indx, &d, vv);
-    lu_backsubstitute(U, m, indx, V);
-    for (int i = 0; i < m; ++i) st->x[i] = V[i];
-    st->mode = 1;
-    st->lam = a.lam_mix;
+  if (tid < 64) {
+    const int sing = lu_solve_wave(U, V, m, indx);
+    if (tid < m) st->x[tid] = V[tid];
+    if (tid == 0) {
+      st->singular = sing;
+      st->mode = 1;
+      st->lam = a.lam_mix;
+    }
   }
 }
 
@@ -504,8 +670,10 @@ __global__ void __launch_bounds__(kThreads) k_mix(double* __restrict__ wring, co
   const double lam = st->lam;
   const int dst = (head + m) % (m + 1);
   __shared__ double x[kMaxHist];
+  __shared__ long off[kMaxHist + 1];  // history entry j = ring slot (head + j) mod (m + 1)
   if (mode == 1)
     for (int i = threadIdx.x; i < m; i += blockDim.x) x[i] = st->x[i];
+  for (int j = threadIdx.x; j <= m; j += blockDim.x) off[j] = (long)((head + j) % (m + 1)) * nvar;
   __syncthreads();
   const double* w0 = wring + (long)head * nvar;
   const double* d0 = dring + (long)head * nvar;
@@ -518,7 +686,7 @@ __global__ void __launch_bounds__(kThreads) k_mix(double* __restrict__ wring, co
       double f = W0, d = D0;
       if (mode == 1) {
         for (int j = 1; j <= m; ++j) {
-          const long o = (long)((head + j) % (m + 1)) * nvar + k;
+          const long o = off[j] + k;
           f = f - x[j - 1] * (W0 - wring[o]);
           d = d - x[j - 1] * (D0 - dring[o]);
         }

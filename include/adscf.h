@@ -48,7 +48,8 @@ typedef struct adscf_batch adscf_batch;
  * (Geometry_Modules/Bulk/mod_fourier.f90:49-54), Calculate_Laplace (:165-198) and
  * Calculate_Volume_Parameters (:152-157) for a runtime grid (the reference fixes
  * gridx/gridy/gridz at compile time in module Global, e.g. Main_Diblock_Melt/mod_global.f90:8-19).
- * ntypes = monomer_types.  precision = ADSCF_FP64 or ADSCF_FP32. */
+ * ntypes = monomer_types.  precision = ADSCF_FP64 or ADSCF_FP32.  At most 2^27 grid
+ * points (512^3); larger grids return ADSCF_ERR_ARG. */
 int adscf_init(adscf_ctx** ctx, int nx, int ny, int nz, int ntypes, double lx, double ly, double lz, double ds,
                int precision);
 /* Finish / End_Fourier (sub_wrap.f90:48-59, mod_fourier.f90:62-67) */
