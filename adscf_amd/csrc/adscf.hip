@@ -289,19 +289,26 @@ struct RegShape {
   int n, N1, N2;
 };
 
+// N2 == 0: the R^3 cube engine with R = N1 (fft_cube.hpp)
 const RegShape kRegShapes[] = {
     {2, 1, 2},    {3, 1, 3},   {4, 1, 4},   {5, 1, 5},    {6, 1, 6},    {8, 1, 8},   {16, 1, 16},  {10, 2, 5},
     {12, 4, 3},   {20, 4, 5},  {24, 8, 3},  {24, 4, 6},   {32, 4, 8},   {32, 8, 4},  {32, 2, 16},  {36, 6, 6},
-    {40, 8, 5},   {48, 8, 6},  {48, 6, 8},  {48, 16, 3},  {64, 8, 8},   {64, 4, 16}, {64, 16, 4},  {80, 16, 5},
-    {96, 16, 6},  {128, 8, 16}, {128, 16, 8}, {256, 16, 16}};
+    {40, 8, 5},   {48, 8, 6},  {48, 6, 8},  {48, 16, 3},  {64, 8, 8},   {64, 4, 0},  {64, 4, 16},  {64, 16, 4},
+    {80, 16, 5},  {96, 16, 6}, {128, 8, 16}, {128, 16, 8}, {256, 16, 16}};
 
-int reg_tpl_rt(int N1, int N2) { return N1 == 1 ? 1 : std::max(N1, N2); }
+std::string shape_name(int N1, int N2) {
+  if (N1 == 0) return "stockham";
+  if (N2 == 0) return "cube" + std::to_string(N1);
+  return std::to_string(N1) + "x" + std::to_string(N2);
+}
+
+int reg_tpl_rt(int N1, int N2) { return N2 == 0 ? N1 * N1 : (N1 == 1 ? 1 : std::max(N1, N2)); }
 
 // Tile geometry of a register-resident pass: `waves` wavefronts per block, L lines
 // per block; fills p.L and p.gx and returns the number of blocks.
 template <typename T>
 int prep_reg(const adscf_ctx* c, PassArgs<T>& p, int N1, int N2, int waves, int* threads, size_t* smem) {
-  const int TPL = N1 == 1 ? 1 : std::max(N1, N2);
+  const int TPL = reg_tpl_rt(N1, N2);
   int L = std::max(1, 64 * waves / TPL);
   if (p.axis != 0) L = std::min(L, c->nx);
   p.L = L;
@@ -311,7 +318,8 @@ int prep_reg(const adscf_ctx* c, PassArgs<T>& p, int N1, int N2, int waves, int*
   else if (p.axis == 1) items = c->nz * p.gx;
   else items = c->ny * p.gx;
   if (threads) *threads = TPL * L;
-  if (smem) *smem = N1 == 1 ? 0 : (size_t)L * (size_t)N2 * (N1 + 1) * sizeof(C2<T>);
+  const size_t per_line = N2 == 0 ? (size_t)2 * N1 * N1 * (N1 + 1) : (N1 == 1 ? 0 : (size_t)N2 * (N1 + 1));
+  if (smem) *smem = (size_t)L * per_line * sizeof(C2<T>);
   return items;
 }
 
@@ -342,7 +350,7 @@ RegLauncher<T> reg_launcher(int n, int N1, int N2) {
   ADSCF_SHAPE(8, 1, 8) ADSCF_SHAPE(16, 1, 16) ADSCF_SHAPE(10, 2, 5) ADSCF_SHAPE(12, 4, 3) ADSCF_SHAPE(20, 4, 5)
   ADSCF_SHAPE(24, 8, 3) ADSCF_SHAPE(24, 4, 6) ADSCF_SHAPE(32, 4, 8) ADSCF_SHAPE(32, 8, 4) ADSCF_SHAPE(32, 2, 16)
   ADSCF_SHAPE(36, 6, 6) ADSCF_SHAPE(40, 8, 5) ADSCF_SHAPE(48, 8, 6) ADSCF_SHAPE(48, 6, 8) ADSCF_SHAPE(48, 16, 3)
-  ADSCF_SHAPE(64, 8, 8) ADSCF_SHAPE(64, 4, 16) ADSCF_SHAPE(64, 16, 4) ADSCF_SHAPE(80, 16, 5)
+  ADSCF_SHAPE(64, 8, 8) ADSCF_SHAPE(64, 4, 0) ADSCF_SHAPE(64, 4, 16) ADSCF_SHAPE(64, 16, 4) ADSCF_SHAPE(80, 16, 5)
   ADSCF_SHAPE(96, 16, 6) ADSCF_SHAPE(128, 8, 16) ADSCF_SHAPE(128, 16, 8) ADSCF_SHAPE(256, 16, 16)
 #undef ADSCF_SHAPE
   return nullptr;
@@ -1029,8 +1037,8 @@ std::string autotune_impl(adscf_ctx* c, int reps) {
         ax.N2 = sh.second;
         ax.waves = waves;
         const double ms = sweep_timing<T>(c, reps);
-        snprintf(line, sizeof line, "axis %d n=%d shape=%s%dx%d waves=%d sweep=%.4f ms\n", a, ax.n,
-                 sh.first ? "" : "stockham:", sh.first, sh.second, waves, ms);
+        snprintf(line, sizeof line, "axis %d n=%d shape=%s waves=%d sweep=%.4f ms\n", a, ax.n,
+                 shape_name(sh.first, sh.second).c_str(), waves, ms);
         rep += line;
         if (ms < best_ms) {
           best_ms = ms;
@@ -1039,8 +1047,8 @@ std::string autotune_impl(adscf_ctx* c, int reps) {
       }
     }
     ax = best;
-    snprintf(line, sizeof line, "axis %d chosen: %dx%d waves=%d (%.4f ms)\n", a, best.N1, best.N2, best.waves,
-             best_ms);
+    snprintf(line, sizeof line, "axis %d chosen: %s waves=%d (%.4f ms)\n", a, shape_name(best.N1, best.N2).c_str(),
+             best.waves, best_ms);
     rep += line;
   }
   if (c->graph) {
@@ -2512,7 +2520,8 @@ int adscf_batch_autotune(adscf_batch* b, int reps, char* report, int report_cap)
             }
         const double ms = b->prec == ADSCF_FP64 ? batch_sweep_timing<double>(b, std::max(reps, 1))
                                                 : batch_sweep_timing<float>(b, std::max(reps, 1));
-        snprintf(line, sizeof line, "n=%d shape=%dx%d batched sweep=%.4f ms\n", n, sh.first, sh.second, ms);
+        snprintf(line, sizeof line, "n=%d shape=%s batched sweep=%.4f ms\n", n, shape_name(sh.first, sh.second).c_str(),
+                 ms);
         rep += line;
         if (ms < best) {
           best = ms;
@@ -2525,7 +2534,8 @@ int adscf_batch_autotune(adscf_batch* b, int reps, char* report, int report_cap)
             c->ax[a].N1 = bestsh.first;
             c->ax[a].N2 = bestsh.second;
           }
-      snprintf(line, sizeof line, "n=%d chosen %dx%d (%.4f ms)\n", n, bestsh.first, bestsh.second, best);
+      snprintf(line, sizeof line, "n=%d chosen %s (%.4f ms)\n", n, shape_name(bestsh.first, bestsh.second).c_str(),
+               best);
       rep += line;
     }
     batch_drop_graph(b);

@@ -1,0 +1,103 @@
+// Register-resident line operator L = F^-1 diag(K) F for n = R^3 held by R^2
+// threads of one line, R points each (index j = t + R^2 m).  With every thread
+// busy in every phase it gives a 64-point line 16 threads (four per thread in
+// registers) where the 8 x 8 four-step of fft_reg.hpp gives 8: twice the waves
+// at a small grid, and radix-4 butterflies need no multiplications.
+//   forward  A  thread t:        DFT_R over m, twiddle W_n^(t k2)        -> S[t][k2]
+//            B  thread (k2, a):  DFT_R over b of S[a + R b][k2], twiddle W_(R^2)^(a c) -> S2[k2][c][a]
+//            C  thread (k2, c):  DFT_R over a -> bin k = k2 + R (c + R d); multiply by K[k]
+//   inverse  C, B, A with conjugate DFTs and twiddles in reverse order
+// (n = N1 N2 four-step with N1 = R^2, N2 = R, its inner DFT_(R^2) split again as R x R).
+#pragma once
+
+#include "fft_reg.hpp"
+
+namespace adscf {
+
+template <int R>
+struct Cube {
+  static constexpr int n = R * R * R;
+  static constexpr int tpl = R * R;           // threads per line
+  static constexpr int np = R;                // points per thread
+  static constexpr int sp = R + 1;            // padded row of S and S2
+  static constexpr int lds_elems = 2 * R * R * sp;  // S then S2, per line
+};
+
+// twiddles and multipliers of thread t of a line
+template <typename T, int R>
+struct CubeTables {
+  C2<T> wa[R];  // W_n^(t k2)
+  C2<T> wb[R];  // W_n^(R a c), a = t % R, c = 0..R-1
+  T k[R];       // K[k2 + R (c + R d)], (k2, c) = (t / R, t % R), d = 0..R-1
+  ADSCF_HD void load(int t, const C2<T>* tw, const T* kf) {
+    const int a = t % R, k2 = t / R;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      wa[q] = tw[t * q];
+      wb[q] = tw[R * a * q];
+      k[q] = kf[k2 + R * (a + R * q)];
+    }
+  }
+};
+
+// The five phases; thread t of the line, lds = S (R^2 padded rows) then S2.
+template <typename T, int R>
+ADSCF_HD void cube_a(C2<T>* v, C2<T>* lds, int t, const CubeTables<T, R>& W) {
+  constexpr int sp = Cube<R>::sp;
+  dft_reg<T, R, -1>(v);
+#pragma unroll
+  for (int q = 0; q < R; ++q) lds[t * sp + q] = (q > 0 && t > 0) ? cmul(v[q], W.wa[q]) : v[q];
+}
+template <typename T, int R>
+ADSCF_HD void cube_b(C2<T>* lds, int t, const CubeTables<T, R>& W) {
+  constexpr int sp = Cube<R>::sp;
+  C2<T>* S2 = lds + R * R * sp;
+  const int a = t % R, k2 = t / R;
+  C2<T> u[R];
+#pragma unroll
+  for (int b = 0; b < R; ++b) u[b] = lds[(a + R * b) * sp + k2];
+  dft_reg<T, R, -1>(u);
+#pragma unroll
+  for (int c = 0; c < R; ++c) S2[(k2 * R + c) * sp + a] = (c > 0 && a > 0) ? cmul(u[c], W.wb[c]) : u[c];
+}
+template <typename T, int R>
+ADSCF_HD void cube_c(C2<T>* lds, int t, const CubeTables<T, R>& W) {
+  constexpr int sp = Cube<R>::sp;
+  C2<T>* row = lds + R * R * sp + t * sp;  // (k2, c) = (t / R, t % R): row k2 R + c = t
+  C2<T> u[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) u[q] = row[q];
+  dft_reg<T, R, -1>(u);
+#pragma unroll
+  for (int d = 0; d < R; ++d) u[d] = cscale(u[d], W.k[d]);
+  dft_reg<T, R, +1>(u);
+#pragma unroll
+  for (int q = 0; q < R; ++q) row[q] = u[q];
+}
+template <typename T, int R>
+ADSCF_HD void cube_bi(C2<T>* lds, int t, const CubeTables<T, R>& W) {
+  constexpr int sp = Cube<R>::sp;
+  const C2<T>* S2 = lds + R * R * sp;
+  const int a = t % R, k2 = t / R;
+  C2<T> u[R];
+#pragma unroll
+  for (int c = 0; c < R; ++c) {
+    const C2<T> z = S2[(k2 * R + c) * sp + a];
+    u[c] = (c > 0 && a > 0) ? cmulc(z, W.wb[c]) : z;
+  }
+  dft_reg<T, R, +1>(u);
+#pragma unroll
+  for (int b = 0; b < R; ++b) lds[(a + R * b) * sp + k2] = u[b];
+}
+template <typename T, int R>
+ADSCF_HD void cube_ai(C2<T>* v, const C2<T>* lds, int t, const CubeTables<T, R>& W) {
+  constexpr int sp = Cube<R>::sp;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const C2<T> z = lds[t * sp + q];
+    v[q] = (q > 0 && t > 0) ? cmulc(z, W.wa[q]) : z;
+  }
+  dft_reg<T, R, +1>(v);
+}
+
+}  // namespace adscf

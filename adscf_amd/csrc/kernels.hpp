@@ -20,7 +20,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
+#include "fft_cube.hpp"
 #include "fft_line.hpp"
 #include "fft_reg.hpp"
 
@@ -268,6 +270,40 @@ constexpr int reg_tpl() {
   return N1 == 1 ? 1 : FourStep<N1, N2>::tpl;
 }
 
+// Register line engines of a pass: TPL threads per line, NP points per thread,
+// thread t (t < NACT) holds points t + STRIDE * j (j < NP); LDS elements per line.
+template <typename T, int N1, int N2>
+struct EngFour {  // n = N1 * N2 four-step (fft_reg.hpp)
+  static constexpr int NP = N2, TPL = reg_tpl<N1, N2>(), NACT = N1, STRIDE = N1;
+  static constexpr int LDS = N1 == 1 ? 0 : FourStep<N1, N2>::lds_elems;
+  using Tables = RegTables<T, N1, N2>;
+  static __device__ __forceinline__ void op(C2<T> (&v)[NP], C2<T>* lds, int t, const Tables& R) {
+    reg_line_op<T, N1, N2>(v, lds, t, R);
+  }
+};
+
+template <typename T, int R>
+struct EngCube {  // n = R^3 on R^2 threads (fft_cube.hpp)
+  static constexpr int NP = R, TPL = R * R, NACT = R * R, STRIDE = R * R;
+  static constexpr int LDS = Cube<R>::lds_elems;
+  using Tables = CubeTables<T, R>;
+  static __device__ __forceinline__ void op(C2<T> (&v)[NP], C2<T>* lds, int t, const Tables& W) {
+    cube_a<T, R>(v, lds, t, W);
+    __syncthreads();
+    cube_b<T, R>(lds, t, W);
+    __syncthreads();
+    cube_c<T, R>(lds, t, W);
+    __syncthreads();
+    cube_bi<T, R>(lds, t, W);
+    __syncthreads();
+    cube_ai<T, R>(v, lds, t, W);
+  }
+};
+
+// shape (N1, N2) of the schedule tables: N2 == 0 selects the cube engine with R = N1
+template <typename T, int N1, int N2>
+using PassEngine = typename std::conditional<N2 == 0, EngCube<T, N1>, EngFour<T, N1, N2 == 0 ? 1 : N2>>::type;
+
 // Pass variants fixed at compile time.  The four shapes of a propagator sweep
 // (first pass from q = 1, plain single-operator pass, merged pass with the mid
 // store, last pass with planar output) get kernels without runtime branches on
@@ -289,8 +325,8 @@ inline int pass_mode(const PassArgs<T>& p) {
   return PM_GENERIC;
 }
 
-// Same contract as k_line_pass; a line lives in registers of reg_tpl threads,
-// each holding the N2 points j1 + N1*j2 of its j1.  blockDim = reg_tpl * L.
+// Same contract as k_line_pass; a line lives in registers of ENG::TPL threads,
+// thread t < ENG::NACT holding the points t + STRIDE * j.  blockDim = TPL * L.
 // At the small grids of the SCF loop a pass has few waves per SIMD, so its
 // time is mostly memory latency: every global load of the pass (the stop flag,
 // the lines, the E factors of both steps, twiddles, multipliers and rescale
@@ -308,9 +344,9 @@ __device__ __forceinline__ void sto(V* base, unsigned off, V x) {
   *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + off) = x;
 }
 
-template <typename T, int N1, int N2, int MODE = PM_GENERIC>
+template <typename T, class ENG, int MODE = PM_GENERIC>
 __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, unsigned char* smem_raw) {
-  using FS = FourStep<N1, N2>;
+  constexpr int NP = ENG::NP, STRIDE = ENG::STRIDE;
   constexpr bool G = MODE == PM_GENERIC;
   const int in_kind = G ? a.in_kind : (MODE == PM_FIRST ? (int)IN_ONES : (int)IN_TMP);
   const int nops = G ? a.nops : (MODE == PM_MERGED ? 2 : 1);
@@ -318,7 +354,7 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
   const int pre_E = G ? a.pre_E : (MODE == PM_FIRST ? 1 : 0);
   const int post_E = G ? a.post_E : (MODE == PM_LAST ? 1 : 0);
   const bool two = G ? a.h2 != nullptr : true;  // second chain present
-  constexpr int TPL = reg_tpl<N1, N2>();
+  constexpr int TPL = ENG::TPL;
   const int tid = threadIdx.x;
   int l, t;
   if (a.axis == 0) {
@@ -332,18 +368,18 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
   const long base = pass_line_base(a, item, l, valid);
   // byte offsets of this thread's points j2 in a real (fo) and a complex (fc) array
   const unsigned f0 = (unsigned)(base + (long)t * a.stride) * (unsigned)sizeof(T);
-  const unsigned fstep = (unsigned)(N1 * a.stride) * (unsigned)sizeof(T);
+  const unsigned fstep = (unsigned)(STRIDE * a.stride) * (unsigned)sizeof(T);
   auto fo = [&](int j2) { return f0 + (unsigned)j2 * fstep; };
   auto fc = [&](int j2) { return 2u * (f0 + (unsigned)j2 * fstep); };
-  C2<T>* lds = reinterpret_cast<C2<T>*>(smem_raw) + l * FS::lds_elems;
-  const bool act = t < N1 && valid;
+  C2<T>* lds = reinterpret_cast<C2<T>*>(smem_raw) + l * ENG::LDS;
+  const bool act = t < ENG::NACT && valid;
   const int stop = *(a.done ? a.done : &g_no_stop);
   const bool useEa = pre_E != 0 || nops == 2 || out_kind == OUT_DOT || (out_kind == OUT_PLANAR && post_E);
   const bool useEb = nops == 2;
-  C2<T> v[N2];
-  T ea1[N2], ea2[N2], eb1[N2], eb2[N2];
+  C2<T> v[NP];
+  T ea1[NP], ea2[NP], eb1[NP], eb2[NP];
 #pragma unroll
-  for (int j2 = 0; j2 < N2; ++j2) {
+  for (int j2 = 0; j2 < NP; ++j2) {
     C2<T> x = {T(0), T(0)};
     ea1[j2] = ea2[j2] = eb1[j2] = eb2[j2] = T(1);
     if (act) {
@@ -366,32 +402,32 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
     }
     v[j2] = x;
   }
-  RegTables<T, N1, N2> R;
+  typename ENG::Tables R;
   R.load(t, a.tw, a.kf);
   const bool scaled = G ? a.dexp1 != nullptr : (MODE == PM_MERGED || MODE == PM_LAST);
   const T sc1 = scaled ? exp2i<T>(a.dexp1[a.sidx]) : T(1);
   const T sc2 = (G ? a.dexp2 != nullptr : scaled) ? exp2i<T>(a.dexp2[a.sidx]) : T(1);
   if (pre_E > 0) {
 #pragma unroll
-    for (int j2 = 0; j2 < N2; ++j2) {
+    for (int j2 = 0; j2 < NP; ++j2) {
       v[j2].x *= ea1[j2];
       v[j2].y *= ea2[j2];
     }
   } else if (pre_E < 0) {
 #pragma unroll
-    for (int j2 = 0; j2 < N2; ++j2) {
+    for (int j2 = 0; j2 < NP; ++j2) {
       v[j2].x /= ea1[j2];
       v[j2].y /= ea2[j2];
     }
   }
-  reg_line_op<T, N1, N2>(v, lds, t, R);
+  ENG::op(v, lds, t, R);
   // The stop flag is tested only here, after the first operator: its barriers keep
   // every load above in flight together, and nothing has been stored yet.
   if (stop) return;  // uniform over the block
   if (nops == 2) {
     if (act) {
 #pragma unroll
-      for (int j2 = 0; j2 < N2; ++j2) {
+      for (int j2 = 0; j2 < NP; ++j2) {
         C2<T> x = v[j2];
         x.x = x.x * ea1[j2] * sc1;
         x.y = x.y * ea2[j2] * sc2;
@@ -402,13 +438,13 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
         v[j2] = x;
       }
     }
-    reg_line_op<T, N1, N2>(v, lds, t, R);
+    ENG::op(v, lds, t, R);
   }
   if (out_kind == OUT_DOT) {
     double acc = 0.0;
     if (act) {
 #pragma unroll
-      for (int j2 = 0; j2 < N2; ++j2) acc += dot_term(a, base + (long)(t + N1 * j2) * a.stride, v[j2]);
+      for (int j2 = 0; j2 < NP; ++j2) acc += dot_term(a, base + (long)(t + STRIDE * j2) * a.stride, v[j2]);
     }
     block_dot_out(acc, a.dot_out);
     return;
@@ -416,10 +452,10 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
   if (!act) return;
   if (out_kind == OUT_TMP) {
 #pragma unroll
-    for (int j2 = 0; j2 < N2; ++j2) sto(a.tmp_out, fc(j2), v[j2]);
+    for (int j2 = 0; j2 < NP; ++j2) sto(a.tmp_out, fc(j2), v[j2]);
   } else {
 #pragma unroll
-    for (int j2 = 0; j2 < N2; ++j2) {
+    for (int j2 = 0; j2 < NP; ++j2) {
       C2<T> x = v[j2];
       if (post_E) {
         x.x = x.x * ea1[j2] * sc1;
@@ -459,7 +495,7 @@ template <typename T, int N1, int N2, int MODE>
 __global__ void __launch_bounds__(kThreads) k_line_pass_reg(PassArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
   if constexpr (MODE != PM_GENERIC) pin_pass_args(a);
-  reg_pass_body<T, N1, N2, MODE>(a, xcd_tile(blockIdx.x, gridDim.x), smem_raw);
+  reg_pass_body<T, PassEngine<T, N1, N2>, MODE>(a, xcd_tile(blockIdx.x, gridDim.x), smem_raw);
 }
 
 // One pass of many independent instances in one launch: blk[b] packs the
@@ -485,25 +521,26 @@ __global__ void __launch_bounds__(64) k_line_pass_batch(BatchPass<T> tb) {
   const PassArgs<T> a = tb.args[i];
   extern __shared__ unsigned char smem_raw[];
   switch (tb.shape[i]) {
-    case 408: reg_pass_body<T, 4, 8>(a, item, smem_raw); break;
-    case 804: reg_pass_body<T, 8, 4>(a, item, smem_raw); break;
-    case 216: reg_pass_body<T, 2, 16>(a, item, smem_raw); break;
-    case 806: reg_pass_body<T, 8, 6>(a, item, smem_raw); break;
-    case 608: reg_pass_body<T, 6, 8>(a, item, smem_raw); break;
-    case 1603: reg_pass_body<T, 16, 3>(a, item, smem_raw); break;
-    case 808: reg_pass_body<T, 8, 8>(a, item, smem_raw); break;
-    case 416: reg_pass_body<T, 4, 16>(a, item, smem_raw); break;
-    case 1604: reg_pass_body<T, 16, 4>(a, item, smem_raw); break;
-    case 816: reg_pass_body<T, 8, 16>(a, item, smem_raw); break;
-    case 1608: reg_pass_body<T, 16, 8>(a, item, smem_raw); break;
-    case 116: reg_pass_body<T, 1, 16>(a, item, smem_raw); break;
-    case 108: reg_pass_body<T, 1, 8>(a, item, smem_raw); break;
+    case 408: reg_pass_body<T, EngFour<T, 4, 8>>(a, item, smem_raw); break;
+    case 804: reg_pass_body<T, EngFour<T, 8, 4>>(a, item, smem_raw); break;
+    case 216: reg_pass_body<T, EngFour<T, 2, 16>>(a, item, smem_raw); break;
+    case 806: reg_pass_body<T, EngFour<T, 8, 6>>(a, item, smem_raw); break;
+    case 608: reg_pass_body<T, EngFour<T, 6, 8>>(a, item, smem_raw); break;
+    case 1603: reg_pass_body<T, EngFour<T, 16, 3>>(a, item, smem_raw); break;
+    case 808: reg_pass_body<T, EngFour<T, 8, 8>>(a, item, smem_raw); break;
+    case 416: reg_pass_body<T, EngFour<T, 4, 16>>(a, item, smem_raw); break;
+    case 1604: reg_pass_body<T, EngFour<T, 16, 4>>(a, item, smem_raw); break;
+    case 816: reg_pass_body<T, EngFour<T, 8, 16>>(a, item, smem_raw); break;
+    case 1608: reg_pass_body<T, EngFour<T, 16, 8>>(a, item, smem_raw); break;
+    case 116: reg_pass_body<T, EngFour<T, 1, 16>>(a, item, smem_raw); break;
+    case 108: reg_pass_body<T, EngFour<T, 1, 8>>(a, item, smem_raw); break;
+    case 400: reg_pass_body<T, EngCube<T, 4>>(a, item, smem_raw); break;
     default: break;
   }
 }
 
 constexpr int kBatchShapes[][2] = {{4, 8}, {8, 4}, {2, 16}, {8, 6}, {6, 8}, {16, 3}, {8, 8},
-                                   {4, 16}, {16, 4}, {8, 16}, {16, 8}, {1, 16}, {1, 8}};
+                                   {4, 16}, {16, 4}, {8, 16}, {16, 8}, {1, 16}, {1, 8}, {4, 0}};
 
 // ---------------------------------------------------------------------------
 // generic line FFT along one axis of a complex array (compat r2c / c2r)

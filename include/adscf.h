@@ -150,7 +150,8 @@ int adscf_scf_timing(adscf_ctx* ctx, double* ms_per_iter, double* ms_sweep);
  * wavefronts per workgroup, and the LDS Stockham kernel) and keeps the fastest.  Needs
  * adscf_scf_setup + adscf_scf_set_field.  report (may be NULL) receives the timings. */
 int adscf_autotune(adscf_ctx* ctx, int reps, char* report, int report_cap);
-/* current schedule of an axis (0 x, 1 y, 2 z); n1 = 0 means the Stockham kernel */
+/* current schedule of an axis (0 x, 1 y, 2 z): n1 x n2 four-step register engine, n1 = 0 the
+ * Stockham kernel, n2 = 0 the n1^3 cube engine (n = n1^3 points, n1^2 threads per line) */
 int adscf_get_schedule(const adscf_ctx* ctx, int axis, int* n, int* n1, int* n2, int* waves);
 int adscf_set_schedule(adscf_ctx* ctx, int axis, int n1, int n2, int waves);
 

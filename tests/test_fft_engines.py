@@ -1,7 +1,8 @@
-"""CPU host emulation of both device line-FFT engines: the LDS Stockham passes
-(adscf_amd/csrc/fft_line.hpp) and the register-resident four-step operator
-(adscf_amd/csrc/fft_reg.hpp) are compiled for the host with g++ and checked
-against direct long-double DFTs for every supported length."""
+"""CPU host emulation of the device line-FFT engines: the LDS Stockham passes
+(adscf_amd/csrc/fft_line.hpp), the register-resident four-step operator
+(adscf_amd/csrc/fft_reg.hpp) and the R^3 cube operator (adscf_amd/csrc/fft_cube.hpp)
+are compiled for the host with g++ and checked against direct long-double DFTs for
+every supported length."""
 import os
 import subprocess
 
@@ -10,7 +11,7 @@ import pytest
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-@pytest.mark.parametrize("src", ["fft_line_check.cpp", "fft_reg_check.cpp"])
+@pytest.mark.parametrize("src", ["fft_line_check.cpp", "fft_reg_check.cpp", "fft_cube_check.cpp"])
 def test_engine_host_emulation(src, tmp_path):
     exe = tmp_path / "check"
     subprocess.run(["g++", "-O2", "-std=c++17", os.path.join(HERE, "cpu", src), "-o", str(exe)], check=True)

@@ -226,7 +226,8 @@ def test_scf_converges_to_oracle_saddle_point(amd, orc):
 
 @pytest.mark.parametrize("n,shape", [(32, (4, 8)), (32, (8, 4)), (32, (2, 16)), (48, (8, 6)), (48, (6, 8)),
                                      (48, (16, 3)), (64, (8, 8)), (64, (4, 16)), (64, (16, 4)), (24, (8, 3)),
-                                     (24, (4, 6)), (128, (8, 16)), (128, (16, 8)), (16, (1, 16)), (64, (0, 0))])
+                                     (24, (4, 6)), (128, (8, 16)), (128, (16, 8)), (16, (1, 16)), (64, (0, 0)),
+                                     (64, (4, 0))])
 @pytest.mark.parametrize("waves", [1, 2, 4])
 def test_every_schedule_matches_oracle(amd, orc, n, shape, waves):
     if shape[0] == 0 and waves > 1:
