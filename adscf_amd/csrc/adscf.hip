@@ -356,16 +356,23 @@ RegLauncher<T> reg_launcher(int n, int N1, int N2) {
   return nullptr;
 }
 
-// default schedule for an axis of n points: the first registered shape, one wave per block
-void default_schedule(AxisTables& ax) {
+// Default schedule for an axis of n points: the first registered shape, one wave
+// per block; at n = 64 the choice the autotuner makes on the MI355X for the SCF
+// sweep (the cube engine for FP64 on every axis and for FP32 on x, 8 x 8 for FP32
+// on the strided y and z axes).
+void default_schedule(AxisTables& ax, int axis, int prec) {
   ax.N1 = ax.N2 = 0;
   ax.waves = 1;
   for (const RegShape& s : kRegShapes)
     if (s.n == ax.n) {
       ax.N1 = s.N1;
       ax.N2 = s.N2;
-      return;
+      break;
     }
+  if (ax.n == 64 && (prec == ADSCF_FP64 || axis == 0)) {
+    ax.N1 = 4;
+    ax.N2 = 0;
+  }
 }
 
 template <typename T>
@@ -1135,7 +1142,7 @@ void reset_axes(adscf_ctx* c) {
     c->ax[a].plan = make_line_plan(d[a]);
     if (c->ax[a].plan.npass < 0) throw ArgError{"grid dimension has more than 12 prime factors"};
     c->ax[a].L = choose_tile(a, d[a], c->nx, c->ny, c->nz);
-    default_schedule(c->ax[a]);
+    default_schedule(c->ax[a], a, c->prec);
   }
 }
 

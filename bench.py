@@ -42,7 +42,13 @@ def algorithmic_bytes_iter(ng, ns, m, p):
             + (3 * m + 6) * 2 * ng * 8)
 
 
-def run_gpu(adscf_amd, fields, precision, steps, warmup, w0, hipfft=True, tune=True):
+def shape_name(sch):
+    if sch["N1"] == 0:
+        return "stockham"
+    return "cube%d" % sch["N1"] if sch["N2"] == 0 else "%dx%d" % (sch["N1"], sch["N2"])
+
+
+def run_gpu(adscf_amd, fields, precision, steps, warmup, w0, hipfft=True, tune=False):
     segs = fields.block_segments([FA, 1 - FA], NS)
     ctx = adscf_amd.Context(N, N, N, L, L, L, 1.0 / NS, precision=precision)
     chi = [[0.0, CHIN], [CHIN, 0.0]]
@@ -51,7 +57,7 @@ def run_gpu(adscf_amd, fields, precision, steps, warmup, w0, hipfft=True, tune=T
     ctx.scf_set_field(w0)
     # per-axis line-pass schedule (shape, waves per workgroup)
     report = ctx.autotune(reps=3) if tune else "".join(
-        "axis %d chosen: %dx%d waves=%d (default)\n" % ((a,) + tuple(ctx.schedule(a)[k] for k in ("N1", "N2", "waves")))
+        "axis %d chosen: %s waves=%d (default)\n" % (a, shape_name(ctx.schedule(a)), ctx.schedule(a)["waves"])
         for a in range(3))
     ctx.scf_iterate(warmup, tol=-1.0)
     t0 = time.perf_counter()
@@ -125,6 +131,21 @@ def run_batched(adscf_amd, fields, steps, warmup):
                 schedules=[l for l in report.splitlines() if "chosen" in l])
 
 
+def profiled_traffic():
+    """Traffic per line-pass launch (bytes) of the sweep from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE runs, copy-calibrated); None if it is absent."""
+    path = os.path.join(ROOT, "profiles", "line_pass_64cubed_pmc.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("bench traffic_bytes_per_launch"):
+                    kv = dict(tok.split("=") for tok in line.split()[2:])
+                    return {k: float(v) for k, v in kv.items()}
+    except OSError:
+        pass
+    return None
+
+
 def copy_probe(mib=256, reps=3):
     """Plain device copies of known size for calibrating FETCH_SIZE / WRITE_SIZE against bytes:
     a contiguous copy of mib MiB (hipMemcpy's kernel, wide requests) and a copy that reads
@@ -167,7 +188,8 @@ def main():
     ap.add_argument("--converge-max-iter", type=int, default=4000)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-hipfft", action="store_true", help="skip the hipFFT comparator")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default line-pass schedule")
+    ap.add_argument("--autotune", action="store_true",
+                    help="re-tune the line-pass schedules first (the defaults are the tuner's MI355X choice)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 256 MiB device-to-device copy (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -183,16 +205,17 @@ def main():
     if args.copy_probe:
         copy_probe()
     ctx64, r64 = run_gpu(adscf_amd, fields, adscf_amd.FP64, args.steps, args.warmup, w0, not args.no_hipfft,
-                         not args.no_autotune)
+                         args.autotune)
     ctx64.close()
     ctx32, r32 = run_gpu(adscf_amd, fields, adscf_amd.FP32, args.steps, args.warmup, w0, not args.no_hipfft,
-                         not args.no_autotune)
+                         args.autotune)
     ctx32.close()
     npass = 2 * NS + 1
     sweep_bytes32 = 2 * NS * 3 * ng * 4
     sweep_bytes64 = 2 * NS * 3 * ng * 8
     ach32 = sweep_bytes32 / (r32["ms_sweep"] * 1e-3) / 1e9
     ach64 = sweep_bytes64 / (r64["ms_sweep"] * 1e-3) / 1e9
+    traffic = profiled_traffic()
     biter32 = algorithmic_bytes_iter(ng, NS, M, 4)
     biter64 = algorithmic_bytes_iter(ng, NS, M, 8)
     line = {
@@ -212,14 +235,16 @@ def main():
                                "kappa=25, Anderson m=20, lam_mix=0.5",
                    "grid": [N, N, N], "contour_steps": NS, "mixing_dim": M, "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach32 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_line_pass<float> (propagator line pass)",
+                     "frac": ach32 / HBM_PEAK_GBS, "traffic": traffic.get("fp32") if traffic else None,
+                     "traffic_source": "profiles/line_pass_64cubed_pmc.md" if traffic else None,
+                     "kernel": "k_line_pass_reg<float, *> (propagator line passes, all variants of the sweep)",
                      "bytes_per_launch": sweep_bytes32 / npass,
                      "us_per_launch": r32["ms_sweep"] * 1e3 / npass},
         "schedule": r32["schedule"],
         "fp64": {"iters_per_s": r64["iters_per_s"], "ms_per_iter": r64["ms_per_iter"], "schedule": r64["schedule"],
                  "roofline_achieved_GBs": ach64, "roofline_frac": ach64 / HBM_PEAK_GBS,
                  "us_per_pass": r64["ms_sweep"] * 1e3 / npass,
+                 "traffic_per_pass": traffic.get("fp64") if traffic else None,
                  "iteration_bytes_frac": biter64 * r64["iters_per_s"] / (HBM_PEAK_GBS * 1e9)},
         "fp32mixed_iteration_bytes_frac": biter32 * r32["iters_per_s"] / (HBM_PEAK_GBS * 1e9),
         "speedup_fp32mixed_vs_fp64": r32["iters_per_s"] / r64["iters_per_s"],
