@@ -57,6 +57,9 @@ SIGNATURES = {
     "adscf_expfield": (C.c_int, [_ctxp, _dp, _dp]),
     "adscf_propagate_step": (C.c_int, [_ctxp, _dp, _dp]),
     "adscf_propagate_chain": (C.c_int, [_ctxp, C.c_int, _ip, _ip, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "adscf_propagate_homopolymer": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+                                              _dp]),
+    "adscf_update_solvent": (C.c_int, [_ctxp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "adscf_set_interactions": (C.c_int, [_ctxp, C.c_double, _dp, _dp]),
     "adscf_mean_fields": (C.c_int, [_ctxp, _dp, _dp]),
     "adscf_interaction_energy": (C.c_int, [_ctxp, _dp, _dp]),
@@ -277,6 +280,27 @@ class Context:
         self._chk(self._lib.adscf_propagate_chain(self._h, nb, _pi(bt), _pi(bs), int(ensemble), C.byref(r),
                                                   C.byref(m), _p(e), _p(dist), _p(den), C.byref(q),
                                                   C.byref(en)))
+        return ChainResult(dist, den, q.value, m.value, r.value, en.value)
+
+    def propagate_homopolymer(self, monomer_type, segments, expfield, ensemble=1, rho=1.0, mu=0.0):
+        """Propagate_Homopolymer (mod_homopolymer.f90:90-141): distribution, density (nz,ny,nx)."""
+        e = _f64(expfield, (self.ntypes,) + self.shape)
+        dist = np.empty(self.shape)
+        den = np.empty(self.shape)
+        r, m, q, en = C.c_double(rho), C.c_double(mu), C.c_double(), C.c_double()
+        self._chk(self._lib.adscf_propagate_homopolymer(self._h, int(monomer_type), int(segments), int(ensemble),
+                                                        C.byref(r), C.byref(m), _p(e), _p(dist), _p(den),
+                                                        C.byref(q), C.byref(en)))
+        return ChainResult(dist, den, q.value, m.value, r.value, en.value)
+
+    def update_solvent(self, field, ensemble=1, rho=1.0, mu=0.0):
+        """Update_Solvent (mod_solvent.f90:85-108) for one field (nz,ny,nx)."""
+        w = _f64(field, self.shape)
+        dist = np.empty(self.shape)
+        den = np.empty(self.shape)
+        r, m, q, en = C.c_double(rho), C.c_double(mu), C.c_double(), C.c_double()
+        self._chk(self._lib.adscf_update_solvent(self._h, int(ensemble), C.byref(r), C.byref(m), _p(w), _p(dist),
+                                                 _p(den), C.byref(q), C.byref(en)))
         return ChainResult(dist, den, q.value, m.value, r.value, en.value)
 
     # -- Interactions_FloryHuggins ----------------------------------------
@@ -586,6 +610,60 @@ def Propagate_Multiblock(ctx: Context, copolymer: Copolymer, expfield):
 
 Propagate_Diblock = Propagate_Multiblock
 Propagate_Triblock = Propagate_Multiblock
+
+
+@dataclass
+class Homopolymer:
+    """Mirror of type homopolymer (mod_homopolymer.f90:17-29)."""
+    label: int
+    monomer_type: int
+    length: float
+    ensemble: int
+    rho: float
+    mu: float
+    segments: int = 0
+    energy: float = 0.0
+    Qsum: float = 0.0
+    distribution: np.ndarray = None
+    density: np.ndarray = None
+
+
+def Initialize_Homopolymer(ctx: Context, label, monomer_type, length, ensemble, rho, mu):
+    """Initialize_Homopolymer (mod_homopolymer.f90:41-80)."""
+    if ensemble not in (1, 2):
+        print("Invalid Ensemble")  # mod_homopolymer.f90:58
+    return Homopolymer(label, monomer_type, length, ensemble, rho, mu, _nint(length / ctx.ds))
+
+
+def Propagate_Homopolymer(ctx: Context, polymer: Homopolymer, expfield):
+    """Propagate_Homopolymer (mod_homopolymer.f90:90-141)."""
+    r = ctx.propagate_homopolymer(polymer.monomer_type, polymer.segments, expfield, polymer.ensemble, polymer.rho,
+                                  polymer.mu)
+    polymer.Qsum, polymer.mu, polymer.rho, polymer.energy = r.Qsum, r.mu, r.rho, r.energy
+    polymer.distribution, polymer.density = r.distribution, r.density
+    return polymer
+
+
+@dataclass
+class MonomericSolvent:
+    """Mirror of type monomeric_solvent (mod_solvent.f90:17-27)."""
+    label: int
+    monomer_type: int
+    ensemble: int
+    rho: float
+    mu: float
+    energy: float = 0.0
+    Qsum: float = 0.0
+    distribution: np.ndarray = None
+    density: np.ndarray = None
+
+
+def Update_Solvent(ctx: Context, solvent: MonomericSolvent, field):
+    """Update_Solvent (mod_solvent.f90:85-108); field is the solvent's own field (nz,ny,nx)."""
+    r = ctx.update_solvent(field, solvent.ensemble, solvent.rho, solvent.mu)
+    solvent.Qsum, solvent.mu, solvent.rho, solvent.energy = r.Qsum, r.mu, r.rho, r.energy
+    solvent.distribution, solvent.density = r.distribution, r.density
+    return solvent
 
 
 def Calculate_Mean_Fields(ctx: Context, density):

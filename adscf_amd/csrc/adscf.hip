@@ -1892,6 +1892,59 @@ int adscf_propagate_chain(adscf_ctx* c, int nblocks, const int* monomer_type, co
   });
 }
 
+int adscf_propagate_homopolymer(adscf_ctx* c, int monomer_type, int segments, int ensemble, double* rho, double* mu,
+                                const double* expfield, double* distribution, double* density, double* qsum,
+                                double* energy) {
+  // mod_homopolymer.f90:90-141 is the one-block chain: q+ runs the same steps as q, so the
+  // packed sweep's complementary half reproduces qq(mono - s) of :119-125
+  return adscf_propagate_chain(c, 1, &monomer_type, &segments, ensemble, rho, mu, expfield, distribution, density,
+                               qsum, energy);
+}
+
+int adscf_update_solvent(adscf_ctx* c, int ensemble, double* rho, double* mu, const double* field,
+                         double* distribution, double* density, double* qsum, double* energy) {
+  return guarded(c, [&]() -> int {
+    if (!rho || !mu || !field || !qsum || !energy) throw ArgError{"null argument"};
+    if (ensemble != 1 && ensemble != 2) {
+      c->err = "Invalid ensemble; Cannot calculate Density";  // mod_solvent.f90:105
+      return ADSCF_ERR_ENSEMBLE;
+    }
+    ensure_partials(c);
+    const long ng = c->ng;
+    c->io.ensure((size_t)3 * ng);
+    double* w = c->io.p;
+    double* dist = w + ng;
+    double* den = w + 2 * ng;
+    const int nb = grid_for(ng);
+    HIPCK(hipMemcpyAsync(w, field, ng * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_solvent_dist, dim3(nb), dim3(kThreads), 0, c->stream, (const double*)w, ng, dist,
+                       c->qpart.p);
+    std::vector<double> part(nb);
+    HIPCK(hipMemcpyAsync(part.data(), c->qpart.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[b];
+    *qsum = c->dvol * s;
+    const double f = ensemble == 1 ? *rho * c->volume / *qsum : exp(*mu);
+    hipLaunchKernelGGL(k_scale_sum, dim3(nb), dim3(kThreads), 0, c->stream, (const double*)dist, ng, f, den,
+                       c->spart.p);
+    HIPCK(hipMemcpyAsync(part.data(), c->spart.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (distribution) HIPCK(hipMemcpyAsync(distribution, dist, ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (density) HIPCK(hipMemcpyAsync(density, den, ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (ensemble == 1) {
+      *mu = log(*rho * c->volume / *qsum);
+      *energy = *rho * c->volume * (*mu - 1.0);
+    } else {
+      double t = 0.0;
+      for (int b = 0; b < nb; ++b) t += part[b];
+      *rho = t * c->dvol / c->volume;
+      *energy = -*rho * c->volume;
+    }
+    return ADSCF_OK;
+  });
+}
+
 int adscf_set_interactions(adscf_ctx* c, double kappa, const double* chi, const double* vol) {
   return guarded(c, [&] {
     if (!chi || !vol) throw ArgError{"null array"};

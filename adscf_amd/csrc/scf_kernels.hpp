@@ -141,6 +141,34 @@ __global__ void __launch_bounds__(kThreads) k_schedule(ChainDesc cd, const doubl
   if (rotate && st && threadIdx.x == 0) st->head = (st->head + m) % (m + 1);
 }
 
+// Update_Solvent (mod_solvent.f90:97-98): distribution = exp(-field), partial sums
+__global__ void __launch_bounds__(kThreads) k_solvent_dist(const double* __restrict__ w, long ng,
+                                                           double* __restrict__ dist, double* __restrict__ part) {
+  __shared__ double sh[kThreads];
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ng; i += (long)gridDim.x * blockDim.x) {
+    const double d = exp(-w[i]);
+    dist[i] = d;
+    acc += d;
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// density = f * distribution (mod_solvent.f90:100-106), with partial sums of the density
+__global__ void __launch_bounds__(kThreads) k_scale_sum(const double* __restrict__ dist, long ng, double f,
+                                                        double* __restrict__ den, double* __restrict__ part) {
+  __shared__ double sh[kThreads];
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ng; i += (long)gridDim.x * blockDim.x) {
+    const double d = f * dist[i];
+    den[i] = d;
+    acc += d;
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
 // partial sums of q'(Ns) for Qsum (mod_multiblock.f90:155)
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_qsum(const T* __restrict__ q, long ng, double* __restrict__ part,

@@ -76,6 +76,26 @@ module adscf_hip
       real(c_double), intent(out) :: distribution(*), density(*)
       real(c_double), intent(out) :: qsum, energy
     end function
+    integer(c_int) function adscf_propagate_homopolymer(ctx, monomer_type, segments, ensemble, rho, mu, &
+        expfield, distribution, density, qsum, energy) bind(C, name='adscf_propagate_homopolymer')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: monomer_type, segments, ensemble
+      real(c_double), intent(inout) :: rho, mu
+      real(c_double), intent(in) :: expfield(*)
+      real(c_double), intent(out) :: distribution(*), density(*)
+      real(c_double), intent(out) :: qsum, energy
+    end function
+    integer(c_int) function adscf_update_solvent(ctx, ensemble, rho, mu, field, distribution, density, qsum, &
+        energy) bind(C, name='adscf_update_solvent')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ensemble
+      real(c_double), intent(inout) :: rho, mu
+      real(c_double), intent(in) :: field(*)
+      real(c_double), intent(out) :: distribution(*), density(*)
+      real(c_double), intent(out) :: qsum, energy
+    end function
     integer(c_int) function adscf_set_interactions(ctx, kappa, chi, vol) bind(C, name='adscf_set_interactions')
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx

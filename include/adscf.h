@@ -90,6 +90,19 @@ int adscf_propagate_chain(adscf_ctx* ctx, int nblocks, const int* monomer_type, 
                           int ensemble, double* rho, double* mu, const double* expfield, double* distribution,
                           double* density, double* qsum, double* energy);
 
+/* ---- Polymer_Modules: Propagate_Homopolymer, Update_Solvent ---------------- */
+
+/* Propagate_Homopolymer (Polymer_Modules/mod_homopolymer.f90:90-141): one propagator of
+ * `segments` steps in monomer_type (1-based); distribution and density (nx,ny,nz), either
+ * may be NULL; rho, mu in/out; qsum, energy out; ensemble 1 or 2 (else ADSCF_ERR_ENSEMBLE). */
+int adscf_propagate_homopolymer(adscf_ctx* ctx, int monomer_type, int segments, int ensemble, double* rho,
+                                double* mu, const double* expfield, double* distribution, double* density,
+                                double* qsum, double* energy);
+/* Update_Solvent (Polymer_Modules/mod_solvent.f90:85-108): distribution = exp(-field) for one
+ * field (nx,ny,nz); Qsum, density and energy per ensemble; rho, mu in/out. */
+int adscf_update_solvent(adscf_ctx* ctx, int ensemble, double* rho, double* mu, const double* field,
+                         double* distribution, double* density, double* qsum, double* energy);
+
 /* ---- Interactions_FloryHuggins (Interaction_Modules/mod_FloryHuggins.f90) - */
 
 /* The values Initialize_Interactions reads from input_interactions (:37-66):

@@ -83,6 +83,8 @@ def _load():
         "orc_propagate_step": (None, [G, _dp, _dp]),
         "orc_propagate_chain": (C.c_int, [G, C.POINTER(_Chain), _dp, _dp, _dp, _dp, _dp]),
         "orc_block_segments": (C.c_int, [C.c_double, C.c_double]),
+        "orc_propagate_homopolymer": (C.c_int, [G, C.POINTER(_Chain), _dp, _dp, _dp]),
+        "orc_update_solvent": (C.c_int, [G, C.POINTER(_Chain), _dp, _dp, _dp]),
         "orc_mean_fields": (None, [G, C.POINTER(_FH), _dp, _dp]),
         "orc_interaction_energy": (C.c_double, [G, C.POINTER(_FH), _dp]),
         "orc_average_volume_fraction": (C.c_double, [G, C.POINTER(_FH), _dp]),
@@ -244,6 +246,35 @@ def propagate_chain(g: Grid, block_types, block_segs, expfield, ensemble=1, rho=
         raise ValueError("Invalid ensemble; Cannot calculate Density")
     return ChainResult(distribution=dist, density=den, Qsum=ch.Qsum, mu=ch.mu, rho=ch.rho,
                        energy=ch.energy, q=q, qdag=qd)
+
+
+def propagate_homopolymer(g: Grid, monomer_type, segments, expfield, ensemble=1, rho=1.0, mu=0.0):
+    """Propagate_Homopolymer (mod_homopolymer.f90:90-141).  Returns a ChainResult with
+    distribution, density (nz, ny, nx), Qsum, mu, rho, energy."""
+    bt = np.array([monomer_type], dtype=np.int32)
+    bs = np.array([segments], dtype=np.int32)
+    e = np.ascontiguousarray(expfield, dtype=np.float64)
+    dist = np.empty(g.shape)
+    den = np.empty(g.shape)
+    ch = _Chain(1, _pi(bt), _pi(bs), int(ensemble), float(rho), float(mu), 0.0, 0.0)
+    rc = lib().orc_propagate_homopolymer(C.byref(g._g), C.byref(ch), _p(e), _p(dist), _p(den))
+    if rc != 0:
+        raise ValueError("Invalid ensemble; Cannot calculate Density")
+    return ChainResult(distribution=dist, density=den, Qsum=ch.Qsum, mu=ch.mu, rho=ch.rho, energy=ch.energy)
+
+
+def update_solvent(g: Grid, field, ensemble=1, rho=1.0, mu=0.0):
+    """Update_Solvent (mod_solvent.f90:85-108) for one field (nz, ny, nx)."""
+    bt = np.array([1], dtype=np.int32)
+    bs = np.array([0], dtype=np.int32)
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    dist = np.empty(g.shape)
+    den = np.empty(g.shape)
+    ch = _Chain(1, _pi(bt), _pi(bs), int(ensemble), float(rho), float(mu), 0.0, 0.0)
+    rc = lib().orc_update_solvent(C.byref(g._g), C.byref(ch), _p(f), _p(dist), _p(den))
+    if rc != 0:
+        raise ValueError("Invalid ensemble; Cannot calculate Density")
+    return ChainResult(distribution=dist, density=den, Qsum=ch.Qsum, mu=ch.mu, rho=ch.rho, energy=ch.energy)
 
 
 class FH:

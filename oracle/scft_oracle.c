@@ -274,6 +274,92 @@ void orc_propagate_step(const orc_grid* g, double* prop, const double* expfield_
 }
 
 /* nint(block_length/ds): Fortran NINT rounds half away from zero (mod_multiblock.f90:85) */
+/* mod_homopolymer.f90:90-141 */
+int orc_propagate_homopolymer(const orc_grid* g, orc_chain* c, const double* expfield, double* distribution,
+                              double* density) {
+  long ng = (long)g->nx * g->ny * g->nz;
+  const int mono = c->block_segments[0];
+  double volume, dvol;
+  orc_volume_parameters(g, &volume, &dvol);
+  const double ds = g->ds;
+  double* qq = (double*)malloc(sizeof(double) * (size_t)ng * (size_t)(mono + 1));
+  double* prop = (double*)malloc(sizeof(double) * (size_t)ng);
+  if (!qq || !prop) {
+    free(qq);
+    free(prop);
+    return -2;
+  }
+  /* :106-115 */
+  for (long i = 0; i < ng; ++i) prop[i] = 1.0;
+  memcpy(qq, prop, sizeof(double) * (size_t)ng);
+  const double* E = expfield + (long)(c->monomer_type[0] - 1) * ng;
+  for (int s = 1; s <= mono; ++s) {
+    orc_propagate_step(g, prop, E);
+    memcpy(qq + (long)s * ng, prop, sizeof(double) * (size_t)ng);
+  }
+  /* :117 */
+  double sum = 0.0;
+  for (long i = 0; i < ng; ++i) sum += qq[(long)mono * ng + i];
+  c->Qsum = dvol * sum;
+  /* :119-125 */
+  for (long i = 0; i < ng; ++i) distribution[i] = 0.5 * ds * qq[i] * qq[(long)mono * ng + i];
+  for (int s = 1; s <= mono - 1; ++s)
+    for (long i = 0; i < ng; ++i)
+      distribution[i] = distribution[i] + ds * qq[(long)s * ng + i] * qq[(long)(mono - s) * ng + i];
+  for (long i = 0; i < ng; ++i) distribution[i] = distribution[i] + 0.5 * ds * qq[(long)mono * ng + i] * qq[i];
+  free(prop);
+  free(qq);
+  /* :127-138 */
+  if (c->ensemble == 1) {
+    for (long i = 0; i < ng; ++i) density[i] = c->rho * volume * distribution[i] / c->Qsum;
+    c->mu = log(c->rho * volume / c->Qsum);
+    c->energy = c->rho * volume * (c->mu - 1.0);
+  } else if (c->ensemble == 2) {
+    const double em = exp(c->mu);
+    double tot = 0.0;
+    for (long i = 0; i < ng; ++i) {
+      density[i] = em * distribution[i];
+      tot += density[i];
+    }
+    c->rho = tot * dvol / volume / (ds * mono);
+    c->energy = -c->rho * volume;
+  } else {
+    return -1;
+  }
+  return 0;
+}
+
+/* mod_solvent.f90:85-108 */
+int orc_update_solvent(const orc_grid* g, orc_chain* c, const double* field, double* distribution,
+                       double* density) {
+  long ng = (long)g->nx * g->ny * g->nz;
+  double volume, dvol;
+  orc_volume_parameters(g, &volume, &dvol);
+  double sum = 0.0;
+  for (long i = 0; i < ng; ++i) {
+    distribution[i] = exp(-field[i]);
+    sum += distribution[i];
+  }
+  c->Qsum = dvol * sum;
+  if (c->ensemble == 1) {
+    for (long i = 0; i < ng; ++i) density[i] = c->rho * volume * distribution[i] / c->Qsum;
+    c->mu = log(c->rho * volume / c->Qsum);
+    c->energy = c->rho * volume * (c->mu - 1.0);
+  } else if (c->ensemble == 2) {
+    const double em = exp(c->mu);
+    double tot = 0.0;
+    for (long i = 0; i < ng; ++i) {
+      density[i] = em * distribution[i];
+      tot += density[i];
+    }
+    c->rho = tot * dvol / volume;
+    c->energy = -c->rho * volume;
+  } else {
+    return -1;
+  }
+  return 0;
+}
+
 int orc_block_segments(double block_length, double ds) { return (int)lround(block_length / ds); }
 
 /* Propagate_Multiblock (mod_multiblock.f90:112-188) */

@@ -70,6 +70,18 @@ int orc_propagate_chain(const orc_grid* g, orc_chain* c, const double* expfield,
 /* block_segments = nint(block_length/ds) (mod_multiblock.f90:85, mod_diblock.f90:74) */
 int orc_block_segments(double block_length, double ds);
 
+/* ---- Homopolymer and solvent (Polymer_Modules/mod_homopolymer.f90, mod_solvent.f90) */
+/* Propagate_Homopolymer (mod_homopolymer.f90:90-141): one propagator qq(s), s = 0..segments,
+ * of monomer_type (1-based); distribution = trapezoid sum of qq(s) qq(mono - s); the
+ * ensemble normalisation of :127-138.  c->nblocks, monomer_type, block_segments give the
+ * single type and segments.  Returns 0, or -1 for an invalid ensemble. */
+int orc_propagate_homopolymer(const orc_grid* g, orc_chain* c, const double* expfield, double* distribution,
+                              double* density);
+/* Update_Solvent (mod_solvent.f90:85-108): distribution = exp(-field), Qsum = dvol sum,
+ * density and energy per ensemble (grand canonical rho without the ds*segments factor). */
+int orc_update_solvent(const orc_grid* g, orc_chain* c, const double* field, double* distribution,
+                       double* density);
+
 /* ---- Flory-Huggins interactions (Interaction_Modules/mod_FloryHuggins.f90) */
 typedef struct {
   int ncomp;

@@ -1,4 +1,6 @@
-"""The unchanged reference driver Main_Diblock_Melt (main_diblock.f90, its
+"""Unchanged reference drivers built against the Fortran replacement modules.
+
+Main_Diblock_Melt (main_diblock.f90, its
 mod_global.f90 with the 64 x 1 x 300 grid, sub_wrap.f90 and mod_mt.f90), built by
 adscf_amd/fortran/Makefile against the Fortran replacement modules and
 libadscf_hip.so, run on the GPU for one SCF iteration and checked against the
@@ -77,3 +79,81 @@ def test_unchanged_diblock_driver_one_iteration(tmp_path, orc):
     e_line = open(tmp_path / "energy_f07-1p.dat").read().split()
     energy = float(e_line[3].replace("D", "E"))
     assert abs(energy - e_ref) < 1e-10 * abs(e_ref)
+
+
+# ---- Main_Blend (homopolymer blend, Polymer_Modules/mod_homopolymer.f90) ----------------
+
+BLEND_EXE = os.path.join(ROOT, "adscf_amd", "fortran", "_build", "Main_Blend", "main_blend.exe")
+# the shipped Main_Blend/input_general with qinit = 3 (random fields from grnd) and the SCF
+# calculation (qepd = .false.); the shipped input_interactions (kappa = chi = 10)
+BLEND_GENERAL = """20.0d0\t\tsizex
+5.0d0\t\tsizey
+1.0d0\t\tsizez
+
+50.0d0\t\tnumber of chains A
+27.0d0 \t\tmu_scf A
+1.0d0\t\tlength of chain A
+
+50.0d0\t\tnumber of chains B
+27.0d0 \t\tmu_scf B
+1.0d0\t\tlength of chain B
+
+0.1d0\t\tds
+
+10\t\tnumber of timesteps
+0.001\t\ttimestep
+
+.false.\t\tqread
+3\t\tqinit
+.false.\t\tqepd
+"""
+BLEND_INTERACTIONS = """10.0d0        ! kappa
+
+10.0d0         ! chi
+
+1.0d0         ! volume of monomer_type 1
+1.0d0         ! volume of monomer_type 2
+"""
+
+
+def _read_rows_in_order(path, nx, ny, nz):
+    """dens.dat / field.dat of main_blend.f90:238-251: rows x*dx, y*dy, z*dz, value_1,
+    value_2 written with x fastest, then y, then z."""
+    rows = [l.split() for l in open(path) if l.strip()]
+    a = np.array([[float(v.replace("D", "E")) for v in r] for r in rows])
+    assert a.shape == (nx * ny * nz, 5)
+    return np.ascontiguousarray(a[:, 3:].T.reshape(2, nz, ny, nx))
+
+
+@pytest.mark.skipif(not os.path.exists(BLEND_EXE), reason="driver not built (adscf_amd/fortran/Makefile needs the reference tree)")
+def test_unchanged_blend_driver_scf(tmp_path, orc):
+    """The unchanged Main_Blend driver (main_blend.f90 with its 64 x 16 x 1 mod_global.f90)
+    runs Find_SCF_Saddle_Point (lambda mixing, accuracy sum|dfield| dvol < 1e-7,
+    main_blend.f90:437-486) on the GPU homopolymer path from random fields, then writes
+    dens.dat and field.dat (:238-251).  Checks, with the oracle's Propagate_Homopolymer:
+    dens.dat are the canonical densities of field.dat, field.dat is self-consistent to the
+    driver's tolerance, and the printed energy and chemical potentials (:213-229) follow."""
+    (tmp_path / "input_general").write_text(BLEND_GENERAL)
+    (tmp_path / "input_interactions").write_text(BLEND_INTERACTIONS)
+    res = subprocess.run([BLEND_EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(res.stdout[-2000:], res.stderr[-2000:])
+    assert res.returncode == 0
+    assert "SCF saddle point found" in res.stdout
+    nx, ny, nz = 64, 16, 1
+    dens = _read_rows_in_order(tmp_path / "dens.dat", nx, ny, nz)
+    field = _read_rows_in_order(tmp_path / "field.dat", nx, ny, nz)
+    g = orc.Grid(nx, ny, nz, 20.0, 5.0, 1.0, 0.1)
+    vol, dvol = g.volume()
+    E = orc.expfield(g, field)
+    pol = [orc.propagate_homopolymer(g, t, 10, E, ensemble=1, rho=50.0 / vol) for t in (1, 2)]
+    den_ref = np.stack([p.density for p in pol])
+    assert np.max(np.abs(dens - den_ref)) < 1e-10 * np.max(den_ref)
+    fh = orc.FH(10.0, [[0.0, 10.0], [10.0, 0.0]], [1.0, 1.0])
+    newfield = orc.mean_fields(g, fh, den_ref)
+    assert np.sum(np.abs(newfield - field)) * dvol < 2e-7
+    e_ref = -dvol * np.sum(den_ref * field) + sum(p.energy for p in pol) + orc.interaction_energy(g, fh, den_ref)
+    nums = [float(t.replace("D", "E")) for t in res.stdout.split("Energy:")[-1].replace("C:", " ").replace(
+        "mu_scf:", " ").split()]
+    energy, mu1, mu2 = nums[0], nums[3], nums[4]
+    assert abs(energy - e_ref) < 1e-10 * abs(e_ref)
+    assert abs(mu1 - pol[0].mu) < 1e-10 and abs(mu2 - pol[1].mu) < 1e-10

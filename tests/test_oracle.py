@@ -394,3 +394,55 @@ def test_scf_anderson_converged_state_matches_numpy_loop(orc):
     F2 = r2["energy"] - dvol * np.sum(den2 * f2) + np_ref.interaction_energy(den2, 25.0, chi, [1.0, 1.0], dvol)
     assert abs(F1 - F2) < 1e-10 * abs(F1)
     assert np.max(np.abs(den - den2)) < 1e-7  # both within the 1e-8 field residual of the saddle point
+
+
+# ---- Propagate_Homopolymer and Update_Solvent (mod_homopolymer.f90, mod_solvent.f90) ----
+
+def test_homopolymer_equals_one_block_chain(orc):
+    """The single-propagator restatement (q+ = q) and the one-block multiblock restatement
+    (separate forward and complementary sweeps) are two independent codings of the same
+    sum; they must agree to rounding in both ensembles."""
+    g = orc.Grid(8, 6, 5, 2.0, 1.7, 1.5, 0.05)
+    rng = np.random.default_rng(11)
+    E = orc.expfield(g, rng.standard_normal((2,) + g.shape))
+    for ens, rho, mu in ((1, 0.8, 0.0), (2, 1.0, 0.4)):
+        h = orc.propagate_homopolymer(g, 2, 13, E, ensemble=ens, rho=rho, mu=mu)
+        c = orc.propagate_chain(g, [2], [13], E, ensemble=ens, rho=rho, mu=mu)
+        assert np.max(np.abs(h.distribution - c.distribution[0])) < 1e-14 * np.max(c.distribution)
+        assert np.max(np.abs(h.density - c.density[0])) < 1e-14 * np.max(c.density)
+        for k in ("Qsum", "mu", "rho", "energy"):
+            assert abs(getattr(h, k) - getattr(c, k)) <= 1e-14 * max(1.0, abs(getattr(c, k)))
+
+
+def test_homopolymer_constant_field_closed_form(orc):
+    """w = c: q(s) = exp(-c s ds), Q = V exp(-c N ds), distribution = N ds exp(-c N ds),
+    canonical density rho N ds and mu = ln(rho V / Q)."""
+    g = orc.Grid(6, 4, 5, 2.0, 1.5, 1.7, 0.1)
+    c, n = 0.7, 9
+    E = orc.expfield(g, np.full((1,) + g.shape, c))
+    h = orc.propagate_homopolymer(g, 1, n, E, ensemble=1, rho=1.3)
+    vol = 2.0 * 1.5 * 1.7
+    assert abs(h.Qsum - vol * np.exp(-c * n * 0.1)) < 1e-13 * h.Qsum
+    assert np.allclose(h.distribution, n * 0.1 * np.exp(-c * n * 0.1), rtol=1e-13, atol=0)
+    assert np.allclose(h.density, 1.3 * n * 0.1, rtol=1e-13, atol=0)
+    assert abs(h.mu - np.log(1.3 * vol / h.Qsum)) < 1e-14
+
+
+def test_solvent_closed_forms(orc):
+    """Update_Solvent: distribution = exp(-w); canonical density rho V exp(-w)/Q with
+    Q = dvol sum exp(-w); grand canonical rho = dvol sum exp(mu - w) / V."""
+    g = orc.Grid(6, 5, 4, 2.0, 1.5, 1.2, 0.1)
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal(g.shape)
+    vol, dvol = g.volume()
+    s1 = orc.update_solvent(g, w, ensemble=1, rho=0.6)
+    q = dvol * np.exp(-w).sum()
+    assert np.allclose(s1.distribution, np.exp(-w), rtol=2e-16, atol=0)
+    assert abs(s1.Qsum - q) < 1e-14 * q
+    assert np.allclose(s1.density, 0.6 * vol * np.exp(-w) / q, rtol=1e-14, atol=0)
+    assert abs(s1.energy - 0.6 * vol * (np.log(0.6 * vol / q) - 1.0)) < 1e-13
+    s2 = orc.update_solvent(g, w, ensemble=2, mu=0.3)
+    assert abs(s2.rho - dvol * np.exp(0.3 - w).sum() / vol) < 1e-14
+    assert abs(s2.energy + s2.rho * vol) < 1e-13
+    with pytest.raises(ValueError):
+        orc.update_solvent(g, w, ensemble=3)
