@@ -357,9 +357,9 @@ RegLauncher<T> reg_launcher(int n, int N1, int N2) {
 }
 
 // Default schedule for an axis of n points: the first registered shape, one wave
-// per block; at n = 64 the choice the autotuner makes on the MI355X for the SCF
-// sweep (the cube engine for FP64 on every axis and for FP32 on x, 8 x 8 for FP32
-// on the strided y and z axes).
+// per block; at n = 64 and 128 the choice the autotuner makes on the MI355X for the
+// SCF sweep (n = 64: the cube engine for FP64 on every axis and for FP32 on x, 8 x 8
+// for FP32 on the strided y and z axes; n = 128: 16 x 8).
 void default_schedule(AxisTables& ax, int axis, int prec) {
   ax.N1 = ax.N2 = 0;
   ax.waves = 1;
@@ -372,6 +372,10 @@ void default_schedule(AxisTables& ax, int axis, int prec) {
   if (ax.n == 64 && (prec == ADSCF_FP64 || axis == 0)) {
     ax.N1 = 4;
     ax.N2 = 0;
+  }
+  if (ax.n == 128) {
+    ax.N1 = 16;
+    ax.N2 = 8;
   }
 }
 

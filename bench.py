@@ -162,6 +162,35 @@ def copy_probe(mib=256, reps=3):
     del x, y, z
 
 
+def run_large(adscf_amd, fields, steps, warmup):
+    """The large mixed-precision configuration, 128^3 x 400 ABA triblock (0.25/0.5/0.25,
+    chiN = 25, box 6.0, BCC seed, m = 20): iterations/s and the line-pass roofline for
+    FP32-mixed and FP64 (default schedules)."""
+    n, L, ns, chin = 128, 6.0, 400, 25.0
+    segs = fields.block_segments([0.25, 0.5, 0.25], ns)
+    w0 = fields.bcc_seed(n, L, chin)
+    out = {}
+    for name, prec, p in (("fp32mixed", adscf_amd.FP32, 4), ("fp64", adscf_amd.FP64, 8)):
+        with adscf_amd.Context(n, n, n, L, L, L, 1.0 / ns, precision=prec) as ctx:
+            ctx.set_interactions(KAPPA, [[0.0, chin], [chin, 0.0]], [1.0, 1.0])
+            ctx.scf_setup([1, 2, 1], segs, ensemble=1, rho=1.0, mixing_dim=M, lam_mix=LAM)
+            ctx.scf_set_field(w0)
+            ctx.scf_iterate(warmup, tol=-1.0)
+            t0 = time.perf_counter()
+            st = ctx.scf_iterate(steps, tol=-1.0)
+            wall = time.perf_counter() - t0
+            _, ms_sweep = ctx.scf_timing()
+        npass = 2 * ns + 1
+        bytes_launch = 2 * ns * 3 * n ** 3 * p / npass
+        ach = bytes_launch / (ms_sweep * 1e-3 / npass) / 1e9
+        out[name] = dict(iters_per_s=steps / wall, ms_per_iter=1e3 * wall / steps, us_per_pass=1e3 * ms_sweep / npass,
+                         roofline_achieved_GBs=ach, roofline_frac=ach / HBM_PEAK_GBS,
+                         contour_steps_per_s=2 * ns * steps / wall, timing_valid=not st.converged)
+    out["speedup_fp32mixed_vs_fp64"] = out["fp32mixed"]["iters_per_s"] / out["fp64"]["iters_per_s"]
+    out["config"] = "ABA triblock melt SCF, 128^3 grid, box 6.0 Rg, Ns=400 (100/200/100), chiN=25, kappa=25, m=20"
+    return out
+
+
 def cpu_baseline(fields, w0):
     """The CPU FP64 oracle (single thread) on one SCF iteration of the same workload."""
     import oracle
@@ -187,6 +216,7 @@ def main():
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--converge-max-iter", type=int, default=4000)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-large", action="store_true", help="skip the 128^3 x 400 configuration")
     ap.add_argument("--no-hipfft", action="store_true", help="skip the hipFFT comparator")
     ap.add_argument("--autotune", action="store_true",
                     help="re-tune the line-pass schedules first (the defaults are the tuner's MI355X choice)")
@@ -259,6 +289,8 @@ def main():
                                   "fp64": {"adscf": r64["ms_sweep"] * 1e3 / (2 * NS), "hipfft": r64["hipfft_step_us"]}}}
     if not args.no_batched:
         line["batched"] = run_batched(adscf_amd, fields, max(5, args.steps // 5), 2)
+    if not args.no_large:
+        line["large"] = run_large(adscf_amd, fields, max(5, args.steps // 10), 3)
     if not args.no_converge:
         line["convergence"] = converge(adscf_amd, fields, w0, args.converge_max_iter)
     if not args.no_cpu_baseline:
