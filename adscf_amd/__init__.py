@@ -80,6 +80,14 @@ SIGNATURES = {
     "adscf_scf_energy": (C.c_int, [_ctxp, _dp, _dp, _dp, _dp, _dp]),
     "adscf_scf_residuals": (C.c_int, [_ctxp, _dp, C.c_int, _ip]),
     "adscf_scf_timing": (C.c_int, [_ctxp, _dp, _dp]),
+    "adscf_scf_set_residual": (C.c_int, [_ctxp, C.c_int, C.c_int, _dp]),
+    "adscf_conjugate_fields": (C.c_int, [_ctxp, _dp, C.c_double, C.c_int, C.c_int, _ip, _dp]),
+    "adscf_mobility_qq": (C.c_int, [_ctxp, C.c_int, C.c_double, _dp, C.c_double, _dp]),
+    "adscf_ddft_set_mobility": (C.c_int, [_ctxp, _dp]),
+    "adscf_ddft_set_density": (C.c_int, [_ctxp, _dp]),
+    "adscf_ddft_get_density": (C.c_int, [_ctxp, _dp]),
+    "adscf_ddft_step": (C.c_int, [_ctxp, C.c_double, C.c_int, C.c_double, C.c_int, _ip, _ip]),
+    "adscf_ddft_energy": (C.c_int, [_ctxp, _dp, _dp, _dp]),
     "adscf_autotune": (C.c_int, [_ctxp, C.c_int, C.c_char_p, C.c_int]),
     "adscf_get_schedule": (C.c_int, [_ctxp, C.c_int, _ip, _ip, _ip, _ip]),
     "adscf_set_schedule": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -383,6 +391,61 @@ class Context:
         a, b = C.c_double(), C.c_double()
         self._chk(self._lib.adscf_scf_timing(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def scf_set_residual(self, kind=0, measure=0, target=None):
+        """Residual of scf_iterate: kind 0 mean fields - field, kind 1 densities - target;
+        measure 0 max|dfield|, 1 sum|dfield| dvol."""
+        t = None if target is None else _f64(target, (self.ntypes,) + self.shape)
+        self._chk(self._lib.adscf_scf_set_residual(self._h, int(kind), int(measure), None if t is None else _p(t)))
+
+    # -- conjugate fields and DDFT (main_multiblock.f90) -------------------------
+    def conjugate_fields(self, density=None, thresh=5e-4, maxit=2000, measure=0):
+        """Get_Conjugate_Fields (main_multiblock.f90:566-640) from the current field; density None
+        uses the device-resident DDFT density.  Returns (iterations, accuracy)."""
+        d = None if density is None else _f64(density, (self.ntypes,) + self.shape)
+        it, acc = C.c_int(), C.c_double()
+        self._chk(self._lib.adscf_conjugate_fields(self._h, None if d is None else _p(d), float(thresh), int(maxit),
+                                                   int(measure), C.byref(it), C.byref(acc)))
+        return it.value, acc.value
+
+    def mobility_qq(self, table, dq, wcutoff):
+        """Read_Mobilities' table part (main_multiblock.f90:486-504).  table: (nt, nt, nq) with
+        table[t2, t1, iq-1] = mobility_table(iq, t1, t2).  Uploads the result for ddft_step and
+        returns it as (nt, nt, nz, ny, nx/2+1)."""
+        t = np.ascontiguousarray(table, dtype=np.float64)
+        nt, nq = t.shape[0], t.shape[2]
+        if t.shape != (self.ntypes, self.ntypes, nq):
+            raise ValueError("table must be (ntypes, ntypes, nq)")
+        out = np.empty((nt, nt) + self.kshape)
+        self._chk(self._lib.adscf_mobility_qq(self._h, int(nq), float(dq), _p(t), float(wcutoff), _p(out)))
+        return out
+
+    def ddft_set_mobility(self, mob_qq):
+        m = _f64(mob_qq, (self.ntypes, self.ntypes) + self.kshape)
+        self._chk(self._lib.adscf_ddft_set_mobility(self._h, _p(m)))
+
+    def ddft_set_density(self, density):
+        d = _f64(density, (self.ntypes,) + self.shape)
+        self._chk(self._lib.adscf_ddft_set_density(self._h, _p(d)))
+
+    def ddft_get_density(self):
+        d = np.empty((self.ntypes,) + self.shape)
+        self._chk(self._lib.adscf_ddft_get_density(self._h, _p(d)))
+        return d
+
+    def ddft_step(self, timestep, qdmin=True, accuracy_scf=1e-3, itermax_scf=2000):
+        """DDFT_Timestep_Euler (main_multiblock.f90:794-880) on the device-resident density and
+        field.  Returns dict(gcfstat=[s1, s2], iterations=[i1, i2])."""
+        st = np.zeros(2, dtype=np.int32)
+        its = np.zeros(2, dtype=np.int32)
+        self._chk(self._lib.adscf_ddft_step(self._h, float(timestep), int(bool(qdmin)), float(accuracy_scf),
+                                            int(itermax_scf), _pi(st), _pi(its)))
+        return dict(gcfstat=st.tolist(), iterations=its.tolist())
+
+    def ddft_energy(self):
+        F, mu, rho = C.c_double(), C.c_double(), C.c_double()
+        self._chk(self._lib.adscf_ddft_energy(self._h, C.byref(F), C.byref(mu), C.byref(rho)))
+        return dict(F=F.value, mu=mu.value, rho=rho.value)
 
 
     def hipfft_step_us(self, reps=200):

@@ -124,6 +124,14 @@ struct adscf_ctx {
   std::vector<const void*> graph_key;  // buffers baked into the captured iteration
   double last_ms_iter = 0, last_ms_sweep = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // residual of the SCF loop: 0 mean field - field, 1 density - target (conjugate fields);
+  // measure 0 max|dfield|, 1 sum|dfield| dvol
+  int resid_kind = 0, measure = 0, graph_mode = -1;
+  int user_kind = 0, user_measure = 0;  // the loop adscf_scf_iterate runs (adscf_scf_set_residual)
+  DevBuf<double> target;             // target densities (nt, ng): the DDFT density
+  DevBuf<double> mob;                // mobility_qq (nk, nt, nt)
+  DevBuf<double> dmu, dmuq, ddq, dd; // DDFT step work arrays
+  bool target_set = false, mob_set = false;
 };
 
 namespace {
@@ -729,8 +737,11 @@ FftArgs<T> fft_args(adscf_ctx* c, int axis, int d0, int dir) {
   return a;
 }
 
+// Real2Fourier on device arrays: din (nx,ny,nz) doubles -> dout (nx/2+1,ny,nz) complex
+// as interleaved doubles, divided by nx*ny*nz (mod_fourier.f90:94-105).  din and dout may
+// be the same buffer.
 template <typename T>
-int r2c_impl(adscf_ctx* c, const double* prop, double* out) {
+void enqueue_r2c(adscf_ctx* c, const double* din, double* dout) {
   ensure_tables<T>(c);
   Engine<T>& En = eng<T>(c);
   const long ng = c->ng;
@@ -738,9 +749,7 @@ int r2c_impl(adscf_ctx* c, const double* prop, double* out) {
   const long nk = (long)nh * c->ny * c->nz;
   En.cq.ensure(2 * ng);
   En.ch.ensure(nk);
-  c->io.ensure(std::max(ng, 2 * nk));
-  HIPCK(hipMemcpyAsync(c->io.p, prop, ng * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_convert<T>, dim3(grid_for(ng)), dim3(kThreads), 0, c->stream, c->io.p, En.cq.p, ng);
+  hipLaunchKernelGGL(k_convert<T>, dim3(grid_for(ng)), dim3(kThreads), 0, c->stream, din, En.cq.p, ng);
   FftArgs<T> a = fft_args<T>(c, 0, nh, -1);
   a.load = LOAD_REAL;
   a.rin = En.cq.p;
@@ -758,7 +767,8 @@ int r2c_impl(adscf_ctx* c, const double* prop, double* out) {
     if (!doz) b.scale = inv;
     launch_fft<T>(c, b);
   }
-  if (doz) {
+  if (doz || !doy) {
+    // the z pass applies the scale; with ny = nz = 1 it is a trivial pass of length 1
     FftArgs<T> b = fft_args<T>(c, 2, nh, -1);
     b.load = LOAD_CPLX;
     b.cin = En.ch.p;
@@ -767,25 +777,15 @@ int r2c_impl(adscf_ctx* c, const double* prop, double* out) {
     b.scale = inv;
     launch_fft<T>(c, b);
   }
-  if (!doy && !doz) {
-    // scale through a trivial z "pass" of length 1
-    FftArgs<T> b = fft_args<T>(c, 2, nh, -1);
-    b.load = LOAD_CPLX;
-    b.cin = En.ch.p;
-    b.store = STORE_CPLX;
-    b.cout = En.ch.p;
-    b.scale = inv;
-    launch_fft<T>(c, b);
-  }
-  hipLaunchKernelGGL(k_widen<T>, dim3(grid_for(2 * nk)), dim3(kThreads), 0, c->stream, (const T*)En.ch.p, c->io.p,
+  hipLaunchKernelGGL(k_widen<T>, dim3(grid_for(2 * nk)), dim3(kThreads), 0, c->stream, (const T*)En.ch.p, dout,
                      2 * nk);
-  HIPCK(hipMemcpyAsync(out, c->io.p, 2 * nk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCK(hipStreamSynchronize(c->stream));
-  return ADSCF_OK;
+  HIPCK(hipGetLastError());
 }
 
+// Fourier2Real on device arrays (unnormalised c2r, mod_fourier.f90:75-86); din and dout
+// may be the same buffer.
 template <typename T>
-int c2r_impl(adscf_ctx* c, const double* in, double* prop) {
+void enqueue_c2r(adscf_ctx* c, const double* din, double* dout) {
   ensure_tables<T>(c);
   Engine<T>& En = eng<T>(c);
   const long ng = c->ng;
@@ -793,10 +793,7 @@ int c2r_impl(adscf_ctx* c, const double* in, double* prop) {
   const long nk = (long)nh * c->ny * c->nz;
   En.cq.ensure(2 * ng);
   En.ch.ensure(nk);
-  c->io.ensure(std::max(ng, 2 * nk));
-  HIPCK(hipMemcpyAsync(c->io.p, in, 2 * nk * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_convert<T>, dim3(grid_for(2 * nk)), dim3(kThreads), 0, c->stream, c->io.p, (T*)En.ch.p,
-                     2 * nk);
+  hipLaunchKernelGGL(k_convert<T>, dim3(grid_for(2 * nk)), dim3(kThreads), 0, c->stream, din, (T*)En.ch.p, 2 * nk);
   if (c->nz > 1) {
     FftArgs<T> b = fft_args<T>(c, 2, nh, +1);
     b.load = LOAD_CPLX;
@@ -819,8 +816,28 @@ int c2r_impl(adscf_ctx* c, const double* in, double* prop) {
   a.store = STORE_REAL;
   a.rout = En.cq.p;
   launch_fft<T>(c, a);
-  hipLaunchKernelGGL(k_widen<T>, dim3(grid_for(ng)), dim3(kThreads), 0, c->stream, (const T*)En.cq.p, c->io.p, ng);
-  HIPCK(hipMemcpyAsync(prop, c->io.p, ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  hipLaunchKernelGGL(k_widen<T>, dim3(grid_for(ng)), dim3(kThreads), 0, c->stream, (const T*)En.cq.p, dout, ng);
+  HIPCK(hipGetLastError());
+}
+
+template <typename T>
+int r2c_impl(adscf_ctx* c, const double* prop, double* out) {
+  const long nk = (long)(c->nx / 2 + 1) * c->ny * c->nz;
+  c->io.ensure(std::max(c->ng, 2 * nk));
+  HIPCK(hipMemcpyAsync(c->io.p, prop, c->ng * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  enqueue_r2c<T>(c, c->io.p, c->io.p);
+  HIPCK(hipMemcpyAsync(out, c->io.p, 2 * nk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return ADSCF_OK;
+}
+
+template <typename T>
+int c2r_impl(adscf_ctx* c, const double* in, double* prop) {
+  const long nk = (long)(c->nx / 2 + 1) * c->ny * c->nz;
+  c->io.ensure(std::max(c->ng, 2 * nk));
+  HIPCK(hipMemcpyAsync(c->io.p, in, 2 * nk * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  enqueue_c2r<T>(c, c->io.p, c->io.p);
+  HIPCK(hipMemcpyAsync(prop, c->io.p, c->ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
   return ADSCF_OK;
 }
@@ -842,6 +859,8 @@ void enqueue_iteration(adscf_ctx* c, bool passes_too = true) {
   a.dring = c->dring.p;
   a.done = done;
   a.w0 = c->wring.p;  // ring base; the kernel reads slot `head` from the device state
+  a.target = c->resid_kind ? c->target.p : nullptr;
+  a.measure = c->measure;
   const int nbc = launch_contour<T>(c->stream, a);
   HIPCK(hipGetLastError());
   const int m = c->m;
@@ -860,6 +879,7 @@ void enqueue_iteration(adscf_ctx* c, bool passes_too = true) {
   s.ds = c->ds;
   s.ns = c->cd.ns;
   s.ensemble = c->cd.ensemble;
+  s.measure = c->measure;
   s.st = c->st.p;
   hipLaunchKernelGGL(k_solve, dim3(1), dim3(kThreads), 0, c->stream, s);
   hipLaunchKernelGGL(k_mix<T>, dim3(nb), dim3(kThreads), 0, c->stream, c->wring.p, (const double*)c->dring.p, ng,
@@ -918,7 +938,7 @@ std::vector<const void*> graph_key(adscf_ctx* c) {
   Engine<T>& En = eng<T>(c);
   return {En.E.p, En.tmp.p, En.h1.p, En.h2.p, En.tw[0].p, En.tw[1].p, En.tw[2].p, En.kf[0].p, En.kf[1].p,
           En.kf[2].p, c->wring.p, c->dring.p, c->phi.p, c->qpart.p, c->rpart.p, c->spart.p, c->e2part.p,
-          c->gpart.p, c->gsum.p, c->resid.p, c->e1.p, c->e2.p, c->d1.p, c->d2.p, c->st.p};
+          c->gpart.p, c->gsum.p, c->resid.p, c->e1.p, c->e2.p, c->d1.p, c->d2.p, c->st.p, c->target.p};
 }
 
 // E and the rescale schedule of the current field in precision T (the history is kept)
@@ -947,7 +967,8 @@ template <typename T>
 void build_graph(adscf_ctx* c) {
   ensure_tables<T>(c);
   std::vector<const void*> key = graph_key<T>(c);
-  if (c->graph && c->graph_prec == c->prec && key == c->graph_key) return;
+  const int mode = c->resid_kind * 2 + c->measure;
+  if (c->graph && c->graph_prec == c->prec && c->graph_mode == mode && key == c->graph_key) return;
   if (c->graph) {
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
@@ -968,6 +989,7 @@ void build_graph(adscf_ctx* c) {
   HIPCK(hipStreamEndCapture(c->stream, &c->graph_src));
   HIPCK(hipGraphInstantiate(&c->graph, c->graph_src, nullptr, nullptr, 0));
   c->graph_prec = c->prec;
+  c->graph_mode = mode;
   c->graph_key = key;
 }
 
@@ -979,7 +1001,10 @@ int scf_iterate_impl(adscf_ctx* c, int niter, double tol, int* iters, double* ac
   int done_iters = 0;
   float ms_total = 0.f;
   while (done_iters < niter) {
-    const int chunk = std::min(niter - done_iters, tol > 0 ? 25 : niter);
+    // with a tolerance, the stop flag is read after 1, 1, 2, 4, ... 32 iterations, so a
+    // short solve (the conjugate-field solves of a DDFT step take ~10-25) replays few
+    // graphs past its exit; without one the whole count is queued at once
+    const int chunk = std::min(niter - done_iters, tol > 0 ? std::max(1, std::min(32, done_iters)) : niter);
     HIPCK(hipEventRecord(c->ev0, c->stream));
     for (int i = 0; i < chunk; ++i) {
       if (c->use_graph) HIPCK(hipGraphLaunch(c->graph, c->stream));
@@ -999,6 +1024,79 @@ int scf_iterate_impl(adscf_ctx* c, int niter, double tol, int* iters, double* ac
   if (iters) *iters = st.iter;
   if (acc) *acc = st.accuracy;
   if (conv) *conv = st.done;
+  return ADSCF_OK;
+}
+
+// Get_Conjugate_Fields (main_multiblock.f90:566-640; main_blend.f90:376-428) on the
+// device-resident loop: the current field is iterated until the densities it produces
+// match the target (resid_kind 1), from iteration count 1.
+template <typename T>
+int conjugate_impl(adscf_ctx* c, double thresh, int maxit, int measure, int* iters, double* acc) {
+  c->resid_kind = 1;
+  c->measure = measure;
+  hipLaunchKernelGGL(k_scf_restart, dim3(1), dim3(64), 0, c->stream, c->st.p);
+  HIPCK(hipGetLastError());
+  int conv = 0;
+  scf_iterate_impl<T>(c, maxit, thresh, iters, acc, &conv);
+  // the reference's loop index at exit: the converging iteration, or maxit + 1
+  if (iters && !conv) *iters = maxit + 1;
+  return ADSCF_OK;
+}
+
+// main_multiblock.f90:619-634
+int gcfstat_of(double acc, double accuracy_scf) {
+  if (!(acc < accuracy_scf * 0.5)) {
+    if (!(acc < accuracy_scf)) return !(acc < 0.1) ? 3 : 2;
+    return 1;
+  }
+  return 0;
+}
+
+// DDFT_Timestep_Euler (main_multiblock.f90:794-880) on device-resident state: the density
+// is the target buffer, the field the SCF ring.
+template <typename T>
+int ddft_step_impl(adscf_ctx* c, double dt, int qdmin, double accuracy_scf, int itermax, int* gcf, int* its) {
+  const long ng = c->ng, nvar = ng * c->nt;
+  const long nk = (long)(c->nx / 2 + 1) * c->ny * c->nz;
+  double acc = 0.0;
+  int it = 0;
+  conjugate_impl<T>(c, accuracy_scf * 0.5, itermax, 0, &it, &acc);  // :823
+  int st = gcfstat_of(acc, accuracy_scf);
+  if (its) its[0] = it;
+  if (gcf) gcf[0] = st;
+  if (st == 3) {
+    set_err(c, "Get_Conjugate_Fields: accuracy %.6e after %d iterations (the reference stops)", acc, it);
+    return ADSCF_ERR_STATE;
+  }
+  c->dmu.ensure(nvar);
+  c->dmuq.ensure(2 * nk * c->nt);
+  c->ddq.ensure(2 * nk * c->nt);
+  c->dd.ensure(nvar);
+  FHDesc fh = c->fh;
+  fh.nt = c->nt;
+  hipLaunchKernelGGL(k_ddft_mulocal, dim3(grid_for(ng)), dim3(kThreads), 0, c->stream, (const double*)c->target.p,
+                     (const double*)c->wring.p, (const ScfState*)c->st.p, fh, ng, c->dmu.p);  // :824-825
+  for (int t = 0; t < c->nt; ++t) enqueue_r2c<T>(c, c->dmu.p + t * ng, c->dmuq.p + 2 * t * nk);  // :834-839
+  hipLaunchKernelGGL(k_ddft_mobility, dim3(grid_for(nk)), dim3(kThreads), 0, c->stream, (const double*)c->dmuq.p,
+                     (const double*)c->mob.p, nk, c->nt, c->ddq.p);  // :843-850
+  for (int t = 0; t < c->nt; ++t) enqueue_c2r<T>(c, c->ddq.p + 2 * t * nk, c->dd.p + t * ng);  // :854-858
+  hipLaunchKernelGGL(k_ddft_euler, dim3(grid_for(nvar)), dim3(kThreads), 0, c->stream, c->target.p,
+                     (const double*)c->dd.p, nvar, dt, 1.0e-4, qdmin ? 1 : 0);  // :865-871
+  HIPCK(hipGetLastError());
+  conjugate_impl<T>(c, accuracy_scf * 0.5, itermax, 0, &it, &acc);  // :873
+  st = gcfstat_of(acc, accuracy_scf);
+  if (its) its[1] = it;
+  if (gcf) gcf[1] = st;
+  if (st == 3) {
+    set_err(c, "Get_Conjugate_Fields: accuracy %.6e after %d iterations (the reference stops)", acc, it);
+    return ADSCF_ERR_STATE;
+  }
+  if (qdmin && st >= 1) {  // :874-876: the densities of the field replace the density
+    enqueue_evaluate<T>(c);
+    HIPCK(hipMemcpyAsync(c->target.p, c->phi.p, nvar * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    c->phi_fresh = true;
+  }
+  HIPCK(hipStreamSynchronize(c->stream));
   return ADSCF_OK;
 }
 
@@ -1494,16 +1592,17 @@ int adjoint_impl(adscf_ctx* c, const ChainDesc& cd, double rho, const double* fi
 template <typename T>
 struct BatchTables {
   DevBuf<PassArgs<T>> args;
-  DevBuf<int> blk, shape;
-  std::vector<size_t> arg_off, blk_off;
-  std::vector<int> count, blocks;
-  size_t smem = 0;
+  DevBuf<int> blk;
+  struct Launch {
+    size_t arg_off, blk_off;
+    int blocks, N1, N2;
+    size_t smem;
+  };
+  std::vector<Launch> launches;  // pass-major, one per shape present in the pass
 };
 
 // instance slots are packed above kBatchItemBits in a signed 32-bit block code
 constexpr int kMaxBatchInstances = 1 << (31 - kBatchItemBits);
-
-int batch_shape_code(const AxisTables& ax) { return ax.N1 * 100 + ax.N2; }
 
 bool batch_shape_ok(int N1, int N2) {
   for (const auto& s : kBatchShapes)
@@ -1558,65 +1657,80 @@ BatchTables<float>& btab<float>(adscf_batch* b) { return b->t32; }
 template <typename T>
 void batch_build_tables(adscf_batch* b) {
   BatchTables<T>& tb = btab<T>(b);
-  std::vector<std::vector<PassArgs<T>>> per;
-  std::vector<std::vector<int>> items, shapes;
+  struct Entry {
+    PassArgs<T> args;
+    int items, N1, N2;
+    size_t smem;
+  };
+  std::vector<std::vector<Entry>> per;
   size_t npass = 0;
-  tb.smem = 0;
   for (adscf_ctx* c : b->inst) {
     ensure_tables<T>(c);
     Engine<T>& En = eng<T>(c);
     auto ps = build_sweep<T>(c, c->cd, En.E.p, En.h1.p, En.h2.p, c->d1.p, c->d2.p, &c->st.p->done);
-    std::vector<int> it, sh;
+    std::vector<Entry> es;
     for (auto& p : ps) {
       const AxisTables& ax = c->ax[p.axis];
-      size_t smem;
-      it.push_back(prep_reg<T>(c, p, ax.N1, ax.N2, 1, nullptr, &smem));
-      sh.push_back(batch_shape_code(ax));
-      tb.smem = std::max(tb.smem, smem);
+      Entry e{p, 0, ax.N1, ax.N2, 0};
+      e.items = prep_reg<T>(c, e.args, ax.N1, ax.N2, 1, nullptr, &e.smem);
+      if (e.items >= (1 << kBatchItemBits)) throw ArgError{"batched pass has too many blocks per instance"};
+      es.push_back(e);
     }
-    npass = std::max(npass, ps.size());
-    per.push_back(std::move(ps));
-    items.push_back(std::move(it));
-    shapes.push_back(std::move(sh));
+    npass = std::max(npass, es.size());
+    per.push_back(std::move(es));
   }
   std::vector<PassArgs<T>> args;
-  std::vector<int> blk, shape;
-  tb.arg_off.clear();
-  tb.blk_off.clear();
-  tb.count.clear();
-  tb.blocks.clear();
+  std::vector<int> blk;
+  tb.launches.clear();
   for (size_t p = 0; p < npass; ++p) {
-    tb.arg_off.push_back(args.size());
-    tb.blk_off.push_back(blk.size());
-    int blocks = 0, cnt = 0;
+    std::vector<std::pair<int, int>> shapes;  // distinct shapes of this pass, first-seen order
     for (size_t i = 0; i < per.size(); ++i) {
       if (p >= per[i].size()) continue;
-      if (items[i][p] >= (1 << kBatchItemBits)) throw ArgError{"batched pass has too many blocks per instance"};
-      args.push_back(per[i][p]);
-      shape.push_back(shapes[i][p]);
-      for (int it = 0; it < items[i][p]; ++it) blk.push_back((cnt << kBatchItemBits) | it);
-      blocks += items[i][p];
-      ++cnt;
+      const std::pair<int, int> sh{per[i][p].N1, per[i][p].N2};
+      if (std::find(shapes.begin(), shapes.end(), sh) == shapes.end()) shapes.push_back(sh);
     }
-    tb.count.push_back(cnt);
-    tb.blocks.push_back(blocks);
+    for (const auto& sh : shapes) {
+      typename BatchTables<T>::Launch L{args.size(), blk.size(), 0, sh.first, sh.second, 0};
+      int slot = 0;
+      for (size_t i = 0; i < per.size(); ++i) {
+        if (p >= per[i].size() || per[i][p].N1 != sh.first || per[i][p].N2 != sh.second) continue;
+        const Entry& e = per[i][p];
+        args.push_back(e.args);
+        for (int it = 0; it < e.items; ++it) blk.push_back((slot << kBatchItemBits) | it);
+        L.blocks += e.items;
+        L.smem = std::max(L.smem, e.smem);
+        ++slot;
+      }
+      tb.launches.push_back(L);
+    }
   }
   tb.args.ensure(args.size());
   tb.blk.ensure(blk.size());
-  tb.shape.ensure(shape.size());
   HIPCK(hipMemcpy(tb.args.p, args.data(), args.size() * sizeof(PassArgs<T>), hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(tb.blk.p, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice));
-  HIPCK(hipMemcpy(tb.shape.p, shape.data(), shape.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+template <typename T>
+void launch_batch_group(hipStream_t s, const typename BatchTables<T>::Launch& L, BatchGroup<T> g) {
+  const dim3 grid(L.blocks), block(64);
+#define ADSCF_BATCH_SHAPE(a, c)                                                             \
+  if (L.N1 == a && L.N2 == c) {                                                             \
+    hipLaunchKernelGGL((k_line_pass_batch<T, a, c>), grid, block, L.smem, s, g);             \
+    return;                                                                                 \
+  }
+  ADSCF_BATCH_SHAPE(4, 8) ADSCF_BATCH_SHAPE(8, 4) ADSCF_BATCH_SHAPE(2, 16) ADSCF_BATCH_SHAPE(8, 6)
+  ADSCF_BATCH_SHAPE(6, 8) ADSCF_BATCH_SHAPE(16, 3) ADSCF_BATCH_SHAPE(8, 8) ADSCF_BATCH_SHAPE(4, 16)
+  ADSCF_BATCH_SHAPE(16, 4) ADSCF_BATCH_SHAPE(8, 16) ADSCF_BATCH_SHAPE(16, 8) ADSCF_BATCH_SHAPE(1, 16)
+  ADSCF_BATCH_SHAPE(1, 8) ADSCF_BATCH_SHAPE(4, 0)
+#undef ADSCF_BATCH_SHAPE
+  throw ArgError{"no batched kernel for this shape"};
 }
 
 template <typename T>
 void batch_enqueue_sweep(adscf_batch* b) {
   BatchTables<T>& tb = btab<T>(b);
-  for (size_t p = 0; p < tb.count.size(); ++p) {
-    BatchPass<T> bp{tb.args.p + tb.arg_off[p], tb.blk.p + tb.blk_off[p], tb.shape.p + tb.arg_off[p],
-                    tb.count[p]};
-    hipLaunchKernelGGL(k_line_pass_batch<T>, dim3(tb.blocks[p]), dim3(64), tb.smem, b->stream, bp);
-  }
+  for (const auto& L : tb.launches)
+    launch_batch_group<T>(b->stream, L, BatchGroup<T>{tb.args.p + L.arg_off, tb.blk.p + L.blk_off});
   HIPCK(hipGetLastError());
 }
 
@@ -2244,6 +2358,8 @@ int adscf_scf_iterate(adscf_ctx* c, int niter, double tol, int* iters, double* a
       return ADSCF_ERR_STATE;
     }
     if (niter < 0) throw ArgError{"niter must be >= 0"};
+    c->resid_kind = c->user_kind;
+    c->measure = c->user_measure;
     return c->prec == ADSCF_FP64 ? scf_iterate_impl<double>(c, niter, tol, iters, accuracy, converged)
                                  : scf_iterate_impl<float>(c, niter, tol, iters, accuracy, converged);
   });
@@ -2316,6 +2432,141 @@ int adscf_scf_energy(adscf_ctx* c, double* F, double* Fc, double* qsum, double* 
     if (qsum) *qsum = st.qsum;
     if (mu) *mu = st.mu;
     if (rho) *rho = rho_v;
+    return ADSCF_OK;
+  });
+}
+
+int adscf_scf_set_residual(adscf_ctx* c, int kind, int measure, const double* target) {
+  return guarded(c, [&]() -> int {
+    if (kind != 0 && kind != 1) throw ArgError{"residual kind must be 0 (mean field) or 1 (density)"};
+    if (measure != 0 && measure != 1) throw ArgError{"measure must be 0 (max) or 1 (sum * dvol)"};
+    const long nvar = c->ng * c->nt;
+    if (target) {
+      c->target.ensure(nvar);
+      HIPCK(hipMemcpy(c->target.p, target, nvar * sizeof(double), hipMemcpyHostToDevice));
+      c->target_set = true;
+    }
+    if (kind == 1 && !c->target_set) throw ArgError{"a density residual needs target densities"};
+    c->user_kind = kind;
+    c->user_measure = measure;
+    return ADSCF_OK;
+  });
+}
+
+int adscf_conjugate_fields(adscf_ctx* c, const double* density, double thresh, int maxit, int measure, int* iterations,
+                           double* accuracy) {
+  return guarded(c, [&]() -> int {
+    if (!c->field_set) {
+      c->err = "adscf_scf_set_field has not been called";
+      return ADSCF_ERR_STATE;
+    }
+    if (maxit < 0) throw ArgError{"maxit must be >= 0"};
+    if (measure != 0 && measure != 1) throw ArgError{"measure must be 0 (max) or 1 (sum * dvol)"};
+    const long nvar = c->ng * c->nt;
+    if (density) {
+      c->target.ensure(nvar);
+      HIPCK(hipMemcpy(c->target.p, density, nvar * sizeof(double), hipMemcpyHostToDevice));
+      c->target_set = true;
+    }
+    if (!c->target_set) throw ArgError{"no target densities"};
+    return c->prec == ADSCF_FP64 ? conjugate_impl<double>(c, thresh, maxit, measure, iterations, accuracy)
+                                 : conjugate_impl<float>(c, thresh, maxit, measure, iterations, accuracy);
+  });
+}
+
+int adscf_mobility_qq(adscf_ctx* c, int nq, double dq, const double* table, double wcutoff, double* mobility_qq) {
+  // Read_Mobilities, table part (main_multiblock.f90:486-504), on the host at set-up
+  return guarded(c, [&]() -> int {
+    if (!table || nq < 1 || !(dq > 0.0)) throw ArgError{"bad mobility table"};
+    const int nt = c->nt;
+    const long nk = (long)(c->nx / 2 + 1) * c->ny * c->nz;
+    std::vector<double> lap(2 * nk), mob((size_t)nk * nt * nt);
+    const int rc = adscf_laplace(c, lap.data(), nullptr, nullptr);
+    if (rc != ADSCF_OK) return rc;
+    for (long i = 0; i < nk; ++i) {
+      const double qq = -lap[2 * i];
+      const double q = sqrt(qq);
+      const int iq = (int)(q / dq);
+      for (int t2 = 0; t2 < nt; ++t2)
+        for (int t1 = 0; t1 < nt; ++t1) {
+          const double* col = table + (long)(t1 + t2 * nt) * nq;  // mobility_table(:, t1, t2)
+          double mq;
+          if (iq == 0) mq = col[0] * q / dq;
+          else if (iq >= nq) mq = col[nq - 1];
+          else mq = (col[iq - 1] * (q - (double)iq * dq) + col[iq] * ((double)(iq + 1) * dq - q)) / dq;
+          const double v = qq * mq;
+          mob[(size_t)(t1 + t2 * nt) * nk + i] = v < wcutoff ? v : wcutoff;
+        }
+    }
+    if (mobility_qq) std::copy(mob.begin(), mob.end(), mobility_qq);
+    return adscf_ddft_set_mobility(c, mob.data());
+  });
+}
+
+int adscf_ddft_set_mobility(adscf_ctx* c, const double* mobility_qq) {
+  return guarded(c, [&]() -> int {
+    if (!mobility_qq) throw ArgError{"null array"};
+    const long n = (long)(c->nx / 2 + 1) * c->ny * c->nz * c->nt * c->nt;
+    c->mob.ensure(n);
+    HIPCK(hipMemcpy(c->mob.p, mobility_qq, n * sizeof(double), hipMemcpyHostToDevice));
+    c->mob_set = true;
+    return ADSCF_OK;
+  });
+}
+
+int adscf_ddft_set_density(adscf_ctx* c, const double* density) {
+  return guarded(c, [&]() -> int {
+    if (!density) throw ArgError{"null array"};
+    const long nvar = c->ng * c->nt;
+    c->target.ensure(nvar);
+    HIPCK(hipMemcpy(c->target.p, density, nvar * sizeof(double), hipMemcpyHostToDevice));
+    c->target_set = true;
+    return ADSCF_OK;
+  });
+}
+
+int adscf_ddft_get_density(adscf_ctx* c, double* density) {
+  return guarded(c, [&]() -> int {
+    if (!density) throw ArgError{"null array"};
+    if (!c->target_set) {
+      c->err = "no DDFT density has been set";
+      return ADSCF_ERR_STATE;
+    }
+    HIPCK(hipStreamSynchronize(c->stream));
+    HIPCK(hipMemcpy(density, c->target.p, c->ng * c->nt * sizeof(double), hipMemcpyDeviceToHost));
+    return ADSCF_OK;
+  });
+}
+
+int adscf_ddft_step(adscf_ctx* c, double timestep, int qdmin, double accuracy_scf, int itermax_scf, int* gcfstat,
+                    int* iterations) {
+  return guarded(c, [&]() -> int {
+    if (!c->field_set || !c->target_set || !c->mob_set) {
+      c->err = "adscf_ddft_step needs adscf_scf_set_field, adscf_ddft_set_density and a mobility table";
+      return ADSCF_ERR_STATE;
+    }
+    if (itermax_scf < 1) throw ArgError{"itermax_scf must be >= 1"};
+    return c->prec == ADSCF_FP64 ? ddft_step_impl<double>(c, timestep, qdmin, accuracy_scf, itermax_scf, gcfstat, iterations)
+                                 : ddft_step_impl<float>(c, timestep, qdmin, accuracy_scf, itermax_scf, gcfstat, iterations);
+  });
+}
+
+int adscf_ddft_energy(adscf_ctx* c, double* F, double* mu, double* rho) {
+  // Calculate_Energy (main_multiblock.f90:548-557) with the DDFT density: the chain energy of
+  // the last density evaluation - dvol sum(density*field) + Interaction_Energy(density)
+  return guarded(c, [&]() -> int {
+    if (!c->field_set || !c->target_set) {
+      c->err = "no DDFT state";
+      return ADSCF_ERR_STATE;
+    }
+    ScfState st{};
+    HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    std::vector<double> p;
+    host_energy_parts(c, c->target.p, c->wring.p + (size_t)st.head * c->ng * c->nt, p);
+    if (F) *F = st.energy - c->dvol * p[0] + interaction_from_parts(c, p);
+    if (mu) *mu = st.mu;
+    if (rho) *rho = st.rho;
     return ADSCF_OK;
   });
 }

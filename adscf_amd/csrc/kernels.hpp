@@ -498,45 +498,28 @@ __global__ void __launch_bounds__(kThreads) k_line_pass_reg(PassArgs<T> a) {
   reg_pass_body<T, PassEngine<T, N1, N2>, MODE>(a, xcd_tile(blockIdx.x, gridDim.x), smem_raw);
 }
 
-// One pass of many independent instances in one launch: blk[b] packs the
-// instance slot of block b (bits 20 and up) and the block's item within that
-// instance (low 20 bits); every instance carries its own arguments (grid,
-// buffers, step types) and four-step shape.  Blocks are one wavefront for
-// every shape.
+// One pass of many independent instances that share a register shape, in one
+// launch: blk[b] packs the instance slot of block b (bits 20 and up) and the
+// block's item within that instance (low 20 bits); every instance carries its own
+// arguments (grid, buffers, step types).  Blocks are one wavefront.  A pass of a
+// batch is one such launch per shape present (the shape is a template argument,
+// so each launch has the register budget of its own engine).
 template <typename T>
-struct BatchPass {
+struct BatchGroup {
   const PassArgs<T>* args;
-  const int* blk;    // one entry per block
-  const int* shape;  // n codes N1 * 100 + N2
-  int n;
+  const int* blk;  // one entry per block
 };
 
 constexpr int kBatchItemBits = 20;
 
-template <typename T>
-__global__ void __launch_bounds__(64) k_line_pass_batch(BatchPass<T> tb) {
-  const int code = tb.blk[blockIdx.x];
+template <typename T, int N1, int N2>
+__global__ void __launch_bounds__(64) k_line_pass_batch(BatchGroup<T> g) {
+  const int code = g.blk[blockIdx.x];
   const int i = code >> kBatchItemBits;
   const int item = code & ((1 << kBatchItemBits) - 1);
-  const PassArgs<T> a = tb.args[i];
+  const PassArgs<T> a = g.args[i];
   extern __shared__ unsigned char smem_raw[];
-  switch (tb.shape[i]) {
-    case 408: reg_pass_body<T, EngFour<T, 4, 8>>(a, item, smem_raw); break;
-    case 804: reg_pass_body<T, EngFour<T, 8, 4>>(a, item, smem_raw); break;
-    case 216: reg_pass_body<T, EngFour<T, 2, 16>>(a, item, smem_raw); break;
-    case 806: reg_pass_body<T, EngFour<T, 8, 6>>(a, item, smem_raw); break;
-    case 608: reg_pass_body<T, EngFour<T, 6, 8>>(a, item, smem_raw); break;
-    case 1603: reg_pass_body<T, EngFour<T, 16, 3>>(a, item, smem_raw); break;
-    case 808: reg_pass_body<T, EngFour<T, 8, 8>>(a, item, smem_raw); break;
-    case 416: reg_pass_body<T, EngFour<T, 4, 16>>(a, item, smem_raw); break;
-    case 1604: reg_pass_body<T, EngFour<T, 16, 4>>(a, item, smem_raw); break;
-    case 816: reg_pass_body<T, EngFour<T, 8, 16>>(a, item, smem_raw); break;
-    case 1608: reg_pass_body<T, EngFour<T, 16, 8>>(a, item, smem_raw); break;
-    case 116: reg_pass_body<T, EngFour<T, 1, 16>>(a, item, smem_raw); break;
-    case 108: reg_pass_body<T, EngFour<T, 1, 8>>(a, item, smem_raw); break;
-    case 400: reg_pass_body<T, EngCube<T, 4>>(a, item, smem_raw); break;
-    default: break;
-  }
+  reg_pass_body<T, PassEngine<T, N1, N2>>(a, item, smem_raw);
 }
 
 constexpr int kBatchShapes[][2] = {{4, 8}, {8, 4}, {2, 16}, {8, 6}, {6, 8}, {16, 3}, {8, 8},

@@ -226,7 +226,9 @@ struct ContourArgs {
   double* phi;        // CONTOUR_SCF: species densities (nt, ng)
   double* dring;      // CONTOUR_SCF: residual history ring (m+1, nt*ng) or null
   int m;
-  double* rpart;      // per-block max|dfield|
+  const double* target;  // CONTOUR_SCF: null -> dfield = mean field - field; else dfield = phi - target
+  int measure;        // 0: per-block max|dfield|; 1: per-block sum|dfield|
+  double* rpart;      // per-block max (measure 0) or sum (measure 1) of |dfield|
   double* spart;      // per-block sum of densities (grand canonical rho)
   const int* done;
 };
@@ -238,9 +240,11 @@ struct alignas(sizeof(T) * VP) PackT {
 
 // Trapezoid contour integral (mod_multiblock.f90:157-168) in the reference's
 // summation order, then the ensemble normalisation (173-180).  In SCF mode the
-// species densities (main_multiblock.f90:530-534), the mean fields
-// (mod_FloryHuggins.f90:104-129), dfield = new - field and max|dfield|
-// (main_multiblock.f90:674-676, 780) follow in the same pass.
+// species densities (main_multiblock.f90:530-534) and the residual follow in the
+// same pass: either the mean fields (mod_FloryHuggins.f90:104-129) and
+// dfield = new - field (main_multiblock.f90:674-676), or, for the conjugate-field
+// solve, dfield = newdensity - density (main_multiblock.f90:602-603); then
+// max|dfield| (main_multiblock.f90:780) or sum|dfield| (main_blend.f90:567).
 // Each thread owns VP consecutive points (16-byte loads; ng % VP == 0) and
 // loads kUnroll contour slices ahead of the sum, which keeps the stream of
 // slice reads deep enough to run at HBM rate.
@@ -323,11 +327,17 @@ __global__ void __launch_bounds__(kThreads) k_contour(ContourArgs<T> a) {
 #pragma unroll
       for (int v = 0; v < VP; ++v) {
         const long i = i0 + v;
-        double vf[kMaxTypes], tot = 0.0;
-        for (int t = 0; t < nt; ++t) {
-          a.phi[(long)t * ng + i] = phi[t][v];
-          vf[t] = phi[t][v] * a.fh.vol[t];
+        for (int t = 0; t < nt; ++t) a.phi[(long)t * ng + i] = phi[t][v];
+        if (a.target) {
+          for (int t = 0; t < nt; ++t) {
+            const double df = phi[t][v] - a.target[(long)t * ng + i];
+            if (d0) d0[(long)t * ng + i] = df;
+            rmax = a.measure ? rmax + fabs(df) : nanmax(rmax, fabs(df));
+          }
+          continue;
         }
+        double vf[kMaxTypes], tot = 0.0;
+        for (int t = 0; t < nt; ++t) vf[t] = phi[t][v] * a.fh.vol[t];
         for (int t = 0; t < nt; ++t) tot += vf[t];
         for (int t = 0; t < nt; ++t) {
           double f = a.fh.kappa * (tot - 1.0);
@@ -335,12 +345,12 @@ __global__ void __launch_bounds__(kThreads) k_contour(ContourArgs<T> a) {
           f = f * a.fh.vol[t];
           const double df = f - w0[(long)t * ng + i];
           if (d0) d0[(long)t * ng + i] = df;
-          rmax = nanmax(rmax, fabs(df));
+          rmax = a.measure ? rmax + fabs(df) : nanmax(rmax, fabs(df));
         }
       }
     }
   }
-  rmax = block_max(rmax, sh);
+  rmax = a.measure ? block_sum(rmax, sh) : block_max(rmax, sh);
   dsum = block_sum(dsum, sh);
   if (threadIdx.x == 0) {
     if (a.rpart) a.rpart[blockIdx.x] = rmax;
@@ -609,6 +619,7 @@ struct SolveArgs {
   int ns, ensemble;
   int force_anderson;  // compat: caller decides via iter
   int iter_override;   // compat: iteration number passed by the caller
+  int measure;         // 0: accuracy = max|dfield|; 1: accuracy = sum|dfield| * dvol
   ScfState* st;
 };
 
@@ -629,8 +640,14 @@ __global__ void k_solve(SolveArgs a) {
     it = a.iter_override;
   } else {
     double r = 0.0;
-    for (int b = tid; b < a.nblk_contour; b += blockDim.x) r = nanmax(r, a.rpart[b]);
-    r = block_max(r, sh);
+    if (a.measure) {
+      // block partials summed in a fixed order, then times dvol (main_blend.f90:567)
+      for (int b = tid; b < a.nblk_contour; b += blockDim.x) r += a.rpart[b];
+      r = block_sum(r, sh) * a.dvol;
+    } else {
+      for (int b = tid; b < a.nblk_contour; b += blockDim.x) r = nanmax(r, a.rpart[b]);
+      r = block_max(r, sh);
+    }
     double s = 0.0;
     for (int b = tid; b < a.nblk_contour; b += blockDim.x) s += a.spart[b];
     s = block_sum(s, sh);
@@ -788,6 +805,71 @@ __global__ void __launch_bounds__(kThreads) k_widen(const T* __restrict__ a, dou
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_fill(T* __restrict__ a, T v, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) a[i] = v;
+}
+
+// A new solve on the current field: the iteration count and the stop flag start
+// again (Get_Conjugate_Fields restarts iter_status at 1, main_multiblock.f90:594).
+// The ring keeps its entries; Anderson reads history only from iteration m+1 on,
+// when every entry it reads belongs to the new solve.
+__global__ void k_scf_restart(ScfState* st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->iter = 0;
+    st->done = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// DDFT_Timestep_Euler (Main_Multiblock/main_multiblock.f90:794-880)
+// ---------------------------------------------------------------------------
+
+// mulocal = mean fields of the density - field (:824-825); the field is ring slot `head`
+__global__ void __launch_bounds__(kThreads) k_ddft_mulocal(const double* __restrict__ den,
+                                                           const double* __restrict__ wring, const ScfState* st,
+                                                           FHDesc fh, long ng, double* __restrict__ mu) {
+  const int nt = fh.nt;
+  const double* w = wring + (long)st->head * nt * ng;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ng; i += (long)gridDim.x * blockDim.x) {
+    double vf[kMaxTypes], tot = 0.0;
+    for (int t = 0; t < nt; ++t) vf[t] = den[(long)t * ng + i] * fh.vol[t];
+    for (int t = 0; t < nt; ++t) tot += vf[t];
+    for (int t = 0; t < nt; ++t) {
+      double f = fh.kappa * (tot - 1.0);
+      for (int t2 = 0; t2 < nt; ++t2) f = f + fh.chi[t + t2 * nt] * vf[t2];
+      f = f * fh.vol[t];
+      mu[(long)t * ng + i] = f - w[(long)t * ng + i];
+    }
+  }
+}
+
+// ddensity_q(t1) = 0 - sum over t2 of mobility_qq(:,t2,t1) * mu_q(t2), accumulated in
+// type order (:843-850).  muq, ddq: nt half spectra of 2*nk doubles; mob: (nk, nt, nt).
+__global__ void __launch_bounds__(kThreads) k_ddft_mobility(const double* __restrict__ muq,
+                                                            const double* __restrict__ mob, long nk, int nt,
+                                                            double* __restrict__ ddq) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nk; i += (long)gridDim.x * blockDim.x) {
+    for (int t1 = 0; t1 < nt; ++t1) {
+      double re = 0.0, im = 0.0;
+      for (int t2 = 0; t2 < nt; ++t2) {
+        const double m = mob[(long)(t2 + t1 * nt) * nk + i];
+        re = re - m * muq[2 * ((long)t2 * nk + i)];
+        im = im - m * muq[2 * ((long)t2 * nk + i) + 1];
+      }
+      ddq[2 * ((long)t1 * nk + i)] = re;
+      ddq[2 * ((long)t1 * nk + i) + 1] = im;
+    }
+  }
+}
+
+// density = density + timestep*ddensity, then max(density, floor) (:866-871).  The
+// reference clamps only when minval(density) <= floor; for finite densities the
+// unconditional max gives the same values (nothing is below the floor otherwise).
+__global__ void __launch_bounds__(kThreads) k_ddft_euler(double* __restrict__ den, const double* __restrict__ dd, long n,
+                                                         double dt, double floor, int clamp) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double d = den[i] + dt * dd[i];
+    if (clamp) d = d > floor ? d : floor;
+    den[i] = d;
+  }
 }
 
 }  // namespace adscf

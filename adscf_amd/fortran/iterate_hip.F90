@@ -8,7 +8,12 @@ module Iterate
   use adscf_hip
   implicit none
 
+  ! mod_iterate.f90:13.  A driver whose Global declares mixing_dim itself
+  ! (Main_Multiblock/mod_global.f90:21) is built with -DADSCF_GLOBAL_MIXING_DIM,
+  ! which takes the history depth from Global instead of clashing with it.
+#ifndef ADSCF_GLOBAL_MIXING_DIM
   integer, parameter :: mixing_dim = 2
+#endif
 
 contains
 

@@ -157,6 +157,45 @@ int adscf_scf_residuals(adscf_ctx* ctx, double* out, int cap, int* n);
  * propagator sweep alone, in milliseconds (HIP events on the context stream) */
 int adscf_scf_timing(adscf_ctx* ctx, double* ms_per_iter, double* ms_sweep);
 
+/* Residual of the loop adscf_scf_iterate runs.  kind 0: dfield = mean fields - field
+ * (Find_SCF_Saddle_Point, main_multiblock.f90:673-676); kind 1: dfield = densities - target
+ * (the loop of Get_Conjugate_Fields, main_multiblock.f90:602-604), target (nx,ny,nz,ntypes)
+ * uploaded when non-NULL.  measure 0: accuracy = maxval(abs(dfield)) (main_multiblock.f90:780);
+ * 1: sum(abs(dfield))*dvol (Main_Blend/main_blend.f90:567).  Default: kind 0, measure 0. */
+int adscf_scf_set_residual(adscf_ctx* ctx, int kind, int measure, const double* target);
+
+/* ---- conjugate fields and DDFT (Main_Multiblock/main_multiblock.f90) ------------ */
+/* Get_Conjugate_Fields (main_multiblock.f90:566-640; main_blend.f90:376-428): iterate the
+ * current field (adscf_scf_set_field, or the field a previous call left) with the Anderson
+ * settings of adscf_scf_setup until the densities it produces match `density` (NULL: the
+ * device-resident DDFT density); exits when .not.(accuracy > thresh), where the reference
+ * uses thresh = accuracy_scf*0.5 (main_multiblock.f90:609) or accuracy_scf (main_blend.f90:412).
+ * iterations: the reference's loop index at exit (maxit+1 if the loop ran out). */
+int adscf_conjugate_fields(adscf_ctx* ctx, const double* density, double thresh, int maxit, int measure,
+                           int* iterations, double* accuracy);
+/* Read_Mobilities' table part (main_multiblock.f90:486-504): mobility_table(nq,ntypes,ntypes)
+ * column-major (row iq at q = iq*dq), interpolated at |q| of every half-spectrum point as the
+ * reference does, times |q|^2, capped at wcutoff.  The result (nx/2+1,ny,nz,ntypes,ntypes) is
+ * kept on the device for adscf_ddft_step and copied to mobility_qq when non-NULL. */
+int adscf_mobility_qq(adscf_ctx* ctx, int nq, double dq, const double* table, double wcutoff, double* mobility_qq);
+/* upload a mobility_qq array computed by the caller (the driver's own Read_Mobilities) */
+int adscf_ddft_set_mobility(adscf_ctx* ctx, const double* mobility_qq);
+/* the DDFT density (nx,ny,nz,ntypes), kept on the device between steps */
+int adscf_ddft_set_density(adscf_ctx* ctx, const double* density);
+int adscf_ddft_get_density(adscf_ctx* ctx, double* density);
+/* DDFT_Timestep_Euler (main_multiblock.f90:794-880) on the device-resident density and field:
+ * conjugate fields (accuracy_scf, itermax_scf; the driver passes 1e-3 and 2000), mulocal =
+ * mean fields - field, its spectrum times -mobility_qq summed over types, back to real space,
+ * density += timestep*ddensity, the 1e-4 floor when qdmin, conjugate fields again, and, when
+ * qdmin and that solve was slow or failed, the densities of the field.  gcfstat[2] and
+ * iterations[2] (may be NULL) receive both solves' status (0 ok, 1 slow, 2 failed) and loop
+ * index.  A solve that ends with accuracy >= 0.1 returns ADSCF_ERR_STATE (the reference stops). */
+int adscf_ddft_step(adscf_ctx* ctx, double timestep, int qdmin, double accuracy_scf, int itermax_scf, int* gcfstat,
+                    int* iterations);
+/* Calculate_Energy (main_multiblock.f90:548-557) with the DDFT density and field; mu and rho
+ * of the last density evaluation.  Any pointer may be NULL. */
+int adscf_ddft_energy(adscf_ctx* ctx, double* F, double* mu, double* rho);
+
 /* ---- schedule autotuner (no counterpart in the reference) --------------------- */
 /* Times the captured propagator sweep of the configured SCF problem for every line-FFT
  * schedule of each axis (register-resident four-step shapes N1 x N2 with 1, 2 or 4

@@ -99,6 +99,12 @@ def _load():
         "orc_total_energy": (C.c_double, [G, C.POINTER(_Chain), C.POINTER(_FH), C.c_int, _dp, _dp]),
         "orc_scf_anderson": (C.c_int, [G, C.POINTER(_Chain), C.POINTER(_FH), C.c_int, _dp, _dp,
                                        C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]),
+        "orc_mobility_qq": (None, [G, C.c_int, C.c_int, C.c_double, _dp, C.c_double, _dp]),
+        "orc_conjugate_fields": (C.c_int, [G, C.POINTER(_Chain), C.c_int, _dp, _dp, C.c_int, C.c_double,
+                                           C.c_double, C.c_int, C.c_int, _dp]),
+        "orc_gcfstat": (C.c_int, [C.c_double, C.c_double]),
+        "orc_ddft_step": (C.c_int, [G, C.POINTER(_Chain), C.POINTER(_FH), C.c_int, _dp, _dp, _dp, C.c_double,
+                                    C.c_int, C.c_double, C.c_int, C.c_int, _ip, _ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -390,3 +396,56 @@ def scf_anderson(g: Grid, block_types, block_segs, fh: FH, field, m, lam=0.5, to
     n = min(it, maxit)
     return dict(field=f, density=den, iters=it, accuracy=acc.value, residuals=res[:n],
                 Qsum=ch.Qsum, mu=ch.mu, rho=ch.rho, energy=ch.energy)
+
+
+# ---- DDFT (Main_Multiblock/main_multiblock.f90) ------------------------------
+
+def mobility_qq(g: Grid, table, dq, wcutoff):
+    """Read_Mobilities' table part (main_multiblock.f90:486-504).  table: (nt, nt, nq) in C
+    order, i.e. Fortran mobility_table(nq, nt, nt) with table[t2, t1, iq-1] =
+    mobility_table(iq, t1, t2).  Returns mobility_qq as (nt, nt, nz, ny, nx/2+1), the C view
+    of Fortran mobility_qq(nx/2+1, ny, nz, nt, nt)."""
+    t = np.ascontiguousarray(table, dtype=np.float64)
+    nt, nq = t.shape[0], t.shape[2]
+    out = np.empty((nt, nt) + g.kshape, dtype=np.float64)
+    lib().orc_mobility_qq(C.byref(g._g), nt, nq, float(dq), _p(t), float(wcutoff), _p(out))
+    return out
+
+
+def conjugate_fields(g: Grid, block_types, block_segs, density, field, m, lam=0.5, thresh=5e-4, maxit=2000,
+                     measure=0, ensemble=1, rho=1.0, mu=0.0):
+    """Get_Conjugate_Fields (main_multiblock.f90:566-640 for measure 0, main_blend.f90:376-428
+    for measure 1).  Returns dict(field, iters, accuracy)."""
+    bt = np.ascontiguousarray(block_types, dtype=np.int32)
+    bs = np.ascontiguousarray(block_segs, dtype=np.int32)
+    d = np.ascontiguousarray(density, dtype=np.float64)
+    f = np.array(field, dtype=np.float64, order="C")
+    acc = C.c_double()
+    ch = _Chain(len(bt), _pi(bt), _pi(bs), int(ensemble), float(rho), float(mu), 0.0, 0.0)
+    it = lib().orc_conjugate_fields(C.byref(g._g), C.byref(ch), f.shape[0], _p(d), _p(f), int(m), float(lam),
+                                    float(thresh), int(maxit), int(measure), C.byref(acc))
+    if it < 0:
+        raise ValueError("Invalid ensemble")
+    return dict(field=f, iters=it, accuracy=acc.value)
+
+
+def gcfstat(accuracy, accuracy_scf):
+    return lib().orc_gcfstat(float(accuracy), float(accuracy_scf))
+
+
+def ddft_step(g: Grid, block_types, block_segs, fh: FH, density, field, mob_qq, timestep, qdmin=True,
+              accuracy_scf=1e-3, itermax_scf=2000, m=10, ensemble=1, rho=1.0, mu=0.0):
+    """DDFT_Timestep_Euler (main_multiblock.f90:794-880).  density and field are updated in
+    place (C-contiguous float64).  Returns dict(rc, gcfstat, iters, mu, rho, energy) with the
+    chain scalars of the last density evaluation."""
+    bt = np.ascontiguousarray(block_types, dtype=np.int32)
+    bs = np.ascontiguousarray(block_segs, dtype=np.int32)
+    mq = np.ascontiguousarray(mob_qq, dtype=np.float64)
+    st = np.zeros(2, dtype=np.int32)
+    its = np.zeros(2, dtype=np.int32)
+    ch = _Chain(len(bt), _pi(bt), _pi(bs), int(ensemble), float(rho), float(mu), 0.0, 0.0)
+    rc = lib().orc_ddft_step(C.byref(g._g), C.byref(ch), C.byref(fh._fh), density.shape[0], _p(density),
+                             _p(field), _p(mq), float(timestep), int(bool(qdmin)), float(accuracy_scf),
+                             int(itermax_scf), int(m), _pi(st), _pi(its))
+    return dict(rc=rc, gcfstat=st.tolist(), iters=its.tolist(), mu=ch.mu, rho=ch.rho, energy=ch.energy,
+                Qsum=ch.Qsum)

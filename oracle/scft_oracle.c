@@ -776,3 +776,144 @@ int orc_scf_anderson(const orc_grid* g, orc_chain* c, const orc_fh* fh, int ntyp
   free(var);
   return iter;
 }
+
+/* ======================================================================
+ * DDFT: Main_Multiblock/main_multiblock.f90
+ * ====================================================================== */
+
+/* main_multiblock.f90:486-504 (the table part of Read_Mobilities, 441-506) */
+void orc_mobility_qq(const orc_grid* g, int nt, int nq, double dq, const double* table, double wcutoff,
+                     double* mobility_qq) {
+  int nh = g->nx / 2 + 1;
+  long nk = (long)nh * g->ny * g->nz;
+  double* lap = (double*)malloc(sizeof(double) * 2 * (size_t)nk);
+  orc_calculate_laplace(g, lap, NULL, NULL);
+  for (long i = 0; i < nk; ++i) {
+    double qq = -lap[2 * i];  /* real(-laplace(x,y,z)) (:489) */
+    double q = sqrt(qq);      /* :490 */
+    int iq = (int)(q / dq);   /* int(): truncation (:491) */
+    for (int t2 = 0; t2 < nt; ++t2)
+      for (int t1 = 0; t1 < nt; ++t1) {
+        const double* col = table + (long)(t1 + t2 * nt) * nq; /* mobility_table(:, t1, t2) */
+        double mq;
+        if (iq == 0)
+          mq = col[0] * q / dq;                                 /* :493 */
+        else if (iq >= nq)
+          mq = col[nq - 1];                                     /* :495 */
+        else                                                    /* :497-499, the reference's weights */
+          mq = (col[iq - 1] * (q - (double)iq * dq) + col[iq] * ((double)(iq + 1) * dq - q)) / dq;
+        double v = qq * mq;                                     /* :501 */
+        mobility_qq[(long)(t1 + t2 * nt) * nk + i] = v < wcutoff ? v : wcutoff;
+      }
+  }
+  free(lap);
+}
+
+/* main_multiblock.f90:566-640 (measure 0) and main_blend.f90:376-428 (measure 1) */
+int orc_conjugate_fields(const orc_grid* g, orc_chain* c, int ntypes, const double* density, double* field, int m,
+                         double lam, double thresh, int maxit, int measure, double* accuracy) {
+  long ng = (long)g->nx * g->ny * g->nz;
+  long nvar = ng * ntypes;
+  double volume, dvol;
+  orc_volume_parameters(g, &volume, &dvol);
+  double* var = (double*)calloc((size_t)nvar * (size_t)(m + 1), sizeof(double));
+  double* dvar = (double*)calloc((size_t)nvar * (size_t)(m + 1), sizeof(double));
+  double* newdensity = (double*)malloc(sizeof(double) * (size_t)nvar);
+  double acc = 0.0;
+  int iter;
+  for (iter = 1; iter <= maxit; ++iter) {
+    if (orc_species_densities(g, c, ntypes, field, newdensity) != 0) { iter = -1; break; } /* :602 */
+    acc = 0.0;
+    for (long i = 0; i < nvar; ++i) {                                                       /* :603-604, 780 / main_blend.f90:567 */
+      double df = newdensity[i] - density[i];
+      dvar[i] = df;
+      double a = fabs(df);
+      if (measure == 0) {
+        if (a > acc || isnan(a)) acc = isnan(acc) ? acc : a;
+      } else {
+        acc += a;
+      }
+    }
+    if (measure != 0) acc = acc * dvol;
+    if (!(acc > thresh)) break;                                                             /* :609 */
+    int iter_status = iter < m + 1 ? iter : m + 1;                                          /* :614 */
+    memcpy(var, field, sizeof(double) * (size_t)nvar);
+    orc_mixing_anderson(var, dvar, nvar, m, lam, iter_status);                              /* :615 */
+    memcpy(field, var, sizeof(double) * (size_t)nvar);
+  }
+  *accuracy = acc;
+  free(newdensity);
+  free(dvar);
+  free(var);
+  return iter;
+}
+
+/* main_multiblock.f90:619-634 */
+int orc_gcfstat(double accuracy, double accuracy_scf) {
+  if (!(accuracy < accuracy_scf * 0.5)) {
+    if (!(accuracy < accuracy_scf)) return (!(accuracy < 0.1)) ? 3 : 2;
+    return 1;
+  }
+  return 0;
+}
+
+/* main_multiblock.f90:794-880 */
+int orc_ddft_step(const orc_grid* g, orc_chain* c, const orc_fh* fh, int ntypes, double* density, double* field,
+                  const double* mobility_qq, double timestep, int qdmin, double accuracy_scf, int itermax_scf, int m,
+                  int* gcfstat, int* iterations) {
+  long ng = (long)g->nx * g->ny * g->nz;
+  long nvar = ng * ntypes;
+  int nh = g->nx / 2 + 1;
+  long nk = (long)nh * g->ny * g->nz;
+  double acc;
+  int it, st;
+  it = orc_conjugate_fields(g, c, ntypes, density, field, m, 0.5, accuracy_scf * 0.5, itermax_scf, 0, &acc); /* :823 */
+  if (it < 0) return -1;
+  st = orc_gcfstat(acc, accuracy_scf);
+  if (iterations) iterations[0] = it;
+  if (gcfstat) gcfstat[0] = st;
+  if (st == 3) return 3;
+  double* newfield = (double*)malloc(sizeof(double) * (size_t)nvar);
+  double* mulocal = (double*)malloc(sizeof(double) * (size_t)nvar);
+  double* muq = (double*)malloc(sizeof(double) * 2 * (size_t)nk * (size_t)ntypes);
+  double* ddq = (double*)malloc(sizeof(double) * 2 * (size_t)nk);
+  double* dd = (double*)malloc(sizeof(double) * (size_t)nvar);
+  orc_mean_fields(g, fh, density, newfield);                                 /* :824 */
+  for (long i = 0; i < nvar; ++i) mulocal[i] = newfield[i] - field[i];       /* :825 */
+  for (int t = 0; t < ntypes; ++t)                                           /* :834-839 */
+    orc_real2fourier(g, mulocal + (long)t * ng, muq + 2 * (long)t * nk);
+  for (int t1 = 0; t1 < ntypes; ++t1) {                                      /* :843-850 */
+    for (long i = 0; i < 2 * nk; ++i) ddq[i] = 0.0;
+    for (int t2 = 0; t2 < ntypes; ++t2) {
+      const double* mob = mobility_qq + (long)(t2 + t1 * ntypes) * nk;       /* mobility_qq(:,:,:,type2,type1) */
+      const double* mu = muq + 2 * (long)t2 * nk;
+      for (long i = 0; i < nk; ++i) {
+        ddq[2 * i] = ddq[2 * i] - mob[i] * mu[2 * i];
+        ddq[2 * i + 1] = ddq[2 * i + 1] - mob[i] * mu[2 * i + 1];
+      }
+    }
+    orc_fourier2real(g, ddq, dd + (long)t1 * ng);                            /* :854-858 */
+  }
+  for (long i = 0; i < nvar; ++i) density[i] = density[i] + timestep * dd[i]; /* :866 */
+  if (qdmin) {                                                                /* :865-871 */
+    const double dsmall = 1.0e-4;
+    double dum = density[0];
+    for (long i = 1; i < nvar; ++i) dum = density[i] < dum ? density[i] : dum;
+    if (dum <= dsmall)
+      for (long i = 0; i < nvar; ++i) density[i] = density[i] > dsmall ? density[i] : dsmall;
+  }
+  free(dd);
+  free(ddq);
+  free(muq);
+  free(mulocal);
+  free(newfield);
+  it = orc_conjugate_fields(g, c, ntypes, density, field, m, 0.5, accuracy_scf * 0.5, itermax_scf, 0, &acc); /* :873 */
+  if (it < 0) return -1;
+  st = orc_gcfstat(acc, accuracy_scf);
+  if (iterations) iterations[1] = it;
+  if (gcfstat) gcfstat[1] = st;
+  if (st == 3) return 3;
+  if (qdmin && st >= 1)                                                       /* :874-876 */
+    if (orc_species_densities(g, c, ntypes, field, density) != 0) return -1;
+  return 0;
+}

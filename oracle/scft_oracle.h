@@ -121,6 +121,33 @@ int orc_scf_anderson(const orc_grid* g, orc_chain* c, const orc_fh* fh, int ntyp
                      double* density, int m, double lam, double tol, int maxit, double* accuracy,
                      double* residual_history);
 
+/* ---- DDFT (Main_Multiblock/main_multiblock.f90) ------------------------ */
+/* Read_Mobilities, table part (main_multiblock.f90:486-504): for every half-spectrum
+ * point, |q| = sqrt(-laplace), iq = int(|q|/dq), the reference's interpolation of
+ * table(nq, nt, nt) (column-major, row iq = 1..nq at q = iq*dq), times |q|^2, capped at
+ * wcutoff.  mobility_qq: (nx/2+1, ny, nz, nt, nt) column-major. */
+void orc_mobility_qq(const orc_grid* g, int nt, int nq, double dq, const double* table, double wcutoff,
+                     double* mobility_qq);
+/* Get_Conjugate_Fields (main_multiblock.f90:566-640; main_blend.f90:376-428): Anderson
+ * mixing (history m, step lam; fresh history, iter_status = min(iter, m+1)) of the field
+ * on dfield = newdensity - density.  measure 0: accuracy = maxval(abs(dfield))
+ * (main_multiblock.f90:780); 1: sum(abs(dfield))*dvol (main_blend.f90:567).  Exit when
+ * .not.(accuracy > thresh).  field: inout; density: the target, unchanged.  Returns the
+ * loop index at exit (maxit+1 when the loop ran out) or -1 on an invalid ensemble. */
+int orc_conjugate_fields(const orc_grid* g, orc_chain* c, int ntypes, const double* density, double* field,
+                         int m, double lam, double thresh, int maxit, int measure, double* accuracy);
+/* gcfstat of main_multiblock.f90:619-634 for accuracy_scf: 0 success, 1 slow, 2 failed,
+ * 3 failed with accuracy >= 0.1 (the reference stops there). */
+int orc_gcfstat(double accuracy, double accuracy_scf);
+/* DDFT_Timestep_Euler (main_multiblock.f90:794-880) with the conjugate-field solves of
+ * the reference (accuracy_scf, itermax_scf, Anderson with history m and lam = 0.5).
+ * density, field: inout (nx,ny,nz,ntypes).  gcfstat: the two solves' statuses (may be
+ * NULL); iterations: their loop indices (may be NULL).  Returns 0, -1 on an invalid
+ * ensemble, or 3 when a solve ends with accuracy >= 0.1 (the reference stops). */
+int orc_ddft_step(const orc_grid* g, orc_chain* c, const orc_fh* fh, int ntypes, double* density, double* field,
+                  const double* mobility_qq, double timestep, int qdmin, double accuracy_scf, int itermax_scf,
+                  int m, int* gcfstat, int* iterations);
+
 #ifdef __cplusplus
 }
 #endif
