@@ -204,8 +204,13 @@ double wavenumber(int k, int n, double len) {
   return 2.0 * pi * (double)kk / len;
 }
 
+// Lines per workgroup of the Stockham kernels: up to 1024 points, on y and z up to 16
+// lines adjacent in x (coalesced rows), then halved until the pass has at least 512
+// workgroups or one line each (small grids such as Main_Multiblock's 86 x 1 x 75 would
+// otherwise run on a handful of CUs).
 int choose_tile(int axis, int n, int nx, int ny, int nz) {
-  int L = std::max(1, 1024 / std::max(n, 1));
+  n = std::max(n, 1);
+  int L = std::max(1, 1024 / n);
   if (axis == 0) {
     L = std::min(L, ny * nz);
   } else {
@@ -213,7 +218,10 @@ int choose_tile(int axis, int n, int nx, int ny, int nz) {
     L = std::min(L, 16);
     L = std::min(L, nx);
   }
-  return std::max(L, 1);
+  L = std::max(L, 1);
+  const long lines = (long)nx * ny * nz / n;
+  while (L > 1 && (lines + L - 1) / L < 512) L /= 2;
+  return L;
 }
 
 template <typename T>
@@ -406,7 +414,7 @@ int launch_pass(adscf_ctx* c, const PassArgs<T>& p) {
   if (p.axis == 0) items = (int)(((long)c->ny * c->nz + p.L - 1) / p.L);
   else if (p.axis == 1) items = c->nz * p.gx;
   else items = c->ny * p.gx;
-  const size_t smem = 2 * (size_t)p.L * p.ldn * sizeof(C2<T>);
+  const size_t smem = stockham_lds_bytes<T>(p.L, p.ldn, p.plan.n);
   hipLaunchKernelGGL(k_line_pass<T>, dim3(items), dim3(kThreads), smem, c->stream, p);
   HIPCK(hipGetLastError());
   return items;
@@ -712,7 +720,7 @@ void launch_fft(adscf_ctx* c, FftArgs<T> a) {
   if (a.axis == 0) items = (int)(((long)a.d1 * a.d2 + a.L - 1) / a.L);
   else if (a.axis == 1) items = a.d2 * a.gx;
   else items = a.d1 * a.gx;
-  const size_t smem = 2 * (size_t)a.L * a.ldn * sizeof(C2<T>);
+  const size_t smem = stockham_lds_bytes<T>(a.L, a.ldn, a.n);
   hipLaunchKernelGGL(k_fft_axis<T>, dim3(items), dim3(kThreads), smem, c->stream, a);
   HIPCK(hipGetLastError());
 }
@@ -728,6 +736,10 @@ FftArgs<T> fft_args(adscf_ctx* c, int axis, int d0, int dir) {
   a.n = c->ax[axis].n;
   a.plan = c->ax[axis].plan;
   a.L = axis == 0 ? std::max(1, std::min(1024 / a.n, c->ny * c->nz)) : std::max(1, std::min(16, d0));
+  {  // at least 512 workgroups where the lines allow it (choose_tile)
+    const long lines = axis == 0 ? (long)c->ny * c->nz : (long)d0 * (axis == 1 ? c->nz : c->ny);
+    while (a.L > 1 && (lines + a.L - 1) / a.L < 512) a.L /= 2;
+  }
   a.ldn = a.n + 1;
   a.gx = (d0 + a.L - 1) / a.L;
   a.stride = axis == 1 ? (long)d0 : (long)d0 * c->ny;

@@ -165,7 +165,9 @@ ADSCF_HD void stockham_pass(const C2<T>* src, C2<T>* dst, int n, int ldn, int L,
   }
 }
 
-// Generic radix R (any factor): one output per work item, O(R) each.
+// Generic radix R (any factor): one output per work item, O(R) each.  The twiddle
+// indices advance by addition (r*k*tstep < n needs no reduction; (r*q) mod R wraps once
+// per step), so the inner loop is loads and complex multiply-adds only.
 template <typename T>
 ADSCF_HD void stockham_pass_generic(const C2<T>* src, C2<T>* dst, int n, int ldn, int L, int Ns, int R, const C2<T>* tw,
                                     int dir, int tid, int nthr) {
@@ -180,11 +182,15 @@ ADSCF_HD void stockham_pass_generic(const C2<T>* src, C2<T>* dst, int n, int ldn
     const int q = (idx / Ns) % R;
     const int j = (idx / (Ns * R)) * Ns + k;
     const C2<T>* s = src + line * ldn;
-    C2<T> acc = {T(0), T(0)};
-    for (int r = 0; r < R; ++r) {
+    C2<T> acc = s[j];
+    int i1 = 0, rq = 0;
+    for (int r = 1; r < R; ++r) {
+      i1 += k * tstep;
+      rq += q;
+      if (rq >= R) rq -= R;
       C2<T> v = s[j + r * nb];
-      if (Ns > 1 && r > 0) v = cmul(v, twid(tw, r * k * tstep, dir));
-      acc = cadd(acc, cmul(v, twid(tw, ((r * q) % R) * rstep, dir)));
+      if (Ns > 1) v = cmul(v, twid(tw, i1, dir));
+      acc = cadd(acc, cmul(v, twid(tw, rq * rstep, dir)));
     }
     dst[line * ldn + idx] = acc;
   }

@@ -136,11 +136,11 @@ __device__ __forceinline__ T exp2i(int e) {
 }
 
 template <typename T>
-__device__ C2<T>* pass_line_op(const PassArgs<T>& a, C2<T>* b0, C2<T>* b1) {
+__device__ C2<T>* pass_line_op(const PassArgs<T>& a, C2<T>* b0, C2<T>* b1, const C2<T>* tw) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int n = a.plan.n;
   auto sync = [] { __syncthreads(); };
-  C2<T>* s = line_fft<T>(b0, b1, a.plan, a.ldn, a.L, a.tw, -1, tid, nthr, sync);
+  C2<T>* s = line_fft<T>(b0, b1, a.plan, a.ldn, a.L, tw, -1, tid, nthr, sync);
   const int tot = a.L * n;
   for (int e = tid; e < tot; e += nthr) {
     const int l = e / n, k = e - l * n;
@@ -150,7 +150,14 @@ __device__ C2<T>* pass_line_op(const PassArgs<T>& a, C2<T>* b0, C2<T>* b1) {
   }
   __syncthreads();
   C2<T>* o = (s == b0) ? b1 : b0;
-  return line_fft<T>(s, o, a.plan, a.ldn, a.L, a.tw, +1, tid, nthr, sync);
+  return line_fft<T>(s, o, a.plan, a.ldn, a.L, tw, +1, tid, nthr, sync);
+}
+
+// LDS size of the Stockham kernels: two tiles of L lines (leading dimension ldn) and
+// the n twiddles of the axis
+template <typename T>
+__host__ __device__ constexpr size_t stockham_lds_bytes(int L, int ldn, int n) {
+  return (2 * (size_t)L * ldn + (size_t)n) * sizeof(C2<T>);
 }
 
 template <typename T>
@@ -159,10 +166,12 @@ __global__ void __launch_bounds__(kThreads) k_line_pass(PassArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
   C2<T>* b0 = reinterpret_cast<C2<T>*>(smem_raw);
   C2<T>* b1 = b0 + a.L * a.ldn;
+  C2<T>* twl = b1 + a.L * a.ldn;  // the axis twiddles: every butterfly reads them
   const int n = a.plan.n;
   const int tot = a.L * n;
   const int item = blockIdx.x;
   const int tid = threadIdx.x, nthr = blockDim.x;
+  for (int e = tid; e < n; e += nthr) twl[e] = a.tw[e];
   // ---- load
   for (int e = tid; e < tot; e += nthr) {
     int l, j;
@@ -185,7 +194,7 @@ __global__ void __launch_bounds__(kThreads) k_line_pass(PassArgs<T> a) {
     b0[l * a.ldn + j] = v;
   }
   __syncthreads();
-  C2<T>* r = pass_line_op(a, b0, b1);
+  C2<T>* r = pass_line_op(a, b0, b1, twl);
   if (a.nops == 2) {
     const T sc1 = exp2i<T>(a.dexp1[a.sidx]);
     const T sc2 = exp2i<T>(a.dexp2[a.sidx]);
@@ -207,7 +216,7 @@ __global__ void __launch_bounds__(kThreads) k_line_pass(PassArgs<T> a) {
     }
     __syncthreads();
     C2<T>* other = (r == b0) ? b1 : b0;
-    r = pass_line_op(a, r, other);
+    r = pass_line_op(a, r, other, twl);
   }
   // ---- store
   if (a.out_kind == OUT_TMP) {
@@ -554,9 +563,11 @@ __global__ void __launch_bounds__(kThreads) k_fft_axis(FftArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
   C2<T>* b0 = reinterpret_cast<C2<T>*>(smem_raw);
   C2<T>* b1 = b0 + a.L * a.ldn;
+  C2<T>* twl = b1 + a.L * a.ldn;
   const int n = a.n, tot = a.L * n, item = blockIdx.x;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int nh = n / 2 + 1;
+  for (int e = tid; e < n; e += nthr) twl[e] = a.tw[e];
   for (int e = tid; e < tot; e += nthr) {
     int l, j;
     if (a.axis == 0) { l = e / n; j = e - l * n; } else { l = e % a.L; j = e / a.L; }
@@ -581,7 +592,7 @@ __global__ void __launch_bounds__(kThreads) k_fft_axis(FftArgs<T> a) {
   }
   __syncthreads();
   auto sync = [] { __syncthreads(); };
-  C2<T>* r = line_fft<T>(b0, b1, a.plan, a.ldn, a.L, a.tw, a.dir, tid, nthr, sync);
+  C2<T>* r = line_fft<T>(b0, b1, a.plan, a.ldn, a.L, twl, a.dir, tid, nthr, sync);
   for (int e = tid; e < tot; e += nthr) {
     int l, j;
     if (a.axis == 0) { l = e / n; j = e - l * n; } else { l = e % a.L; j = e / a.L; }

@@ -131,6 +131,64 @@ def run_batched(adscf_amd, fields, steps, warmup):
                 schedules=[l for l in report.splitlines() if "chosen" in l])
 
 
+def run_ddft(adscf_amd, fields, nsteps=6):
+    """Main_Multiblock's DDFT as shipped (86 x 1 x 75 grid, box 17.2 x 1 x 1, the ABABAB
+    polymer of lengths 0.5/0.1 x 5, ds = 0.01, kappa 200, chi 70, the shipped mobility table,
+    wcutoff 10, dt = 5e-5, qdmin) from grnd's random fields (seed 2007): the device-resident
+    DDFT_Timestep_Euler, and the unchanged driver on the compat path when it is built
+    (wall time of nsteps steps minus a zero-step run, as one times the reference)."""
+    import subprocess
+    import tempfile
+    from adscf_amd import formats
+
+    nx, ny, nz, box, ds = 86, 1, 75, (17.2, 1.0, 1.0), 0.01
+    bt = [1, 2, 1, 2, 1, 2]
+    bs = fields.block_segments([0.5, 0.1, 0.1, 0.1, 0.1, 0.1], 100)
+    mob_path = os.path.join(ROOT, "tests", "golden", "multiblock_input_mobilities")
+    mob = formats.read_mobilities(mob_path)
+    field = fields.MTStream(2007).grnd(2 * nx * ny * nz).reshape(2, nz, ny, nx)
+    with adscf_amd.Context(nx, ny, nz, *box, ds, precision=adscf_amd.FP64) as ctx:
+        ctx.set_interactions(200.0, [[0.0, 70.0], [70.0, 0.0]], [1.0, 1.0])
+        ctx.scf_setup(bt, bs, ensemble=1, rho=1.0, mixing_dim=10, lam_mix=0.5)
+        ctx.mobility_qq(mob.table, mob.dq, 10.0)
+        ctx.scf_set_field(field)
+        ctx.scf_evaluate()
+        ctx.ddft_set_density(ctx.scf_get()[1])
+        ctx.ddft_step(5e-5)  # warm-up: graph capture and work buffers
+        iters = []
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            iters.append(ctx.ddft_step(5e-5)["iterations"])
+        wall = time.perf_counter() - t0
+        ms_it = ctx.scf_timing()[0]
+    out = dict(workload="Main_Multiblock DDFT, 86x1x75, Ns=100 (6 blocks), shipped inputs, dt=5e-5",
+               steps=nsteps, ms_per_step=1000.0 * wall / nsteps, solve_iterations=iters,
+               ms_per_solve_iteration=ms_it)
+    exe = os.path.join(ROOT, "adscf_amd", "fortran", "_build", "Main_Multiblock", "main_multiblock.exe")
+    if os.path.exists(exe):
+        gen = ("17.2d0\n1.0d0\n1.0d0\n\n1.0d0\n5.65d0\n\n0.01d0\n\n0.\n{n}\n0.00005\n\n"
+               ".false.\n3\n.true.\n\n10.\n")
+        inter = "200.0d0\n\n70.0d0\n\n1.0d0\n1.0d0\n"
+        walls = {}
+        for n in (0, nsteps):
+            with tempfile.TemporaryDirectory() as d:
+                with open(os.path.join(d, "input_general"), "w") as f:
+                    f.write(gen.format(n=n))
+                with open(os.path.join(d, "input_interactions"), "w") as f:
+                    f.write(inter)
+                with open(mob_path) as src, open(os.path.join(d, "input_mobilities"), "w") as dst:
+                    dst.write(src.read())
+                t0 = time.perf_counter()
+                r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=600)
+                walls[n] = time.perf_counter() - t0
+                if r.returncode != 0:
+                    walls = None
+                    break
+        if walls:
+            out["unchanged_driver_ms_per_step"] = 1000.0 * (walls[nsteps] - walls[0]) / nsteps
+    return out
+
+
 def profiled_traffic():
     """Traffic per line-pass launch (bytes) of the sweep from the committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE runs, copy-calibrated); None if it is absent."""
@@ -218,6 +276,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the 128^3 x 400 configuration")
     ap.add_argument("--no-hipfft", action="store_true", help="skip the hipFFT comparator")
+    ap.add_argument("--no-ddft", action="store_true", help="skip the Main_Multiblock DDFT timing")
     ap.add_argument("--autotune", action="store_true",
                     help="re-tune the line-pass schedules first (the defaults are the tuner's MI355X choice)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -291,6 +350,8 @@ def main():
         line["batched"] = run_batched(adscf_amd, fields, max(5, args.steps // 5), 2)
     if not args.no_large:
         line["large"] = run_large(adscf_amd, fields, max(5, args.steps // 10), 3)
+    if not args.no_ddft:
+        line["ddft"] = run_ddft(adscf_amd, fields)
     if not args.no_converge:
         line["convergence"] = converge(adscf_amd, fields, w0, args.converge_max_iter)
     if not args.no_cpu_baseline:

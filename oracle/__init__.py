@@ -60,6 +60,11 @@ class _FH(C.Structure):
     _fields_ = [("ncomp", C.c_int), ("kappa", C.c_double), ("chi", _dp), ("volume", _dp)]
 
 
+class _Blend(C.Structure):
+    _fields_ = [("npol", C.c_int), ("types", _ip), ("segs", _ip), ("rho", _dp), ("mu", C.c_double * 8),
+                ("energy", C.c_double * 8)]
+
+
 def build():
     """Compile liboracle.so with the committed Makefile."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -105,6 +110,11 @@ def _load():
         "orc_gcfstat": (C.c_int, [C.c_double, C.c_double]),
         "orc_ddft_step": (C.c_int, [G, C.POINTER(_Chain), C.POINTER(_FH), C.c_int, _dp, _dp, _dp, C.c_double,
                                     C.c_int, C.c_double, C.c_int, C.c_int, _ip, _ip]),
+        "orc_blend_densities": (C.c_int, [G, C.POINTER(_Blend), C.c_int, _dp, _dp]),
+        "orc_blend_conjugate_fields": (C.c_int, [G, C.POINTER(_Blend), C.c_int, _dp, _dp, C.c_int, C.c_double,
+                                                 C.c_double, C.c_int, _dp]),
+        "orc_epd_step": (C.c_int, [G, C.POINTER(_Blend), C.POINTER(_FH), C.c_int, _dp, _dp, C.c_double, C.c_int,
+                                   C.c_double, C.c_int, _ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -449,3 +459,59 @@ def ddft_step(g: Grid, block_types, block_segs, fh: FH, density, field, mob_qq, 
                              int(itermax_scf), int(m), _pi(st), _pi(its))
     return dict(rc=rc, gcfstat=st.tolist(), iters=its.tolist(), mu=ch.mu, rho=ch.rho, energy=ch.energy,
                 Qsum=ch.Qsum)
+
+
+# ---- homopolymer blend and EPD (Main_Blend/main_blend.f90) ----------------------
+
+# single-precision literals assigned to doubles in Main_Blend
+EPD_EPS = float(np.float32(0.0001))          # main_blend.f90:630
+BLEND_CONJ_ACCURACY = float(np.float32(0.01))  # main_blend.f90:389
+
+
+class Blend:
+    """npol homopolymers: polymer i of monomer type types[i] with segs[i] segments,
+    canonical at concentration rho[i]."""
+
+    def __init__(self, types, segs, rho):
+        self.types = np.ascontiguousarray(types, dtype=np.int32)
+        self.segs = np.ascontiguousarray(segs, dtype=np.int32)
+        self.rho = np.ascontiguousarray(rho, dtype=np.float64)
+        self._b = _Blend(len(self.types), _pi(self.types), _pi(self.segs), _p(self.rho))
+
+    @property
+    def mu(self):
+        return list(self._b.mu[:len(self.types)])
+
+    @property
+    def energy(self):
+        return list(self._b.energy[:len(self.types)])
+
+
+def blend_densities(g: Grid, blend: Blend, field):
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    den = np.empty_like(f)
+    if lib().orc_blend_densities(C.byref(g._g), C.byref(blend._b), f.shape[0], _p(f), _p(den)) != 0:
+        raise ValueError("Invalid ensemble")
+    return den
+
+
+def blend_conjugate_fields(g: Grid, blend: Blend, density, field, m=2, lam=0.5, thresh=BLEND_CONJ_ACCURACY,
+                           maxit=100000):
+    d = np.ascontiguousarray(density, dtype=np.float64)
+    f = np.array(field, dtype=np.float64, order="C")
+    acc = C.c_double()
+    it = lib().orc_blend_conjugate_fields(C.byref(g._g), C.byref(blend._b), f.shape[0], _p(d), _p(f), int(m),
+                                          float(lam), float(thresh), int(maxit), C.byref(acc))
+    return dict(field=f, iters=it, accuracy=acc.value)
+
+
+def epd_step(g: Grid, blend: Blend, fh: FH, density, field, timestep, m=2, accuracy_scf=BLEND_CONJ_ACCURACY,
+             itermax=100000):
+    """Chain_Dynamics_Timestep_Euler (main_blend.f90:605-667); density and field (C-contiguous
+    float64) are updated in place.  Returns the conjugate solve's loop index."""
+    it = C.c_int()
+    rc = lib().orc_epd_step(C.byref(g._g), C.byref(blend._b), C.byref(fh._fh), density.shape[0], _p(density),
+                            _p(field), float(timestep), int(m), float(accuracy_scf), int(itermax), C.byref(it))
+    if rc != 0:
+        raise ValueError("Invalid ensemble")
+    return it.value

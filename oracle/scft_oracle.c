@@ -917,3 +917,143 @@ int orc_ddft_step(const orc_grid* g, orc_chain* c, const orc_fh* fh, int ntypes,
     if (orc_species_densities(g, c, ntypes, field, density) != 0) return -1;
   return 0;
 }
+
+/* ======================================================================
+ * Homopolymer blend and EPD: Main_Blend/main_blend.f90
+ * ====================================================================== */
+
+/* main_blend.f90:321-345 */
+int orc_blend_densities(const orc_grid* g, orc_blend* b, int ntypes, const double* field, double* density) {
+  long ng = (long)g->nx * g->ny * g->nz;
+  double* E = (double*)malloc(sizeof(double) * (size_t)ng * (size_t)ntypes);
+  double* dist = (double*)malloc(sizeof(double) * (size_t)ng);
+  double* den = (double*)malloc(sizeof(double) * (size_t)ng);
+  int rc = 0;
+  orc_calculate_expfield(g, ntypes, field, E);                                  /* :333 */
+  memset(density, 0, sizeof(double) * (size_t)ng * (size_t)ntypes);             /* :338 */
+  for (int i = 0; i < b->npol && rc == 0; ++i) {                                /* :334-336 */
+    orc_chain c = {1, &b->types[i], &b->segs[i], 1, b->rho[i], 0.0, 0.0, 0.0};
+    rc = orc_propagate_homopolymer(g, &c, E, dist, den);
+    b->mu[i] = c.mu;
+    b->energy[i] = c.energy;
+    double* d = density + (long)(b->types[i] - 1) * ng;                         /* :339-342 */
+    for (long k = 0; k < ng; ++k) d[k] = d[k] + den[k];
+  }
+  free(den);
+  free(dist);
+  free(E);
+  return rc;
+}
+
+/* main_blend.f90:376-428 with Mix_Fields (:490-554) and Check_Accuracy (:561-569) */
+int orc_blend_conjugate_fields(const orc_grid* g, orc_blend* b, int ntypes, const double* density, double* field,
+                               int m, double lam, double thresh, int maxit, double* accuracy) {
+  long ng = (long)g->nx * g->ny * g->nz;
+  long nvar = ng * ntypes;
+  double volume, dvol;
+  orc_volume_parameters(g, &volume, &dvol);
+  double* var = (double*)calloc((size_t)nvar * (size_t)(m + 1), sizeof(double));
+  double* dvar = (double*)calloc((size_t)nvar * (size_t)(m + 1), sizeof(double));
+  double* newdensity = (double*)malloc(sizeof(double) * (size_t)nvar);
+  double acc = 0.0;
+  int iter;
+  for (iter = 1; iter <= maxit; ++iter) {
+    if (orc_blend_densities(g, b, ntypes, field, newdensity) != 0) { iter = -1; break; } /* :405 */
+    acc = 0.0;
+    for (long i = 0; i < nvar; ++i) {                                                   /* :406-407 */
+      double df = newdensity[i] - density[i];
+      dvar[i] = df;
+      acc += fabs(df);
+    }
+    acc = acc * dvol;                                                                   /* :567 */
+    if (!(acc > thresh)) break;                                                         /* :412 */
+    memcpy(var, field, sizeof(double) * (size_t)nvar);
+    orc_mixing_anderson(var, dvar, nvar, m, lam, iter);                                 /* :418, :532 */
+    memcpy(field, var, sizeof(double) * (size_t)nvar);
+  }
+  *accuracy = acc;
+  free(newdensity);
+  free(dvar);
+  free(var);
+  return iter;
+}
+
+/* nabla_a * spectrum, nabla pure imaginary (mod_fourier.f90:189-191): (0 + i b)(x + i y) */
+static void mul_nabla(long nk, const double* nab, const double* in, double* out) {
+  for (long i = 0; i < nk; ++i) {
+    const double bb = nab[2 * i + 1], x = in[2 * i], y = in[2 * i + 1];
+    out[2 * i] = 0.0 * x - bb * y;
+    out[2 * i + 1] = 0.0 * y + bb * x;
+  }
+}
+
+/* main_blend.f90:605-667 */
+int orc_epd_step(const orc_grid* g, orc_blend* b, const orc_fh* fh, int ntypes, double* density, double* field,
+                 double timestep, int m, double accuracy_scf, int itermax, int* iterations) {
+  /* eps = 0.0001 (:630) assigns a default-kind (single precision) literal to a double:
+     eps is the float nearest 1e-4, 9.99999974737875e-5 */
+  const double eps = (double)0.0001f;
+  long ng = (long)g->nx * g->ny * g->nz;
+  long nvar = ng * ntypes;
+  int nh = g->nx / 2 + 1;
+  long nk = (long)nh * g->ny * g->nz;
+  double* nabla = (double*)malloc(sizeof(double) * 2 * (size_t)nk * 3);
+  double* old = (double*)malloc(sizeof(double) * (size_t)nvar);
+  double* newfield = (double*)malloc(sizeof(double) * (size_t)nvar);
+  double* mulocal = (double*)malloc(sizeof(double) * (size_t)nvar);
+  double* dfield = (double*)malloc(sizeof(double) * (size_t)nvar * 3);
+  double* dden = (double*)malloc(sizeof(double) * (size_t)nvar * 3);
+  double* newden = (double*)malloc(sizeof(double) * (size_t)nvar);
+  double* f0 = (double*)malloc(sizeof(double) * 2 * (size_t)nk);
+  double* f1 = (double*)malloc(sizeof(double) * 2 * (size_t)nk);
+  double* tmp = (double*)malloc(sizeof(double) * (size_t)ng);
+  int rc = 0;
+  orc_calculate_laplace(g, NULL, NULL, nabla);
+  if (orc_blend_densities(g, b, ntypes, field, old) != 0) rc = -1;               /* :632 */
+  if (rc == 0) {
+    orc_mean_fields(g, fh, old, newfield);                                      /* :633 */
+    for (long i = 0; i < nvar; ++i) mulocal[i] = newfield[i] - field[i];        /* :634 */
+    for (int t = 0; t < ntypes; ++t) {                                          /* :636-644 */
+      orc_real2fourier(g, mulocal + (long)t * ng, f0);
+      for (int a = 0; a < 3; ++a) {
+        mul_nabla(nk, nabla + 2 * (long)a * nk, f0, f1);
+        orc_fourier2real(g, f1, dfield + ((long)a * ntypes + t) * ng);
+      }
+    }
+    for (int a = 0; a < 3 && rc == 0; ++a) {                                     /* :646-657 */
+      const double* dfa = dfield + (long)a * ntypes * ng;
+      for (long i = 0; i < nvar; ++i) newfield[i] = field[i] - eps * dfa[i];
+      if (orc_blend_densities(g, b, ntypes, newfield, newden) != 0) { rc = -1; break; }
+      for (int t = 0; t < ntypes; ++t) {
+        double* dd = dden + ((long)a * ntypes + t) * ng;
+        for (long i = 0; i < ng; ++i) tmp[i] = newden[(long)t * ng + i] - old[(long)t * ng + i];
+        orc_real2fourier(g, tmp, f0);
+        mul_nabla(nk, nabla + 2 * (long)a * nk, f0, f1);
+        orc_fourier2real(g, f1, dd);
+      }
+    }
+  }
+  if (rc == 0) {
+    for (long i = 0; i < nvar; ++i) {                                           /* :662 */
+      double sum = dden[i];
+      sum = sum + dden[nvar + i];
+      sum = sum + dden[2 * nvar + i];
+      density[i] = density[i] + sum * timestep / eps;
+    }
+    double acc;
+    int it = orc_blend_conjugate_fields(g, b, ntypes, density, field, m, 0.5, accuracy_scf, itermax, &acc); /* :663 */
+    if (iterations) *iterations = it;
+    if (it < 0) rc = -1;
+  }
+  free(tmp);
+  free(f1);
+  free(f0);
+  free(newden);
+  free(dden);
+  free(dfield);
+  free(mulocal);
+  free(newfield);
+  free(old);
+  free(nabla);
+  return rc;
+}

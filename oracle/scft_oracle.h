@@ -148,6 +148,37 @@ int orc_ddft_step(const orc_grid* g, orc_chain* c, const orc_fh* fh, int ntypes,
                   const double* mobility_qq, double timestep, int qdmin, double accuracy_scf, int itermax_scf,
                   int m, int* gcfstat, int* iterations);
 
+/* ---- Homopolymer blend and EPD (Main_Blend/main_blend.f90) -------------- */
+/* A blend of npol homopolymers, polymer i of monomer type types[i] (1-based) with segs[i]
+ * segments, canonical with concentration rho[i] (Initialize_Homopolymer,
+ * mod_homopolymer.f90:43-78; main_blend.f90:93-100). */
+typedef struct {
+  int npol;
+  const int* types;
+  const int* segs;
+  const double* rho;
+  double mu[8], energy[8]; /* outputs of the last density evaluation */
+} orc_blend;
+/* Calculate_Densities of the blend (main_blend.f90:321-345): expfield of the field, every
+ * polymer's Propagate_Homopolymer, species densities summed in polymer order. */
+int orc_blend_densities(const orc_grid* g, orc_blend* b, int ntypes, const double* field, double* density);
+/* Get_Conjugate_Fields of Main_Blend (main_blend.f90:376-428): Anderson (history m, lam)
+ * on dfield = newdensity - density with accuracy sum(abs(dfield))*dvol, exit when
+ * .not.(accuracy > thresh) (the driver's accuracy_scf = 0.01, a single-precision literal:
+ * (double)0.01f).  Returns the loop index at
+ * exit (maxit+1 when the loop ran out), -1 on an invalid ensemble. */
+int orc_blend_conjugate_fields(const orc_grid* g, orc_blend* b, int ntypes, const double* density, double* field,
+                               int m, double lam, double thresh, int maxit, double* accuracy);
+/* Chain_Dynamics_Timestep_Euler (main_blend.f90:605-667): densities and mean fields of the
+ * field, mulocal = newfield - field, its gradient along each axis through nabla, the density
+ * response to field - eps*gradient (eps = (double)0.0001f, the driver's single-precision
+ * literal) per axis, its divergence through nabla,
+ * density += sum over axes * timestep/eps, then the conjugate fields of the new density
+ * (history m, lam 0.5, accuracy_scf, itermax).  density, field: inout.  iterations: the
+ * solve's loop index (may be NULL).  Returns 0 or -1. */
+int orc_epd_step(const orc_grid* g, orc_blend* b, const orc_fh* fh, int ntypes, double* density, double* field,
+                 double timestep, int m, double accuracy_scf, int itermax, int* iterations);
+
 #ifdef __cplusplus
 }
 #endif

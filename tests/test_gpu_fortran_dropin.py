@@ -157,3 +157,40 @@ def test_unchanged_blend_driver_scf(tmp_path, orc):
     energy, mu1, mu2 = nums[0], nums[3], nums[4]
     assert abs(energy - e_ref) < 1e-10 * abs(e_ref)
     assert abs(mu1 - pol[0].mu) < 1e-10 and abs(mu2 - pol[1].mu) < 1e-10
+
+
+@pytest.mark.skipif(not os.path.exists(BLEND_EXE), reason="driver not built (adscf_amd/fortran/Makefile needs the reference tree)")
+def test_unchanged_blend_driver_epd(tmp_path, orc):
+    """The unchanged Main_Blend driver with qepd = .true. and one time step of
+    Chain_Dynamics_Timestep_Euler (main_blend.f90:605-667): four blend density evaluations,
+    the nabla products through Real2Fourier/Fourier2Real and the host nabla array, and the
+    Anderson conjugate-field solve (mixing_dim 2, accuracy sum|dfield| dvol < 0.01) on the
+    GPU path.  The density after the step is deterministic given the start (the random
+    fields of grnd and their densities) and is compared with the oracle's step; the field the
+    solve returns is checked by its outcome (Anderson's first solve amplifies rounding, see
+    tests/test_oracle_ddft.py), and the printed energy is recomputed from the written state."""
+    (tmp_path / "input_general").write_text(BLEND_GENERAL.replace("10\t\tnumber of timesteps", "1\t\tnumber of timesteps")
+                                            .replace(".false.\t\tqepd", ".true.\t\tqepd"))
+    (tmp_path / "input_interactions").write_text(BLEND_INTERACTIONS)
+    res = subprocess.run([BLEND_EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(res.stdout[-2000:], res.stderr[-2000:])
+    assert res.returncode == 0 and "Final values after" in res.stdout
+    nx, ny, nz = 64, 16, 1
+    dens = _read_rows_in_order(tmp_path / "dens.dat", nx, ny, nz)
+    field = _read_rows_in_order(tmp_path / "field.dat", nx, ny, nz)
+    g = orc.Grid(nx, ny, nz, 20.0, 5.0, 1.0, 0.1)
+    vol, dvol = g.volume()
+    bl = orc.Blend([1, 2], [10, 10], [50.0 / vol, 50.0 / vol])
+    fh = orc.FH(10.0, [[0.0, 10.0], [10.0, 0.0]], [1.0, 1.0])
+    f_ref = orc.MT(2007).grnd(2 * nx * ny * nz).reshape(2, nz, ny, nx).copy()
+    d_ref = orc.blend_densities(g, bl, f_ref)
+    it = orc.epd_step(g, bl, fh, d_ref, f_ref, 0.001)
+    print(f"oracle solve {it} iterations; max|dphi| driver-oracle {np.max(np.abs(dens - d_ref)):.2e}")
+    assert np.max(np.abs(dens - d_ref)) <= 1e-11 * np.max(d_ref)
+    chain = orc.blend_densities(g, bl, field)
+    assert np.sum(np.abs(chain - dens)) * dvol <= orc.BLEND_CONJ_ACCURACY * (1 + 1e-9)
+    e_ref = -dvol * np.sum(dens * field) + sum(bl.energy) + orc.interaction_energy(g, fh, dens)
+    nums = [float(t.replace("D", "E")) for t in res.stdout.split("Energy:")[-1].replace("C:", " ").replace(
+        "mu_scf:", " ").split()]
+    assert abs(nums[0] - e_ref) <= 1e-10 * abs(e_ref)
+    assert abs(nums[3] - bl.mu[0]) <= 1e-10 and abs(nums[4] - bl.mu[1]) <= 1e-10

@@ -163,3 +163,64 @@ def test_conjugate_solve_first_anderson_step_amplifies_rounding(orc):
         b = orc.conjugate_fields(g, BT6, bs, target, w1, m=10, thresh=-1.0, maxit=n)
         diff.append(np.max(np.abs(a["field"] - b["field"])))
     assert diff[0] < 1e-14 and diff[1] > 1e-5
+
+
+# ---- Main_Blend EPD (Chain_Dynamics_Timestep_Euler, main_blend.f90:605-667) ----------
+
+def _blend_setup(orc, nx=32, ny=8, lx=10.0, ly=2.5):
+    g = orc.Grid(nx, ny, 1, lx, ly, 1.0, 0.1)
+    vol, dvol = g.volume()
+    bl = orc.Blend([1, 2], [10, 10], [25.0 / vol, 25.0 / vol])
+    fh = orc.FH(10.0, [[0.0, 10.0], [10.0, 0.0]], [1.0, 1.0])
+    return g, vol, dvol, bl, fh
+
+
+def _blend_energy(orc, g, dvol, bl, fh, den, field):
+    orc.blend_densities(g, bl, field)  # chain energies of the field (Calculate_Energy, main_blend.f90:356-366)
+    return -dvol * np.sum(den * field) + sum(bl.energy) + orc.interaction_energy(g, fh, den)
+
+
+def test_epd_step_conserves_mass_and_lowers_free_energy(orc):
+    """Chain dynamics is a gradient flow of F: with conjugate fields solved tightly, F falls
+    at every small step and each type's mass stays; a sign slip in either nabla product would
+    make F rise."""
+    g, vol, dvol, bl, fh = _blend_setup(orc)
+    field = orc.MT(2007).grnd(2 * g.ng).reshape(2, 1, g.ny, g.nx).copy()
+    den = orc.blend_densities(g, bl, field)
+    mass = den.sum(axis=(1, 2, 3))
+    F = [_blend_energy(orc, g, dvol, bl, fh, den, field)]
+    for _ in range(3):
+        orc.epd_step(g, bl, fh, den, field, 0.001, accuracy_scf=1e-10)
+        F.append(_blend_energy(orc, g, dvol, bl, fh, den, field))
+        assert np.allclose(den.sum(axis=(1, 2, 3)), mass, rtol=1e-13, atol=0)
+    assert all(b < a for a, b in zip(F, F[1:])), F
+    # the returned field is conjugate to the new density
+    d = orc.blend_densities(g, bl, field)
+    assert np.sum(np.abs(d - den)) * dvol <= 1e-10
+
+
+def test_epd_step_reduces_to_one_dimension(orc):
+    """A state uniform along y evolves exactly as the same state on an (nx, 1, 1) grid: pins
+    the per-axis wave numbers and spacings of the nabla products."""
+    g, vol, dvol, bl, fh = _blend_setup(orc)
+    g1 = orc.Grid(g.nx, 1, 1, g.lx, 1.0, 1.0, 0.1)
+    bl1 = orc.Blend([1, 2], [10, 10], [25.0 / vol, 25.0 / vol])
+    prof = orc.MT(2009).grnd(2 * g.nx).reshape(2, 1, 1, g.nx)
+    f3, f1 = np.repeat(prof, g.ny, axis=2).copy(), prof.copy()
+    d3, d1 = orc.blend_densities(g, bl, f3), orc.blend_densities(g1, bl1, f1)
+    # two simple-mixing iterations of the solve (Anderson, m = 2, would start at the third)
+    orc.epd_step(g, bl, fh, d3, f3, 0.001, accuracy_scf=-1.0, itermax=2)
+    orc.epd_step(g1, bl1, fh, d1, f1, 0.001, accuracy_scf=-1.0, itermax=2)
+    for y in range(g.ny):
+        assert np.allclose(d3[:, :, y, :], d1[:, :, 0, :], rtol=0, atol=1e-12)
+        assert np.allclose(f3[:, :, y, :], f1[:, :, 0, :], rtol=0, atol=1e-12)
+
+
+def test_epd_step_uniform_state_is_stationary(orc):
+    g, vol, dvol, bl, fh = _blend_setup(orc)
+    field = np.empty((2, 1, g.ny, g.nx))
+    field[0], field[1] = 0.3, -0.2
+    den = orc.blend_densities(g, bl, field)
+    d0, f0 = den.copy(), field.copy()
+    it = orc.epd_step(g, bl, fh, den, field, 0.001)
+    assert it == 1 and np.allclose(den, d0, rtol=1e-14, atol=0) and np.array_equal(field, f0)
