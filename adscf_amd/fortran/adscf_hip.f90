@@ -175,6 +175,72 @@ module adscf_hip
       real(c_double), intent(out) :: x(*)
       integer(c_int), value :: n
     end function
+    ! device-resident loops (no counterpart in the reference; see include/adscf.h)
+    integer(c_int) function adscf_scf_setup(ctx, nblocks, monomer_type, block_segments, ensemble, rho, mu, &
+                                            mixing_dim, lam_mix) bind(C, name='adscf_scf_setup')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nblocks, ensemble, mixing_dim
+      integer(c_int), intent(in) :: monomer_type(*), block_segments(*)
+      real(c_double), value :: rho, mu, lam_mix
+    end function
+    integer(c_int) function adscf_scf_set_field(ctx, field) bind(C, name='adscf_scf_set_field')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: field(*)
+    end function
+    integer(c_int) function adscf_scf_iterate(ctx, niter, tol, iters, accuracy, converged) &
+        bind(C, name='adscf_scf_iterate')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: niter
+      real(c_double), value :: tol
+      integer(c_int), intent(out) :: iters, converged
+      real(c_double), intent(out) :: accuracy
+    end function
+    integer(c_int) function adscf_scf_get(ctx, field, density) bind(C, name='adscf_scf_get')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: field(*), density(*)
+    end function
+    integer(c_int) function adscf_conjugate_fields(ctx, density, thresh, maxit, measure, iterations, accuracy) &
+        bind(C, name='adscf_conjugate_fields')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: density(*)
+      real(c_double), value :: thresh
+      integer(c_int), value :: maxit, measure
+      integer(c_int), intent(out) :: iterations
+      real(c_double), intent(out) :: accuracy
+    end function
+    integer(c_int) function adscf_ddft_set_mobility(ctx, mobility_qq) bind(C, name='adscf_ddft_set_mobility')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: mobility_qq(*)
+    end function
+    integer(c_int) function adscf_ddft_set_density(ctx, density) bind(C, name='adscf_ddft_set_density')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: density(*)
+    end function
+    integer(c_int) function adscf_ddft_get_density(ctx, density) bind(C, name='adscf_ddft_get_density')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: density(*)
+    end function
+    integer(c_int) function adscf_ddft_step(ctx, timestep, qdmin, accuracy_scf, itermax_scf, gcfstat, iterations) &
+        bind(C, name='adscf_ddft_step')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: timestep, accuracy_scf
+      integer(c_int), value :: qdmin, itermax_scf
+      integer(c_int), intent(out) :: gcfstat(2), iterations(2)
+    end function
+    integer(c_int) function adscf_ddft_energy(ctx, energy, mu, rho) bind(C, name='adscf_ddft_energy')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: energy, mu, rho
+    end function
     integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
       import :: c_ptr, c_size_t
       type(c_ptr), value :: s
