@@ -88,6 +88,9 @@ SIGNATURES = {
     "adscf_ddft_get_density": (C.c_int, [_ctxp, _dp]),
     "adscf_ddft_step": (C.c_int, [_ctxp, C.c_double, C.c_int, C.c_double, C.c_int, _ip, _ip]),
     "adscf_ddft_energy": (C.c_int, [_ctxp, _dp, _dp, _dp]),
+    "adscf_blend_setup": (C.c_int, [_ctxp, C.c_int, _ip, _ip, _dp, C.c_int, C.c_double]),
+    "adscf_blend_scalars": (C.c_int, [_ctxp, _dp, _dp, _dp]),
+    "adscf_epd_step": (C.c_int, [_ctxp, C.c_double, C.c_double, C.c_int, _ip]),
     "adscf_autotune": (C.c_int, [_ctxp, C.c_int, C.c_char_p, C.c_int]),
     "adscf_get_schedule": (C.c_int, [_ctxp, C.c_int, _ip, _ip, _ip, _ip]),
     "adscf_set_schedule": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -441,6 +444,30 @@ class Context:
         self._chk(self._lib.adscf_ddft_step(self._h, float(timestep), int(bool(qdmin)), float(accuracy_scf),
                                             int(itermax_scf), _pi(st), _pi(its)))
         return dict(gcfstat=st.tolist(), iterations=its.tolist())
+
+    # -- homopolymer blend and EPD (main_blend.f90) ------------------------------
+    def blend_setup(self, monomer_types, segments, rho, mixing_dim=2, lam_mix=0.5):
+        """The device-resident loop over a blend of homopolymers (canonical, one per entry)."""
+        t = np.ascontiguousarray(monomer_types, dtype=np.int32)
+        sg = np.ascontiguousarray(segments, dtype=np.int32)
+        r = np.ascontiguousarray(rho, dtype=np.float64)
+        self._npol = len(t)
+        self._chk(self._lib.adscf_blend_setup(self._h, len(t), _pi(t), _pi(sg), _p(r), int(mixing_dim),
+                                              float(lam_mix)))
+
+    def blend_scalars(self):
+        n = self._npol
+        q, mu, e = np.empty(n), np.empty(n), np.empty(n)
+        self._chk(self._lib.adscf_blend_scalars(self._h, _p(q), _p(mu), _p(e)))
+        return dict(Qsum=q, mu=mu, energy=e)
+
+    def epd_step(self, timestep, accuracy_scf=float(np.float32(0.01)), itermax=100000):
+        """Chain_Dynamics_Timestep_Euler (main_blend.f90:605-667); returns the conjugate
+        solve's loop index."""
+        it = C.c_int()
+        self._chk(self._lib.adscf_epd_step(self._h, float(timestep), float(accuracy_scf), int(itermax),
+                                           C.byref(it)))
+        return it.value
 
     def ddft_energy(self):
         F, mu, rho = C.c_double(), C.c_double(), C.c_double()

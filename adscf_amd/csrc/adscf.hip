@@ -132,6 +132,12 @@ struct adscf_ctx {
   DevBuf<double> mob;                // mobility_qq (nk, nt, nt)
   DevBuf<double> dmu, dmuq, ddq, dd; // DDFT step work arrays
   bool target_set = false, mob_set = false;
+  // homopolymer blend (adscf_blend_setup): one single-block chain per polymer, canonical;
+  // empty for a single copolymer chain (adscf_scf_setup)
+  std::vector<ChainDesc> bl_cd;
+  std::vector<double> bl_rho;
+  DevBuf<ScfState> pst;              // chain scalars (Q, norm, mu, energy) of each polymer
+  DevBuf<double> nab, dold, dfa;     // EPD: nabla (imaginary parts), old densities, gradients
 };
 
 namespace {
@@ -614,6 +620,49 @@ void fill_ones_slot0(adscf_ctx* c) {
   HIPCK(hipGetLastError());
 }
 
+// contour slices the history must hold: the chain, or the longest polymer of a blend
+int hist_ns(const adscf_ctx* c) {
+  int ns = c->cd.ns;
+  for (const ChainDesc& cd : c->bl_cd) ns = std::max(ns, cd.ns);
+  return ns;
+}
+
+// Calculate_Densities of a homopolymer blend (Main_Blend/main_blend.f90:321-345): per
+// polymer the rescale schedule, the propagator sweep, Q and the normalisation
+// (Propagate_Homopolymer, mod_homopolymer.f90:90-141, as a one-block chain whose packed
+// complementary sweep is the same propagator), then its densities added to phi in polymer
+// order.  With `resid` the last polymer's contour also forms the residual of the loop.
+// Returns the contour's block count.
+template <typename T>
+int enqueue_blend_densities(adscf_ctx* c, const int* done, bool resid) {
+  const long ng = c->ng;
+  const int nb = grid_for(ng);
+  int nbc = 0;
+  for (size_t i = 0; i < c->bl_cd.size(); ++i) {
+    const ChainDesc& cd = c->bl_cd[i];
+    hipLaunchKernelGGL(k_schedule, dim3(1), dim3(kThreads), 0, c->stream, cd, (const double*)c->e2part.p, nb, ng,
+                       c->nt, c->e1.p, c->e2.p, c->d1.p, c->d2.p, (ScfState*)nullptr, 0, c->m, done);
+    run_chain<T>(c, cd, c->pst.p + i, done, c->bl_rho[i], 0.0);
+    ContourArgs<T> a = contour_args<T>(c, cd, c->pst.p + i);
+    a.mode = CONTOUR_SCF;
+    a.rst = c->st.p;
+    a.phi = c->phi.p;
+    a.w0 = c->wring.p;
+    a.done = done;
+    a.accumulate = i > 0;
+    const bool last = i + 1 == c->bl_cd.size();
+    a.no_resid = !(resid && last);
+    if (resid && last) {
+      a.dring = c->dring.p;
+      a.target = c->resid_kind ? c->target.p : nullptr;
+      a.measure = c->measure;
+    }
+    nbc = launch_contour<T>(c->stream, a);
+  }
+  HIPCK(hipGetLastError());
+  return nbc;
+}
+
 // ---------------------------------------------------------------------------
 // compat: Propagate_Multiblock
 // ---------------------------------------------------------------------------
@@ -863,18 +912,23 @@ void enqueue_iteration(adscf_ctx* c, bool passes_too = true) {
   const long ng = c->ng;
   const long nvar = ng * c->nt;
   const int* done = &c->st.p->done;
-  run_chain<T>(c, c->cd, c->st.p, done, c->rho, c->mu, passes_too);
   const int nb = grid_for(ng);
-  ContourArgs<T> a = contour_args<T>(c, c->cd, c->st.p);
-  a.mode = CONTOUR_SCF;
-  a.phi = c->phi.p;
-  a.dring = c->dring.p;
-  a.done = done;
-  a.w0 = c->wring.p;  // ring base; the kernel reads slot `head` from the device state
-  a.target = c->resid_kind ? c->target.p : nullptr;
-  a.measure = c->measure;
-  const int nbc = launch_contour<T>(c->stream, a);
-  HIPCK(hipGetLastError());
+  int nbc;
+  if (!c->bl_cd.empty()) {
+    nbc = enqueue_blend_densities<T>(c, done, true);
+  } else {
+    run_chain<T>(c, c->cd, c->st.p, done, c->rho, c->mu, passes_too);
+    ContourArgs<T> a = contour_args<T>(c, c->cd, c->st.p);
+    a.mode = CONTOUR_SCF;
+    a.phi = c->phi.p;
+    a.dring = c->dring.p;
+    a.done = done;
+    a.w0 = c->wring.p;  // ring base; the kernel reads slot `head` from the device state
+    a.target = c->resid_kind ? c->target.p : nullptr;
+    a.measure = c->measure;
+    nbc = launch_contour<T>(c->stream, a);
+    HIPCK(hipGetLastError());
+  }
   const int m = c->m;
   launch_gram(c->stream, c->dring.p, nvar, m, (const ScfState*)c->st.p, 0, c->gpart.p, c->gsum.p, done);
   SolveArgs s{};
@@ -904,6 +958,10 @@ void enqueue_iteration(adscf_ctx* c, bool passes_too = true) {
 template <typename T>
 void enqueue_evaluate(adscf_ctx* c) {
   const long ng = c->ng;
+  if (!c->bl_cd.empty()) {
+    enqueue_blend_densities<T>(c, nullptr, false);
+    return;
+  }
   run_chain<T>(c, c->cd, c->st.p, nullptr, c->rho, c->mu);
   ContourArgs<T> a = contour_args<T>(c, c->cd, c->st.p);
   a.mode = CONTOUR_SCF;
@@ -918,7 +976,7 @@ void enqueue_evaluate(adscf_ctx* c) {
 template <typename T>
 void scf_set_field_impl(adscf_ctx* c, const double* field) {
   ensure_tables<T>(c);
-  ensure_hist<T>(c, c->cd.ns);
+  ensure_hist<T>(c, hist_ns(c));
   ensure_partials(c);
   Engine<T>& En = eng<T>(c);
   const long ng = c->ng, nvar = ng * c->nt;
@@ -950,14 +1008,15 @@ std::vector<const void*> graph_key(adscf_ctx* c) {
   Engine<T>& En = eng<T>(c);
   return {En.E.p, En.tmp.p, En.h1.p, En.h2.p, En.tw[0].p, En.tw[1].p, En.tw[2].p, En.kf[0].p, En.kf[1].p,
           En.kf[2].p, c->wring.p, c->dring.p, c->phi.p, c->qpart.p, c->rpart.p, c->spart.p, c->e2part.p,
-          c->gpart.p, c->gsum.p, c->resid.p, c->e1.p, c->e2.p, c->d1.p, c->d2.p, c->st.p, c->target.p};
+          c->gpart.p, c->gsum.p, c->resid.p, c->e1.p, c->e2.p, c->d1.p, c->d2.p, c->st.p, c->target.p,
+          c->pst.p};
 }
 
 // E and the rescale schedule of the current field in precision T (the history is kept)
 template <typename T>
 void refresh_E(adscf_ctx* c) {
   ensure_tables<T>(c);
-  ensure_hist<T>(c, c->cd.ns);
+  ensure_hist<T>(c, hist_ns(c));
   ensure_partials(c);
   fill_ones_slot0<T>(c);
   ScfState st{};
@@ -1110,6 +1169,76 @@ int ddft_step_impl(adscf_ctx* c, double dt, int qdmin, double accuracy_scf, int 
   }
   HIPCK(hipStreamSynchronize(c->stream));
   return ADSCF_OK;
+}
+
+// nabla's imaginary parts (nk, 3) on the device, from the host table of Calculate_Laplace
+void ensure_nabla(adscf_ctx* c) {
+  const long nk = (long)(c->nx / 2 + 1) * c->ny * c->nz;
+  if (c->nab.n >= (size_t)(3 * nk)) return;
+  std::vector<double> nab(6 * nk), im(3 * nk);
+  const int rc = adscf_laplace(c, nullptr, nullptr, nab.data());
+  if (rc != ADSCF_OK) throw ArgError{"nabla table"};
+  for (long i = 0; i < 3 * nk; ++i) im[i] = nab[2 * i + 1];
+  c->nab.ensure(3 * nk);
+  HIPCK(hipMemcpy(c->nab.p, im.data(), 3 * nk * sizeof(double), hipMemcpyHostToDevice));
+}
+
+// Chain_Dynamics_Timestep_Euler (Main_Blend/main_blend.f90:605-667) on the device-resident
+// density (target buffer) and field (ring): densities and mean fields of the field, the
+// gradient of mulocal along each axis, the density response to field - eps*gradient per
+// axis and its divergence, the Euler update, then the conjugate fields of the new density
+// (sum|dfield| dvol accuracy, main_blend.f90:567).
+template <typename T>
+int epd_step_impl(adscf_ctx* c, double dt, double thresh, int itermax, int* iters) {
+  const double eps = (double)0.0001f;  // eps = 0.0001 (:630): a single-precision literal
+  const long ng = c->ng, nvar = ng * c->nt;
+  const long nk = (long)(c->nx / 2 + 1) * c->ny * c->nz;
+  const int nbv = grid_for(nvar), nbk = grid_for(nk);
+  ensure_nabla(c);
+  c->dold.ensure(nvar);
+  c->dmu.ensure(nvar);
+  c->dfa.ensure(3 * nvar);
+  c->dd.ensure(3 * nvar);
+  c->dmuq.ensure(2 * nk);
+  c->ddq.ensure(2 * nk);
+  Engine<T>& En = eng<T>(c);
+  FHDesc fh = c->fh;
+  fh.nt = c->nt;
+  enqueue_evaluate<T>(c);  // olddensity (:632); E is that of the ring's field
+  HIPCK(hipMemcpyAsync(c->dold.p, c->phi.p, nvar * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_ddft_mulocal, dim3(grid_for(ng)), dim3(kThreads), 0, c->stream, (const double*)c->dold.p,
+                     (const double*)c->wring.p, (const ScfState*)c->st.p, fh, ng, c->dmu.p);  // :633-634
+  for (int t = 0; t < c->nt; ++t) {  // :636-644
+    enqueue_r2c<T>(c, c->dmu.p + t * ng, c->dmuq.p);
+    for (int ax = 0; ax < 3; ++ax) {
+      hipLaunchKernelGGL(k_nabla_mul, dim3(nbk), dim3(kThreads), 0, c->stream, (const double*)c->dmuq.p,
+                         (const double*)(c->nab.p + ax * nk), nk, c->ddq.p);
+      enqueue_c2r<T>(c, c->ddq.p, c->dfa.p + ax * nvar + t * ng);
+    }
+  }
+  for (int ax = 0; ax < 3; ++ax) {  // :646-657
+    hipLaunchKernelGGL(k_field_step, dim3(nbv), dim3(kThreads), 0, c->stream, (const double*)c->wring.p,
+                       (const ScfState*)c->st.p, (const double*)(c->dfa.p + ax * nvar), nvar, eps, c->dmu.p);
+    hipLaunchKernelGGL(k_expfield<T>, dim3(grid_for(ng)), dim3(kThreads), 0, c->stream, (const double*)c->dmu.p,
+                       En.E.p, ng, c->nt, c->ds, c->e2part.p, (const int*)nullptr);
+    enqueue_evaluate<T>(c);
+    double* dda = c->dd.p + ax * nvar;
+    hipLaunchKernelGGL(k_sub, dim3(nbv), dim3(kThreads), 0, c->stream, (const double*)c->phi.p,
+                       (const double*)c->dold.p, nvar, dda);
+    for (int t = 0; t < c->nt; ++t) {
+      enqueue_r2c<T>(c, dda + t * ng, c->dmuq.p);
+      hipLaunchKernelGGL(k_nabla_mul, dim3(nbk), dim3(kThreads), 0, c->stream, (const double*)c->dmuq.p,
+                         (const double*)(c->nab.p + ax * nk), nk, c->ddq.p);
+      enqueue_c2r<T>(c, c->ddq.p, dda + t * ng);
+    }
+  }
+  hipLaunchKernelGGL(k_epd_update, dim3(nbv), dim3(kThreads), 0, c->stream, c->target.p, (const double*)c->dd.p, nvar,
+                     dt, eps);  // :662
+  HIPCK(hipGetLastError());
+  refresh_E<T>(c);  // E and the rescale schedule of the ring's field again
+  c->phi_fresh = false;
+  double acc = 0.0;
+  return conjugate_impl<T>(c, thresh, itermax, 1, iters, &acc);  // :663
 }
 
 template <typename T>
@@ -2337,6 +2466,8 @@ int adscf_scf_setup(adscf_ctx* c, int nblocks, const int* monomer_type, const in
     if (mixing_dim < 1 || mixing_dim > kMaxHist) throw ArgError{"mixing_dim must be in [1, 40]"};
     if (!c->fh_set) throw ArgError{"adscf_set_interactions has not been called"};
     c->cd = make_chain(c, nblocks, monomer_type, block_segments, ensemble);
+    c->bl_cd.clear();
+    c->bl_rho.clear();
     c->m = mixing_dim;
     c->lam_mix = lam_mix;
     c->rho = rho;
@@ -2435,6 +2566,18 @@ int adscf_scf_energy(adscf_ctx* c, double* F, double* Fc, double* qsum, double* 
       rho_v = s * c->dvol / c->volume / (c->ds * c->cd.ns);
       energy = -rho_v * c->volume;
     }
+    if (!c->bl_cd.empty()) {  // Calculate_Energy of the blend (main_blend.f90:356-366)
+      std::vector<ScfState> ps(c->bl_cd.size());
+      HIPCK(hipMemcpy(ps.data(), c->pst.p, ps.size() * sizeof(ScfState), hipMemcpyDeviceToHost));
+      energy = 0.0;
+      rho_v = 0.0;
+      for (size_t i = 0; i < ps.size(); ++i) {
+        energy += ps[i].energy;
+        rho_v += c->bl_rho[i];
+      }
+      st.qsum = ps[0].qsum;
+      st.mu = ps[0].mu;
+    }
     std::vector<double> p;
     host_energy_parts(c, c->phi.p, c->wring.p + (size_t)st.head * c->ng * c->nt, p);
     const double chain_energy = energy - c->dvol * p[0];
@@ -2445,6 +2588,60 @@ int adscf_scf_energy(adscf_ctx* c, double* F, double* Fc, double* qsum, double* 
     if (mu) *mu = st.mu;
     if (rho) *rho = rho_v;
     return ADSCF_OK;
+  });
+}
+
+int adscf_blend_setup(adscf_ctx* c, int npol, const int* monomer_type, const int* segments, const double* rho,
+                      int mixing_dim, double lam_mix) {
+  return guarded(c, [&]() -> int {
+    if (!monomer_type || !segments || !rho) throw ArgError{"null array"};
+    if (npol < 1 || npol > 16) throw ArgError{"npol must be in [1, 16]"};
+    if (mixing_dim < 1 || mixing_dim > kMaxHist) throw ArgError{"mixing_dim must be in [1, 40]"};
+    if (!c->fh_set) throw ArgError{"adscf_set_interactions has not been called"};
+    std::vector<ChainDesc> cds;
+    for (int i = 0; i < npol; ++i) cds.push_back(make_chain(c, 1, monomer_type + i, segments + i, 1));
+    c->bl_cd = cds;
+    c->bl_rho.assign(rho, rho + npol);
+    c->cd = cds[0];
+    c->rho = rho[0];
+    c->mu = 0.0;
+    c->m = mixing_dim;
+    c->lam_mix = lam_mix;
+    c->pst.ensure(npol);
+    c->scf_ready = true;
+    c->field_set = false;
+    invalidate_tables(c);
+    return ADSCF_OK;
+  });
+}
+
+int adscf_blend_scalars(adscf_ctx* c, double* qsum, double* mu, double* energy) {
+  return guarded(c, [&]() -> int {
+    if (c->bl_cd.empty()) {
+      c->err = "adscf_blend_setup has not been called";
+      return ADSCF_ERR_STATE;
+    }
+    std::vector<ScfState> ps(c->bl_cd.size());
+    HIPCK(hipStreamSynchronize(c->stream));
+    HIPCK(hipMemcpy(ps.data(), c->pst.p, ps.size() * sizeof(ScfState), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < ps.size(); ++i) {
+      if (qsum) qsum[i] = ps[i].qsum;
+      if (mu) mu[i] = ps[i].mu;
+      if (energy) energy[i] = ps[i].energy;
+    }
+    return ADSCF_OK;
+  });
+}
+
+int adscf_epd_step(adscf_ctx* c, double timestep, double accuracy_scf, int itermax, int* iterations) {
+  return guarded(c, [&]() -> int {
+    if (!c->field_set || !c->target_set) {
+      c->err = "adscf_epd_step needs adscf_scf_set_field and adscf_ddft_set_density";
+      return ADSCF_ERR_STATE;
+    }
+    if (itermax < 1) throw ArgError{"itermax must be >= 1"};
+    return c->prec == ADSCF_FP64 ? epd_step_impl<double>(c, timestep, accuracy_scf, itermax, iterations)
+                                 : epd_step_impl<float>(c, timestep, accuracy_scf, itermax, iterations);
   });
 }
 
@@ -2574,9 +2771,18 @@ int adscf_ddft_energy(adscf_ctx* c, double* F, double* mu, double* rho) {
     ScfState st{};
     HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
+    double chain = st.energy;
+    if (!c->bl_cd.empty()) {  // the polymers' energies (main_blend.f90:359-362)
+      std::vector<ScfState> ps(c->bl_cd.size());
+      HIPCK(hipMemcpy(ps.data(), c->pst.p, ps.size() * sizeof(ScfState), hipMemcpyDeviceToHost));
+      chain = 0.0;
+      for (const ScfState& q : ps) chain += q.energy;
+      st.mu = ps[0].mu;
+      st.rho = c->bl_rho[0];
+    }
     std::vector<double> p;
     host_energy_parts(c, c->target.p, c->wring.p + (size_t)st.head * c->ng * c->nt, p);
-    if (F) *F = st.energy - c->dvol * p[0] + interaction_from_parts(c, p);
+    if (F) *F = chain - c->dvol * p[0] + interaction_from_parts(c, p);
     if (mu) *mu = st.mu;
     if (rho) *rho = st.rho;
     return ADSCF_OK;

@@ -228,6 +228,9 @@ struct ContourArgs {
   int m;
   const double* target;  // CONTOUR_SCF: null -> dfield = mean field - field; else dfield = phi - target
   int measure;        // 0: per-block max|dfield|; 1: per-block sum|dfield|
+  const ScfState* rst;  // CONTOUR_SCF: state holding the ring head (null: st)
+  int accumulate;     // CONTOUR_SCF: add this chain's densities to phi (blends: one chain per polymer)
+  int no_resid;       // CONTOUR_SCF: densities only (a blend's chains before the last)
   double* rpart;      // per-block max (measure 0) or sum (measure 1) of |dfield|
   double* spart;      // per-block sum of densities (grand canonical rho)
   const int* done;
@@ -269,7 +272,7 @@ __global__ void __launch_bounds__(kThreads) k_contour(ContourArgs<T> a) {
     double phi[kMaxTypes][VP];
     for (int t = 0; t < a.fh.nt; ++t)
 #pragma unroll
-      for (int v = 0; v < VP; ++v) phi[t][v] = 0.0;
+      for (int v = 0; v < VP; ++v) phi[t][v] = a.accumulate ? a.phi[(long)t * ng + i0 + v] : 0.0;
     for (int b = 0; b < a.cd.nblocks; ++b) {
       const int s0 = a.cd.start[b], nb = a.cd.seg[b];
       double acc[VP];
@@ -320,10 +323,15 @@ __global__ void __launch_bounds__(kThreads) k_contour(ContourArgs<T> a) {
         }
       }
     }
-    if (a.mode == CONTOUR_SCF) {
+    if (a.mode == CONTOUR_SCF && a.no_resid) {
+      for (int t = 0; t < a.fh.nt; ++t)
+#pragma unroll
+        for (int v = 0; v < VP; ++v) a.phi[(long)t * ng + i0 + v] = phi[t][v];
+    } else if (a.mode == CONTOUR_SCF) {
       const int nt = a.fh.nt;
-      double* d0 = a.dring ? a.dring + (long)a.st->head * nt * ng : nullptr;
-      const double* w0 = a.w0 + (long)a.st->head * nt * ng;  // current field: ring slot `head`
+      const int head = (a.rst ? a.rst : a.st)->head;
+      double* d0 = a.dring ? a.dring + (long)head * nt * ng : nullptr;
+      const double* w0 = a.w0 + (long)head * nt * ng;  // current field: ring slot `head`
 #pragma unroll
       for (int v = 0; v < VP; ++v) {
         const long i = i0 + v;
@@ -869,6 +877,49 @@ __global__ void __launch_bounds__(kThreads) k_ddft_euler(double* __restrict__ de
     double d = den[i] + dt * dd[i];
     if (clamp) d = d > floor ? d : floor;
     den[i] = d;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Chain_Dynamics_Timestep_Euler (Main_Blend/main_blend.f90:605-667)
+// ---------------------------------------------------------------------------
+
+// nabla_a * spectrum with nabla_a = i b (mod_fourier.f90:189-191), as Fortran's complex
+// product (0 + i b)(x + i y) = (0 x - b y) + i (0 y + b x)
+__global__ void __launch_bounds__(kThreads) k_nabla_mul(const double* __restrict__ in, const double* __restrict__ nab,
+                                                        long nk, double* __restrict__ out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nk; i += (long)gridDim.x * blockDim.x) {
+    const double b = nab[i], x = in[2 * i], y = in[2 * i + 1];
+    out[2 * i] = 0.0 * x - b * y;
+    out[2 * i + 1] = 0.0 * y + b * x;
+  }
+}
+
+// newfield = field - eps*dfield (:647); the field is ring slot `head`
+__global__ void __launch_bounds__(kThreads) k_field_step(const double* __restrict__ wring, const ScfState* st,
+                                                         const double* __restrict__ d, long nvar, double eps,
+                                                         double* __restrict__ out) {
+  const double* w = wring + (long)st->head * nvar;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nvar; i += (long)gridDim.x * blockDim.x)
+    out[i] = w[i] - eps * d[i];
+}
+
+// out = a - b
+__global__ void __launch_bounds__(kThreads) k_sub(const double* __restrict__ a, const double* __restrict__ b, long n,
+                                                  double* __restrict__ out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = a[i] - b[i];
+}
+
+// density = density + sum(ddensity, dim=5) * timestep/eps (:662): the three axes summed in
+// order, then times timestep, then divided by eps
+__global__ void __launch_bounds__(kThreads) k_epd_update(double* __restrict__ den, const double* __restrict__ dd,
+                                                         long n, double dt, double eps) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double sum = dd[i];
+    sum = sum + dd[n + i];
+    sum = sum + dd[2 * n + i];
+    den[i] = den[i] + sum * dt / eps;
   }
 }
 

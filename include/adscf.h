@@ -180,7 +180,8 @@ int adscf_conjugate_fields(adscf_ctx* ctx, const double* density, double thresh,
 int adscf_mobility_qq(adscf_ctx* ctx, int nq, double dq, const double* table, double wcutoff, double* mobility_qq);
 /* upload a mobility_qq array computed by the caller (the driver's own Read_Mobilities) */
 int adscf_ddft_set_mobility(adscf_ctx* ctx, const double* mobility_qq);
-/* the DDFT density (nx,ny,nz,ntypes), kept on the device between steps */
+/* the density of the time-stepping entry points (DDFT and EPD), (nx,ny,nz,ntypes), kept on
+ * the device between steps */
 int adscf_ddft_set_density(adscf_ctx* ctx, const double* density);
 int adscf_ddft_get_density(adscf_ctx* ctx, double* density);
 /* DDFT_Timestep_Euler (main_multiblock.f90:794-880) on the device-resident density and field:
@@ -195,6 +196,27 @@ int adscf_ddft_step(adscf_ctx* ctx, double timestep, int qdmin, double accuracy_
 /* Calculate_Energy (main_multiblock.f90:548-557) with the DDFT density and field; mu and rho
  * of the last density evaluation.  Any pointer may be NULL. */
 int adscf_ddft_energy(adscf_ctx* ctx, double* F, double* mu, double* rho);
+
+/* ---- homopolymer blend and EPD (Main_Blend/main_blend.f90) --------------------- */
+/* The device-resident loop over a blend of npol homopolymers instead of one copolymer:
+ * polymer i of monomer type monomer_type[i] (1-based) with segments[i] segments, canonical
+ * at concentration rho[i] (Initialize_Homopolymer, Polymer_Modules/mod_homopolymer.f90:43-78;
+ * main_blend.f90:93-100).  Calculate_Densities of the blend (main_blend.f90:321-345) then
+ * replaces the chain's in adscf_scf_iterate/_evaluate/_get/_energy, adscf_conjugate_fields
+ * and adscf_epd_step; Anderson history mixing_dim (2 in Main_Blend's Iterate), step lam_mix.
+ * adscf_scf_setup returns the context to a single chain. */
+int adscf_blend_setup(adscf_ctx* ctx, int npol, const int* monomer_type, const int* segments, const double* rho,
+                      int mixing_dim, double lam_mix);
+/* per polymer (arrays of npol, any may be NULL): Qsum, mu and energy of the last evaluation */
+int adscf_blend_scalars(adscf_ctx* ctx, double* qsum, double* mu, double* energy);
+/* Chain_Dynamics_Timestep_Euler (main_blend.f90:605-667) on the device-resident density
+ * (adscf_ddft_set_density) and field: densities and mean fields of the field, the gradient of
+ * mulocal along each axis through nabla, the density response to field - eps*gradient
+ * (eps = 0.0001 as the driver's single-precision literal) and its divergence, density +=
+ * timestep/eps times their sum, then Get_Conjugate_Fields of the new density with the
+ * accuracy sum|dfield|*dvol and exit threshold accuracy_scf (the driver: 0.01 single
+ * precision, itermax 100000).  iterations: the solve's loop index. */
+int adscf_epd_step(adscf_ctx* ctx, double timestep, double accuracy_scf, int itermax, int* iterations);
 
 /* ---- schedule autotuner (no counterpart in the reference) --------------------- */
 /* Times the captured propagator sweep of the configured SCF problem for every line-FFT
