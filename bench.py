@@ -189,6 +189,54 @@ def run_ddft(adscf_amd, fields, nsteps=6):
     return out
 
 
+def run_epd(adscf_amd, fields, nsteps=10):
+    """Main_Blend's EPD as shipped (64 x 16 x 1 grid, box 20 x 5 x 1, two homopolymers of 50
+    chains each, length 1 at ds = 0.1, kappa = chi = 10, dt = 1e-3, Anderson history 2) from
+    grnd's random fields (seed 2007): the device-resident Chain_Dynamics_Timestep_Euler, and
+    the unchanged driver on the compat path when it is built (nsteps minus 0 steps)."""
+    import subprocess
+    import tempfile
+
+    nx, ny, nz, box, ds = 64, 16, 1, (20.0, 5.0, 1.0), 0.1
+    vol = box[0] * box[1] * box[2]
+    rho = [50.0 / vol, 50.0 / vol]
+    field = fields.MTStream(2007).grnd(2 * nx * ny * nz).reshape(2, nz, ny, nx)
+    with adscf_amd.Context(nx, ny, nz, *box, ds, precision=adscf_amd.FP64) as ctx:
+        ctx.set_interactions(10.0, [[0.0, 10.0], [10.0, 0.0]], [1.0, 1.0])
+        ctx.blend_setup([1, 2], [10, 10], rho, mixing_dim=2, lam_mix=0.5)
+        ctx.scf_set_field(field)
+        ctx.scf_evaluate()
+        ctx.ddft_set_density(ctx.scf_get()[1])
+        ctx.epd_step(0.001)  # warm-up
+        iters = []
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            iters.append(ctx.epd_step(0.001))
+        wall = time.perf_counter() - t0
+    out = dict(workload="Main_Blend EPD, 64x16x1, two homopolymers (10 segments), shipped inputs, dt=1e-3",
+               steps=nsteps, ms_per_step=1000.0 * wall / nsteps, solve_iterations=iters)
+    exe = os.path.join(ROOT, "adscf_amd", "fortran", "_build", "Main_Blend", "main_blend.exe")
+    if os.path.exists(exe):
+        gen = ("20.0d0\n5.0d0\n1.0d0\n\n50.0d0\n27.0d0\n1.0d0\n\n50.0d0\n27.0d0\n1.0d0\n\n0.1d0\n\n"
+               "{n}\n0.001\n\n.false.\n3\n.true.\n")
+        walls = {}
+        for n in (0, nsteps):
+            with tempfile.TemporaryDirectory() as d:
+                with open(os.path.join(d, "input_general"), "w") as f:
+                    f.write(gen.format(n=n))
+                with open(os.path.join(d, "input_interactions"), "w") as f:
+                    f.write("10.0d0\n\n10.0d0\n\n1.0d0\n1.0d0\n")
+                t0 = time.perf_counter()
+                r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=600)
+                walls[n] = time.perf_counter() - t0
+                if r.returncode != 0:
+                    walls = None
+                    break
+        if walls:
+            out["unchanged_driver_ms_per_step"] = 1000.0 * (walls[nsteps] - walls[0]) / nsteps
+    return out
+
+
 def profiled_traffic():
     """Traffic per line-pass launch (bytes) of the sweep from the committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE runs, copy-calibrated); None if it is absent."""
@@ -276,7 +324,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the 128^3 x 400 configuration")
     ap.add_argument("--no-hipfft", action="store_true", help="skip the hipFFT comparator")
-    ap.add_argument("--no-ddft", action="store_true", help="skip the Main_Multiblock DDFT timing")
+    ap.add_argument("--no-ddft", action="store_true",
+                    help="skip the Main_Multiblock DDFT and Main_Blend EPD timings")
     ap.add_argument("--autotune", action="store_true",
                     help="re-tune the line-pass schedules first (the defaults are the tuner's MI355X choice)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -352,6 +401,7 @@ def main():
         line["large"] = run_large(adscf_amd, fields, max(5, args.steps // 10), 3)
     if not args.no_ddft:
         line["ddft"] = run_ddft(adscf_amd, fields)
+        line["epd"] = run_epd(adscf_amd, fields)
     if not args.no_converge:
         line["convergence"] = converge(adscf_amd, fields, w0, args.converge_max_iter)
     if not args.no_cpu_baseline:
