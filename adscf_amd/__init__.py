@@ -81,6 +81,7 @@ SIGNATURES = {
     "adscf_scf_residuals": (C.c_int, [_ctxp, _dp, C.c_int, _ip]),
     "adscf_scf_timing": (C.c_int, [_ctxp, _dp, _dp]),
     "adscf_scf_set_residual": (C.c_int, [_ctxp, C.c_int, C.c_int, _dp]),
+    "adscf_scf_set_mixing": (C.c_int, [_ctxp, C.c_int]),
     "adscf_conjugate_fields": (C.c_int, [_ctxp, _dp, C.c_double, C.c_int, C.c_int, _ip, _dp]),
     "adscf_mobility_qq": (C.c_int, [_ctxp, C.c_int, C.c_double, _dp, C.c_double, _dp]),
     "adscf_ddft_set_mobility": (C.c_int, [_ctxp, _dp]),
@@ -400,6 +401,10 @@ class Context:
         measure 0 max|dfield|, 1 sum|dfield| dvol."""
         t = None if target is None else _f64(target, (self.ntypes,) + self.shape)
         self._chk(self._lib.adscf_scf_set_residual(self._h, int(kind), int(measure), None if t is None else _p(t)))
+
+    def scf_set_mixing(self, mixing_type=3):
+        """Mixing of scf_iterate: 1 simple, 2 lambda (lammax = lam_mix), 3 Anderson."""
+        self._chk(self._lib.adscf_scf_set_mixing(self._h, int(mixing_type)))
 
     # -- conjugate fields and DDFT (main_multiblock.f90) -------------------------
     def conjugate_fields(self, density=None, thresh=5e-4, maxit=2000, measure=0):

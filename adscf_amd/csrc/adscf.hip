@@ -128,6 +128,8 @@ struct adscf_ctx {
   // measure 0 max|dfield|, 1 sum|dfield| dvol
   int resid_kind = 0, measure = 0, graph_mode = -1;
   int user_kind = 0, user_measure = 0;  // the loop adscf_scf_iterate runs (adscf_scf_set_residual)
+  int user_mixing = 3, mixing = 3;       // 1 simple, 2 lambda, 3 Anderson (adscf_scf_set_mixing)
+  DevBuf<double> gsum_w;                 // lambda mixing: Gram sums of the field ring
   DevBuf<double> target;             // target densities (nt, ng): the DDFT density
   DevBuf<double> mob;                // mobility_qq (nk, nt, nt)
   DevBuf<double> dmu, dmuq, ddq, dd; // DDFT step work arrays
@@ -930,7 +932,13 @@ void enqueue_iteration(adscf_ctx* c, bool passes_too = true) {
     HIPCK(hipGetLastError());
   }
   const int m = c->m;
-  launch_gram(c->stream, c->dring.p, nvar, m, (const ScfState*)c->st.p, 0, c->gpart.p, c->gsum.p, done);
+  if (c->mixing == 2) {
+    // Mixing_Lambda needs <D0-D1, D0-D1> and <W0-W1, W0-W1> from iteration 2 on
+    launch_gram(c->stream, c->dring.p, nvar, m, (const ScfState*)c->st.p, 1, c->gpart.p, c->gsum.p, done);
+    launch_gram(c->stream, c->wring.p, nvar, m, (const ScfState*)c->st.p, 1, c->gpart.p, c->gsum_w.p, done);
+  } else if (c->mixing == 3) {
+    launch_gram(c->stream, c->dring.p, nvar, m, (const ScfState*)c->st.p, 0, c->gpart.p, c->gsum.p, done);
+  }
   SolveArgs s{};
   s.m = m;
   s.lam_mix = c->lam_mix;
@@ -946,6 +954,8 @@ void enqueue_iteration(adscf_ctx* c, bool passes_too = true) {
   s.ns = c->cd.ns;
   s.ensemble = c->cd.ensemble;
   s.measure = c->measure;
+  s.mixing = c->mixing;
+  s.gsum_w = c->gsum_w.p;
   s.st = c->st.p;
   hipLaunchKernelGGL(k_solve, dim3(1), dim3(kThreads), 0, c->stream, s);
   hipLaunchKernelGGL(k_mix<T>, dim3(nb), dim3(kThreads), 0, c->stream, c->wring.p, (const double*)c->dring.p, ng,
@@ -986,6 +996,7 @@ void scf_set_field_impl(adscf_ctx* c, const double* field) {
   c->phi.ensure(nvar);
   c->gpart.ensure((size_t)kGramBlocks * (m + 1) * (m + 2) / 2);
   c->gsum.ensure((size_t)(m + 1) * (m + 2) / 2);
+  c->gsum_w.ensure((size_t)(m + 1) * (m + 2) / 2);
   c->resid.ensure(200000);
   HIPCK(hipMemsetAsync(c->wring.p, 0, sizeof(double) * (m + 1) * nvar, c->stream));
   HIPCK(hipMemsetAsync(c->dring.p, 0, sizeof(double) * (m + 1) * nvar, c->stream));
@@ -1009,7 +1020,7 @@ std::vector<const void*> graph_key(adscf_ctx* c) {
   return {En.E.p, En.tmp.p, En.h1.p, En.h2.p, En.tw[0].p, En.tw[1].p, En.tw[2].p, En.kf[0].p, En.kf[1].p,
           En.kf[2].p, c->wring.p, c->dring.p, c->phi.p, c->qpart.p, c->rpart.p, c->spart.p, c->e2part.p,
           c->gpart.p, c->gsum.p, c->resid.p, c->e1.p, c->e2.p, c->d1.p, c->d2.p, c->st.p, c->target.p,
-          c->pst.p};
+          c->pst.p, c->gsum_w.p};
 }
 
 // E and the rescale schedule of the current field in precision T (the history is kept)
@@ -1038,7 +1049,7 @@ template <typename T>
 void build_graph(adscf_ctx* c) {
   ensure_tables<T>(c);
   std::vector<const void*> key = graph_key<T>(c);
-  const int mode = c->resid_kind * 2 + c->measure;
+  const int mode = (c->mixing * 2 + c->resid_kind) * 2 + c->measure;
   if (c->graph && c->graph_prec == c->prec && c->graph_mode == mode && key == c->graph_key) return;
   if (c->graph) {
     (void)hipGraphExecDestroy(c->graph);
@@ -1105,6 +1116,7 @@ template <typename T>
 int conjugate_impl(adscf_ctx* c, double thresh, int maxit, int measure, int* iters, double* acc) {
   c->resid_kind = 1;
   c->measure = measure;
+  c->mixing = 3;  // both drivers solve with mixing_type = 3 (main_multiblock.f90:823, main_blend.f90:390)
   hipLaunchKernelGGL(k_scf_restart, dim3(1), dim3(64), 0, c->stream, c->st.p);
   HIPCK(hipGetLastError());
   int conv = 0;
@@ -2503,6 +2515,7 @@ int adscf_scf_iterate(adscf_ctx* c, int niter, double tol, int* iters, double* a
     if (niter < 0) throw ArgError{"niter must be >= 0"};
     c->resid_kind = c->user_kind;
     c->measure = c->user_measure;
+    c->mixing = c->user_mixing;
     return c->prec == ADSCF_FP64 ? scf_iterate_impl<double>(c, niter, tol, iters, accuracy, converged)
                                  : scf_iterate_impl<float>(c, niter, tol, iters, accuracy, converged);
   });
@@ -2658,6 +2671,14 @@ int adscf_scf_set_residual(adscf_ctx* c, int kind, int measure, const double* ta
     if (kind == 1 && !c->target_set) throw ArgError{"a density residual needs target densities"};
     c->user_kind = kind;
     c->user_measure = measure;
+    return ADSCF_OK;
+  });
+}
+
+int adscf_scf_set_mixing(adscf_ctx* c, int mixing_type) {
+  return guarded(c, [&]() -> int {
+    if (mixing_type < 1 || mixing_type > 3) throw ArgError{"mixing_type must be 1 (simple), 2 (lambda) or 3 (Anderson)"};
+    c->user_mixing = mixing_type;
     return ADSCF_OK;
   });
 }

@@ -628,6 +628,8 @@ struct SolveArgs {
   int force_anderson;  // compat: caller decides via iter
   int iter_override;   // compat: iteration number passed by the caller
   int measure;         // 0: accuracy = max|dfield|; 1: accuracy = sum|dfield| * dvol
+  int mixing;          // 3 Anderson (default 0 reads as 3), 1 simple, 2 lambda
+  const double* gsum_w;  // lambda mixing: Gram sums of the field ring
   ScfState* st;
 };
 
@@ -673,6 +675,30 @@ __global__ void k_solve(SolveArgs a) {
     }
     __syncthreads();
     if (done_now) return;
+  }
+  if (a.mixing == 1 || a.mixing == 2) {
+    // Mixing_Simple (mod_iterate.f90:27-38): a fixed step.  Mixing_Lambda (:46-84): the
+    // step sqrt(sum (W0-W1)^2 / sum (D0-D1)^2) clamped to [1e-5, lammax], 1e-5 at
+    // iteration 1 or for a vanishing denominator; pair (1,1) of both Gram sums is
+    // <y_1, y_1> with y_1 = X_0 - X_1.
+    if (tid == 0) {
+      double lam = a.lam_mix;
+      if (a.mixing == 2) {
+        const double lammin = 0.00001;
+        lam = lammin;
+        if (it != 1) {
+          const double num = a.gsum_w[m + 1], nom = a.gsum[m + 1];
+          if (nom > 1.0e-50) {
+            lam = sqrt(num / nom);
+            if (lam < lammin) lam = lammin;
+            else if (lam > a.lam_mix) lam = a.lam_mix;
+          }
+        }
+      }
+      st->mode = 0;
+      st->lam = lam;
+    }
+    return;
   }
   const int status = it < 1000 ? it : 1000;
   if (status <= m) {

@@ -164,6 +164,13 @@ int adscf_scf_timing(adscf_ctx* ctx, double* ms_per_iter, double* ms_sweep);
  * 1: sum(abs(dfield))*dvol (Main_Blend/main_blend.f90:567).  Default: kind 0, measure 0. */
 int adscf_scf_set_residual(adscf_ctx* ctx, int kind, int measure, const double* target);
 
+/* Mixing of the loop adscf_scf_iterate runs (Mix_Fields' mixing_type, main_multiblock.f90:738-747):
+ * 1 Mixing_Simple (General_Routines/mod_iterate.f90:27-38, step lam_mix), 2 Mixing_Lambda
+ * (:46-84, lammax = lam_mix; Main_Diblock_Melt's and Main_Blend's saddle-point search),
+ * 3 Mixing_Anderson (:92-150, the default).  The conjugate-field solves always use 3, as
+ * both drivers do. */
+int adscf_scf_set_mixing(adscf_ctx* ctx, int mixing_type);
+
 /* ---- conjugate fields and DDFT (Main_Multiblock/main_multiblock.f90) ------------ */
 /* Get_Conjugate_Fields (main_multiblock.f90:566-640; main_blend.f90:376-428): iterate the
  * current field (adscf_scf_set_field, or the field a previous call left) with the Anderson

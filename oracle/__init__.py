@@ -115,6 +115,8 @@ def _load():
                                                  C.c_double, C.c_int, _dp]),
         "orc_epd_step": (C.c_int, [G, C.POINTER(_Blend), C.POINTER(_FH), C.c_int, _dp, _dp, C.c_double, C.c_int,
                                    C.c_double, C.c_int, _ip]),
+        "orc_blend_scf": (C.c_int, [G, C.POINTER(_Blend), C.POINTER(_FH), C.c_int, _dp, _dp, C.c_int, C.c_int,
+                                    C.c_double, C.c_double, C.c_int, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -515,3 +517,18 @@ def epd_step(g: Grid, blend: Blend, fh: FH, density, field, timestep, m=2, accur
     if rc != 0:
         raise ValueError("Invalid ensemble")
     return it.value
+
+
+BLEND_SCF_ACCURACY = float(np.float32(0.0000001))  # main_blend.f90:451
+
+
+def blend_scf(g: Grid, blend: Blend, fh: FH, field, m=2, mixing_type=2, lam=0.5, thresh=BLEND_SCF_ACCURACY,
+              maxit=100000):
+    """Find_SCF_Saddle_Point of Main_Blend (main_blend.f90:437-486).  Returns dict(field,
+    density, iters, accuracy)."""
+    f = np.array(field, dtype=np.float64, order="C")
+    d = np.empty_like(f)
+    acc = C.c_double()
+    it = lib().orc_blend_scf(C.byref(g._g), C.byref(blend._b), C.byref(fh._fh), f.shape[0], _p(f), _p(d), int(m),
+                             int(mixing_type), float(lam), float(thresh), int(maxit), C.byref(acc))
+    return dict(field=f, density=d, iters=it, accuracy=acc.value)

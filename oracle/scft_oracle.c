@@ -978,6 +978,45 @@ int orc_blend_conjugate_fields(const orc_grid* g, orc_blend* b, int ntypes, cons
   return iter;
 }
 
+/* main_blend.f90:437-486 with Mix_Fields (:494-553) and Check_Accuracy (:561-569) */
+int orc_blend_scf(const orc_grid* g, orc_blend* b, const orc_fh* fh, int ntypes, double* field, double* density,
+                  int m, int mixing_type, double lam, double thresh, int maxit, double* accuracy) {
+  long ng = (long)g->nx * g->ny * g->nz;
+  long nvar = ng * ntypes;
+  double volume, dvol;
+  orc_volume_parameters(g, &volume, &dvol);
+  double* var = (double*)calloc((size_t)nvar * (size_t)(m + 1), sizeof(double));
+  double* dvar = (double*)calloc((size_t)nvar * (size_t)(m + 1), sizeof(double));
+  double* newfield = (double*)malloc(sizeof(double) * (size_t)nvar);
+  double acc = 0.0;
+  int iter;
+  for (iter = 1; iter <= maxit; ++iter) {
+    if (orc_blend_densities(g, b, ntypes, field, density) != 0) { iter = -1; break; } /* :460 */
+    orc_mean_fields(g, fh, density, newfield);                                        /* :461 */
+    acc = 0.0;
+    for (long i = 0; i < nvar; ++i) {                                                 /* :462-463 */
+      double df = newfield[i] - field[i];
+      dvar[i] = df;
+      acc += fabs(df);
+    }
+    acc = acc * dvol;
+    if (!(acc > thresh)) break;                                                       /* :468 */
+    memcpy(var, field, sizeof(double) * (size_t)nvar);
+    if (mixing_type == 1)                                                             /* :527-532 */
+      orc_mixing_simple(var, dvar, nvar, m, lam);
+    else if (mixing_type == 2)
+      orc_mixing_lambda(var, dvar, nvar, m, lam, iter);
+    else
+      orc_mixing_anderson(var, dvar, nvar, m, lam, iter);
+    memcpy(field, var, sizeof(double) * (size_t)nvar);
+  }
+  *accuracy = acc;
+  free(newfield);
+  free(dvar);
+  free(var);
+  return iter;
+}
+
 /* nabla_a * spectrum, nabla pure imaginary (mod_fourier.f90:189-191): (0 + i b)(x + i y) */
 static void mul_nabla(long nk, const double* nab, const double* in, double* out) {
   for (long i = 0; i < nk; ++i) {

@@ -169,6 +169,14 @@ int orc_blend_densities(const orc_grid* g, orc_blend* b, int ntypes, const doubl
  * exit (maxit+1 when the loop ran out), -1 on an invalid ensemble. */
 int orc_blend_conjugate_fields(const orc_grid* g, orc_blend* b, int ntypes, const double* density, double* field,
                                int m, double lam, double thresh, int maxit, double* accuracy);
+/* Find_SCF_Saddle_Point of Main_Blend (main_blend.f90:437-486) with Mix_Fields
+ * (:494-553): densities of the blend, mean fields, dfield = newfield - field, accuracy
+ * sum(abs(dfield))*dvol (:567), exit when .not.(accuracy > thresh), else mixing_type 1
+ * (Mixing_Simple), 2 (Mixing_Lambda, the driver's choice, lammax = lam) or 3
+ * (Mixing_Anderson) with iter_status = iter.  field: inout; density: the densities of the
+ * last evaluated field.  Returns the loop index at exit (maxit+1 when the loop ran out). */
+int orc_blend_scf(const orc_grid* g, orc_blend* b, const orc_fh* fh, int ntypes, double* field, double* density,
+                  int m, int mixing_type, double lam, double thresh, int maxit, double* accuracy);
 /* Chain_Dynamics_Timestep_Euler (main_blend.f90:605-667): densities and mean fields of the
  * field, mulocal = newfield - field, its gradient along each axis through nabla, the density
  * response to field - eps*gradient (eps = (double)0.0001f, the driver's single-precision

@@ -132,3 +132,41 @@ def test_epd_steps_match_oracle(orc):
             assert np.max(np.abs(den - d_ref)) <= 1e-12 * np.max(d_ref)
             assert conj <= orc.BLEND_CONJ_ACCURACY * (1 + 1e-9)
             assert abs(e["F"] - e_ref) <= 1e-10 * abs(e_ref)
+
+
+def test_blend_saddle_point_lambda_mixing_matches_oracle(orc):
+    """Main_Blend's own search (main_blend.f90:437-486): Mixing_Lambda with lammax 0.5 and the
+    accuracy sum|dfield| dvol.  Lambda mixing has no Gram solve, but its step (a ratio of norms
+    of differences of consecutive iterates) still amplifies rounding: on the oracle alone a
+    1e-16 change of the start field moves the accuracy after 100 iterations by 7e-10 relative
+    and after 150 by 1e-6.  The trajectory is compared after 100 iterations at 1e-7 (accuracy)
+    and 1e-9 (field); the run to the driver's 1e-7 by its iteration count, and by the oracle."""
+    import adscf_amd as amd
+
+    g = _grid(orc)
+    vol, dvol = g.volume()
+    rho = [25.0 / vol, 25.0 / vol]
+    bl = orc.Blend([1, 2], [10, 10], rho)
+    fh = orc.FH(KAPPA, [[0.0, CHI], [CHI, 0.0]], [1.0, 1.0])
+    w0 = orc.MT(2007).grnd(2 * g.ng).reshape(2, 1, g.ny, g.nx).copy()
+    ref = orc.blend_scf(g, bl, fh, w0, m=2, mixing_type=2, lam=0.5, thresh=-1.0, maxit=100)
+    conv = orc.blend_scf(g, bl, fh, w0, m=2, mixing_type=2, lam=0.5, maxit=20000)
+    with _ctx(amd, g, [1, 2], [10, 10], rho, m=2) as ctx:
+        ctx.scf_set_mixing(2)
+        ctx.scf_set_residual(kind=0, measure=1)
+        ctx.scf_set_field(w0)
+        st = ctx.scf_iterate(100, tol=-1.0)
+        field, phi = ctx.scf_get()
+        print(f"100 lambda iterations: accuracy GPU {st.accuracy:.12e} oracle {ref['accuracy']:.12e}")
+        assert st.iters == 100
+        assert abs(st.accuracy - ref["accuracy"]) <= 1e-7 * ref["accuracy"]
+        assert np.max(np.abs(field - ref["field"])) <= 1e-9 * np.max(np.abs(ref["field"]))
+        st = ctx.scf_iterate(20000, tol=orc.BLEND_SCF_ACCURACY)
+        field, _ = ctx.scf_get()
+    assert st.converged
+    print(f"converged: GPU {st.iters} iterations, oracle {conv['iters']}")
+    assert abs(st.iters - conv["iters"]) <= 5  # the end game amplifies rounding further
+    d = orc.blend_densities(g, bl, field)
+    resid = np.sum(np.abs(orc.mean_fields(g, fh, d) - field)) * dvol
+    print(f"lambda mixing converged after {st.iters} iterations; oracle residual {resid:.3e}")
+    assert resid <= 1.01 * orc.BLEND_SCF_ACCURACY
