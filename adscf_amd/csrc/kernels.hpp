@@ -385,6 +385,15 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
   const int stop = *(a.done ? a.done : &g_no_stop);
   const bool useEa = pre_E != 0 || nops == 2 || out_kind == OUT_DOT || (out_kind == OUT_PLANAR && post_E);
   const bool useEb = nops == 2;
+  // FP64 engines with 8 or more points per thread would hold 4 NP E factors (8 NP VGPRs)
+  // through the first operator and spill there (16x8 merged: 56 VGPRs spilled); they load
+  // the E factors that operator does not need after it instead (one more memory latency per
+  // pass, hidden by the other wavefronts at the grid sizes these shapes serve: 128^3 FP64
+  // 27.4 against 31.5 us per pass).  The FP32 128-point engines gain nothing from it, and
+  // the engines of the 64^3 sweep keep every load up front.
+  constexpr bool LEAN = sizeof(T) == 8 && NP >= 8;
+  const bool earlyEa = useEa && (!LEAN || pre_E != 0);
+  const bool earlyEb = useEb && !LEAN;
   C2<T> v[NP];
   T ea1[NP], ea2[NP], eb1[NP], eb2[NP];
 #pragma unroll
@@ -400,11 +409,11 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
         x.x = ldo(a.q1_in, fo(j2));
         x.y = a.q2_in ? ldo(a.q2_in, fo(j2)) : T(0);
       }
-      if (useEa) {
+      if (earlyEa) {
         ea1[j2] = ldo(a.Ea1, fo(j2));
         ea2[j2] = ldo(a.Ea2, fo(j2));
       }
-      if (useEb) {
+      if (earlyEb) {
         eb1[j2] = ldo(a.Eb1, fo(j2));
         eb2[j2] = ldo(a.Eb2, fo(j2));
       }
@@ -433,6 +442,19 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
   // The stop flag is tested only here, after the first operator: its barriers keep
   // every load above in flight together, and nothing has been stored yet.
   if (stop) return;  // uniform over the block
+  if (LEAN && act) {
+#pragma unroll
+    for (int j2 = 0; j2 < NP; ++j2) {
+      if (useEa && !earlyEa) {
+        ea1[j2] = ldo(a.Ea1, fo(j2));
+        ea2[j2] = ldo(a.Ea2, fo(j2));
+      }
+      if (useEb) {
+        eb1[j2] = ldo(a.Eb1, fo(j2));
+        eb2[j2] = ldo(a.Eb2, fo(j2));
+      }
+    }
+  }
   if (nops == 2) {
     if (act) {
 #pragma unroll
