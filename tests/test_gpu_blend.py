@@ -170,3 +170,29 @@ def test_blend_saddle_point_lambda_mixing_matches_oracle(orc):
     resid = np.sum(np.abs(orc.mean_fields(g, fh, d) - field)) * dvol
     print(f"lambda mixing converged after {st.iters} iterations; oracle residual {resid:.3e}")
     assert resid <= 1.01 * orc.BLEND_SCF_ACCURACY
+
+
+def test_epd_step_fp32_mixed_error(orc):
+    """FP32-mixed EPD step against FP64 from the same start.  The eps = 1e-4 finite difference
+    divides the FP32 rounding of the density response by eps, so the stated bound is 1e-4 of
+    max density at dt = 1e-3 (against 1e-5 for the DDFT step)."""
+    import adscf_amd as amd
+
+    g = _grid(orc)
+    vol, dvol = g.volume()
+    rho = [25.0 / vol, 25.0 / vol]
+    field = orc.MT(2007).grnd(2 * g.ng).reshape(2, 1, g.ny, g.nx).copy()
+    den = orc.blend_densities(g, orc.Blend([1, 2], [10, 10], rho), field)
+    out = {}
+    for prec in (amd.FP64, amd.FP32):
+        ctx = amd.Context(g.nx, g.ny, g.nz, g.lx, g.ly, g.lz, g.ds, precision=prec)
+        with ctx:
+            ctx.set_interactions(KAPPA, [[0.0, CHI], [CHI, 0.0]], [1.0, 1.0])
+            ctx.blend_setup([1, 2], [10, 10], rho, mixing_dim=2, lam_mix=0.5)
+            ctx.scf_set_field(field)
+            ctx.ddft_set_density(den)
+            ctx.epd_step(0.001)
+            out[prec] = ctx.ddft_get_density()
+    err = np.max(np.abs(out[amd.FP32] - out[amd.FP64])) / np.max(out[amd.FP64])
+    print(f"FP32-mixed EPD step: density error {err:.2e}")
+    assert err < 1e-4

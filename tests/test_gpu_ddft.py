@@ -261,3 +261,30 @@ def test_unchanged_multiblock_driver_ddft(tmp_path, orc):
     assert np.max(np.abs(gd - den)) <= 1e-12 * np.max(den)
     chain, _ = orc.species_densities(g, BT, bs, gf, ensemble=1, rho=1.0)
     assert np.max(np.abs(chain - gd)) <= 0.5e-3
+
+
+def test_ddft_step_fp32_mixed_error(orc):
+    """FP32-mixed DDFT step (float propagators and transforms; densities, fields, the mobility
+    operator's inputs and outputs and Anderson in double) against the FP64 step from the same
+    start: stated bound 1e-5 of max density for the density after one step."""
+    import adscf_amd as amd
+
+    nx, ny, nz, box, ds = 24, 1, 20, (4.8, 1.0, 4.0), 0.01
+    bs = _segments(orc, ds)
+    mob = _mobilities()
+    g = orc.Grid(nx, ny, nz, *box, ds)
+    field = orc.MT(2007).grnd(2 * nx * ny * nz).reshape(2, nz, ny, nx).copy()
+    den, _ = orc.species_densities(g, BT, bs, field, ensemble=1, rho=1.0)
+    out = {}
+    for prec in (amd.FP64, amd.FP32):
+        with amd.Context(nx, ny, nz, *box, ds, precision=prec) as ctx:
+            ctx.set_interactions(KAPPA, [[0.0, CHI], [CHI, 0.0]], [1.0, 1.0])
+            ctx.scf_setup(BT, bs, ensemble=1, rho=1.0, mixing_dim=10, lam_mix=0.5)
+            ctx.mobility_qq(mob.table, mob.dq, 10.0)
+            ctx.scf_set_field(field)
+            ctx.ddft_set_density(den)
+            s = ctx.ddft_step(5e-5, qdmin=True)
+            out[prec] = (ctx.ddft_get_density(), s)
+    err = np.max(np.abs(out[amd.FP32][0] - out[amd.FP64][0])) / np.max(out[amd.FP64][0])
+    print(f"FP32-mixed DDFT step: density error {err:.2e}; solves {out[amd.FP32][1]} vs {out[amd.FP64][1]}")
+    assert err < 1e-5
