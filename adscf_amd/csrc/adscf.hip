@@ -1750,8 +1750,9 @@ struct BatchTables {
     size_t arg_off, blk_off;
     int blocks, N1, N2;
     size_t smem;
+    int lane;
   };
-  std::vector<Launch> launches;  // pass-major, one per shape present in the pass
+  std::vector<Launch> launches;  // pass-major, one per lane and shape present in the pass
 };
 
 // instance slots are packed above kBatchItemBits in a signed 32-bit block code
@@ -1784,8 +1785,18 @@ void batch_fix_schedule(adscf_ctx* c) {
 
 }  // namespace
 
+// Independent lanes of a batched iteration: the captured graph forks into this many branches
+// (the batch stream and kBatchLanes - 1 side streams, default priority), each carrying the
+// batched passes and the post-sweep kernels (contour, Gram, solve, mix, schedule) of its own
+// instances, and joins at the end of the iteration.  4 matches the hardware queues a process
+// gets by default.
+constexpr int kBatchLanes = 4;
+
 struct adscf_batch {
   hipStream_t stream = nullptr;
+  std::vector<int> lane_of;  // lane of each instance (batch_build_tables)
+  hipStream_t side[kBatchLanes - 1] = {};
+  hipEvent_t fork = nullptr, join[kBatchLanes - 1] = {};
   int prec = ADSCF_FP32;
   std::vector<adscf_ctx*> inst;
   BatchTables<double> t64;
@@ -1832,29 +1843,66 @@ void batch_build_tables(adscf_batch* b) {
     npass = std::max(npass, es.size());
     per.push_back(std::move(es));
   }
+  // Lanes: instances whose passes have the same register shapes form a group (on cubic
+  // grids, one per axis length); groups go to lanes, largest work first onto the least loaded
+  // lane.  A lane's sweep and post-sweep kernels depend on nothing in another lane, so the
+  // captured iteration runs the lanes as parallel graph branches.
+  std::vector<std::vector<std::pair<int, int>>> sigs;
+  std::vector<int> group_of(per.size());
+  std::vector<double> gcost;
+  for (size_t i = 0; i < per.size(); ++i) {
+    std::vector<std::pair<int, int>> sg;
+    for (const Entry& e : per[i]) sg.push_back({e.N1, e.N2});
+    auto it = std::find(sigs.begin(), sigs.end(), sg);
+    int g = (int)(it - sigs.begin());
+    if (it == sigs.end()) {
+      sigs.push_back(sg);
+      gcost.push_back(0.0);
+    }
+    group_of[i] = g;
+    gcost[g] += (double)b->inst[i]->ng * (double)per[i].size();
+  }
+  std::vector<int> order(gcost.size());
+  for (size_t g = 0; g < order.size(); ++g) order[g] = (int)g;
+  std::sort(order.begin(), order.end(), [&](int x, int y) { return gcost[x] > gcost[y]; });
+  std::vector<int> lane_of_group(gcost.size());
+  double load[kBatchLanes] = {};
+  for (int g : order) {
+    int best = 0;
+    for (int k = 1; k < kBatchLanes; ++k)
+      if (load[k] < load[best]) best = k;
+    lane_of_group[g] = best;
+    load[best] += gcost[g];
+  }
+  b->lane_of.resize(per.size());
+  for (size_t i = 0; i < per.size(); ++i) b->lane_of[i] = lane_of_group[group_of[i]];
+
   std::vector<PassArgs<T>> args;
   std::vector<int> blk;
   tb.launches.clear();
   for (size_t p = 0; p < npass; ++p) {
-    std::vector<std::pair<int, int>> shapes;  // distinct shapes of this pass, first-seen order
-    for (size_t i = 0; i < per.size(); ++i) {
-      if (p >= per[i].size()) continue;
-      const std::pair<int, int> sh{per[i][p].N1, per[i][p].N2};
-      if (std::find(shapes.begin(), shapes.end(), sh) == shapes.end()) shapes.push_back(sh);
-    }
-    for (const auto& sh : shapes) {
-      typename BatchTables<T>::Launch L{args.size(), blk.size(), 0, sh.first, sh.second, 0};
-      int slot = 0;
+    for (int lane = 0; lane < kBatchLanes; ++lane) {
+      std::vector<std::pair<int, int>> shapes;  // distinct shapes of this pass in the lane
       for (size_t i = 0; i < per.size(); ++i) {
-        if (p >= per[i].size() || per[i][p].N1 != sh.first || per[i][p].N2 != sh.second) continue;
-        const Entry& e = per[i][p];
-        args.push_back(e.args);
-        for (int it = 0; it < e.items; ++it) blk.push_back((slot << kBatchItemBits) | it);
-        L.blocks += e.items;
-        L.smem = std::max(L.smem, e.smem);
-        ++slot;
+        if (p >= per[i].size() || b->lane_of[i] != lane) continue;
+        const std::pair<int, int> sh{per[i][p].N1, per[i][p].N2};
+        if (std::find(shapes.begin(), shapes.end(), sh) == shapes.end()) shapes.push_back(sh);
       }
-      tb.launches.push_back(L);
+      for (const auto& sh : shapes) {
+        typename BatchTables<T>::Launch L{args.size(), blk.size(), 0, sh.first, sh.second, 0, lane};
+        int slot = 0;
+        for (size_t i = 0; i < per.size(); ++i) {
+          if (p >= per[i].size() || b->lane_of[i] != lane || per[i][p].N1 != sh.first || per[i][p].N2 != sh.second)
+            continue;
+          const Entry& e = per[i][p];
+          args.push_back(e.args);
+          for (int it = 0; it < e.items; ++it) blk.push_back((slot << kBatchItemBits) | it);
+          L.blocks += e.items;
+          L.smem = std::max(L.smem, e.smem);
+          ++slot;
+        }
+        tb.launches.push_back(L);
+      }
     }
   }
   tb.args.ensure(args.size());
@@ -1879,12 +1927,38 @@ void launch_batch_group(hipStream_t s, const typename BatchTables<T>::Launch& L,
   throw ArgError{"no batched kernel for this shape"};
 }
 
+hipStream_t lane_stream(adscf_batch* b, int lane) { return lane == 0 ? b->stream : b->side[lane - 1]; }
+
+// One batched iteration (post = true) or sweep (post = false) as kBatchLanes branches forked
+// from and joined back into the batch stream (for capture into a graph).
 template <typename T>
-void batch_enqueue_sweep(adscf_batch* b) {
+void batch_enqueue_lanes(adscf_batch* b, bool post) {
   BatchTables<T>& tb = btab<T>(b);
-  for (const auto& L : tb.launches)
-    launch_batch_group<T>(b->stream, L, BatchGroup<T>{tb.args.p + L.arg_off, tb.blk.p + L.blk_off});
-  HIPCK(hipGetLastError());
+  HIPCK(hipEventRecord(b->fork, b->stream));
+  for (hipStream_t s : b->side) HIPCK(hipStreamWaitEvent(s, b->fork, 0));
+  for (int lane = 0; lane < kBatchLanes; ++lane) {
+    hipStream_t s = lane_stream(b, lane);
+    for (const auto& L : tb.launches)
+      if (L.lane == lane) launch_batch_group<T>(s, L, BatchGroup<T>{tb.args.p + L.arg_off, tb.blk.p + L.blk_off});
+    HIPCK(hipGetLastError());
+    if (!post) continue;
+    for (size_t i = 0; i < b->inst.size(); ++i) {
+      if (b->lane_of[i] != lane) continue;
+      adscf_ctx* c = b->inst[i];
+      c->stream = s;
+      try {
+        enqueue_iteration<T>(c, false);
+      } catch (...) {
+        c->stream = b->stream;
+        throw;
+      }
+      c->stream = b->stream;
+    }
+  }
+  for (int k = 0; k < kBatchLanes - 1; ++k) {
+    HIPCK(hipEventRecord(b->join[k], b->side[k]));
+    HIPCK(hipStreamWaitEvent(b->stream, b->join[k], 0));
+  }
 }
 
 void batch_drop_graph(adscf_batch* b) {
@@ -1900,8 +1974,7 @@ void batch_build_graph(adscf_batch* b) {
   batch_build_tables<T>(b);
   HIPCK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   try {
-    batch_enqueue_sweep<T>(b);
-    for (adscf_ctx* c : b->inst) enqueue_iteration<T>(c, false);
+    batch_enqueue_lanes<T>(b, true);
   } catch (...) {
     hipGraph_t g = nullptr;
     (void)hipStreamEndCapture(b->stream, &g);
@@ -1918,7 +1991,13 @@ double batch_sweep_timing(adscf_batch* b, int reps) {
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
   HIPCK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  batch_enqueue_sweep<T>(b);
+  try {
+    batch_enqueue_lanes<T>(b, false);
+  } catch (...) {
+    (void)hipStreamEndCapture(b->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
   HIPCK(hipStreamEndCapture(b->stream, &g));
   HIPCK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
   HIPCK(hipGraphLaunch(ge, b->stream));
@@ -2911,6 +2990,9 @@ int adscf_batch_create(adscf_batch** out, int ninst, const int* grid, const doub
   b->prec = precision;
   int rc = bguard(b, [&]() -> int {
     HIPCK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    for (hipStream_t& s : b->side) HIPCK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIPCK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    for (hipEvent_t& e : b->join) HIPCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCK(hipEventCreate(&b->ev0));
     HIPCK(hipEventCreate(&b->ev1));
     int boff = 0;
@@ -2950,6 +3032,11 @@ int adscf_batch_destroy(adscf_batch* b) {
   }
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  for (hipEvent_t e : b->join)
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t s : b->side)
+    if (s) (void)hipStreamDestroy(s);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return ADSCF_OK;
