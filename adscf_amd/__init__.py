@@ -82,6 +82,7 @@ SIGNATURES = {
     "adscf_scf_timing": (C.c_int, [_ctxp, _dp, _dp]),
     "adscf_scf_set_residual": (C.c_int, [_ctxp, C.c_int, C.c_int, _dp]),
     "adscf_scf_set_mixing": (C.c_int, [_ctxp, C.c_int]),
+    "adscf_scf_set_density": (C.c_int, [_ctxp, _dp]),
     "adscf_conjugate_fields": (C.c_int, [_ctxp, _dp, C.c_double, C.c_int, C.c_int, _ip, _dp]),
     "adscf_mobility_qq": (C.c_int, [_ctxp, C.c_int, C.c_double, _dp, C.c_double, _dp]),
     "adscf_ddft_set_mobility": (C.c_int, [_ctxp, _dp]),
@@ -398,9 +399,14 @@ class Context:
 
     def scf_set_residual(self, kind=0, measure=0, target=None):
         """Residual of scf_iterate: kind 0 mean fields - field, kind 1 densities - target;
-        measure 0 max|dfield|, 1 sum|dfield| dvol."""
+        measure 0 max|dfield|, 1 sum|dfield| dvol, 2 sum of squared density changes."""
         t = None if target is None else _f64(target, (self.ntypes,) + self.shape)
         self._chk(self._lib.adscf_scf_set_residual(self._h, int(kind), int(measure), None if t is None else _p(t)))
+
+    def scf_set_density(self, density):
+        """Densities the next evaluation's measure 2 (sum of squared changes) compares with."""
+        d = _f64(density, (self.ntypes,) + self.shape)
+        self._chk(self._lib.adscf_scf_set_density(self._h, _p(d)))
 
     def scf_set_mixing(self, mixing_type=3):
         """Mixing of scf_iterate: 1 simple, 2 lambda (lammax = lam_mix), 3 Anderson."""

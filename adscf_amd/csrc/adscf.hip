@@ -2740,7 +2740,8 @@ int adscf_epd_step(adscf_ctx* c, double timestep, double accuracy_scf, int iterm
 int adscf_scf_set_residual(adscf_ctx* c, int kind, int measure, const double* target) {
   return guarded(c, [&]() -> int {
     if (kind != 0 && kind != 1) throw ArgError{"residual kind must be 0 (mean field) or 1 (density)"};
-    if (measure != 0 && measure != 1) throw ArgError{"measure must be 0 (max) or 1 (sum * dvol)"};
+    if (measure < 0 || measure > 2)
+      throw ArgError{"measure must be 0 (max), 1 (sum * dvol) or 2 (sum of squared density changes)"};
     const long nvar = c->ng * c->nt;
     if (target) {
       c->target.ensure(nvar);
@@ -2750,6 +2751,19 @@ int adscf_scf_set_residual(adscf_ctx* c, int kind, int measure, const double* ta
     if (kind == 1 && !c->target_set) throw ArgError{"a density residual needs target densities"};
     c->user_kind = kind;
     c->user_measure = measure;
+    return ADSCF_OK;
+  });
+}
+
+int adscf_scf_set_density(adscf_ctx* c, const double* density) {
+  return guarded(c, [&]() -> int {
+    if (!density) throw ArgError{"null array"};
+    if (!c->field_set) {
+      c->err = "adscf_scf_set_field has not been called";
+      return ADSCF_ERR_STATE;
+    }
+    HIPCK(hipMemcpy(c->phi.p, density, c->ng * c->nt * sizeof(double), hipMemcpyHostToDevice));
+    c->phi_fresh = false;
     return ADSCF_OK;
   });
 }

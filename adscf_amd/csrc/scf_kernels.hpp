@@ -227,7 +227,7 @@ struct ContourArgs {
   double* dring;      // CONTOUR_SCF: residual history ring (m+1, nt*ng) or null
   int m;
   const double* target;  // CONTOUR_SCF: null -> dfield = mean field - field; else dfield = phi - target
-  int measure;        // 0: per-block max|dfield|; 1: per-block sum|dfield|
+  int measure;        // 0: per-block max|dfield|; 1: sum|dfield|; 2: sum (phi_old - phi)^2
   const ScfState* rst;  // CONTOUR_SCF: state holding the ring head (null: st)
   int accumulate;     // CONTOUR_SCF: add this chain's densities to phi (blends: one chain per polymer)
   int no_resid;       // CONTOUR_SCF: densities only (a blend's chains before the last)
@@ -335,12 +335,18 @@ __global__ void __launch_bounds__(kThreads) k_contour(ContourArgs<T> a) {
 #pragma unroll
       for (int v = 0; v < VP; ++v) {
         const long i = i0 + v;
+        if (a.measure == 2) {  // sum((density - newdensity)**2) against phi's previous contents
+          for (int t = 0; t < nt; ++t) {
+            const double dd = a.phi[(long)t * ng + i] - phi[t][v];
+            rmax = rmax + dd * dd;
+          }
+        }
         for (int t = 0; t < nt; ++t) a.phi[(long)t * ng + i] = phi[t][v];
         if (a.target) {
           for (int t = 0; t < nt; ++t) {
             const double df = phi[t][v] - a.target[(long)t * ng + i];
             if (d0) d0[(long)t * ng + i] = df;
-            rmax = a.measure ? rmax + fabs(df) : nanmax(rmax, fabs(df));
+            if (a.measure != 2) rmax = a.measure ? rmax + fabs(df) : nanmax(rmax, fabs(df));
           }
           continue;
         }
@@ -353,7 +359,7 @@ __global__ void __launch_bounds__(kThreads) k_contour(ContourArgs<T> a) {
           f = f * a.fh.vol[t];
           const double df = f - w0[(long)t * ng + i];
           if (d0) d0[(long)t * ng + i] = df;
-          rmax = a.measure ? rmax + fabs(df) : nanmax(rmax, fabs(df));
+          if (a.measure != 2) rmax = a.measure ? rmax + fabs(df) : nanmax(rmax, fabs(df));
         }
       }
     }
@@ -627,7 +633,7 @@ struct SolveArgs {
   int ns, ensemble;
   int force_anderson;  // compat: caller decides via iter
   int iter_override;   // compat: iteration number passed by the caller
-  int measure;         // 0: accuracy = max|dfield|; 1: accuracy = sum|dfield| * dvol
+  int measure;         // 0: accuracy = max|dfield|; 1: sum|dfield| * dvol; 2: sum (phi_old - phi)^2
   int mixing;          // 3 Anderson (default 0 reads as 3), 1 simple, 2 lambda
   const double* gsum_w;  // lambda mixing: Gram sums of the field ring
   ScfState* st;
@@ -651,9 +657,11 @@ __global__ void k_solve(SolveArgs a) {
   } else {
     double r = 0.0;
     if (a.measure) {
-      // block partials summed in a fixed order, then times dvol (main_blend.f90:567)
+      // block partials summed in a fixed order, then times dvol (main_blend.f90:567); or,
+      // measure 2, the sum of squared density changes (main_diblock.f90:387)
       for (int b = tid; b < a.nblk_contour; b += blockDim.x) r += a.rpart[b];
-      r = block_sum(r, sh) * a.dvol;
+      r = block_sum(r, sh);
+      if (a.measure == 1) r = r * a.dvol;
     } else {
       for (int b = tid; b < a.nblk_contour; b += blockDim.x) r = nanmax(r, a.rpart[b]);
       r = block_max(r, sh);

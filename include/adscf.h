@@ -161,8 +161,13 @@ int adscf_scf_timing(adscf_ctx* ctx, double* ms_per_iter, double* ms_sweep);
  * (Find_SCF_Saddle_Point, main_multiblock.f90:673-676); kind 1: dfield = densities - target
  * (the loop of Get_Conjugate_Fields, main_multiblock.f90:602-604), target (nx,ny,nz,ntypes)
  * uploaded when non-NULL.  measure 0: accuracy = maxval(abs(dfield)) (main_multiblock.f90:780);
- * 1: sum(abs(dfield))*dvol (Main_Blend/main_blend.f90:567).  Default: kind 0, measure 0. */
+ * 1: sum(abs(dfield))*dvol (Main_Blend/main_blend.f90:567); 2: sum((density-newdensity)**2),
+ * the change of the densities since the previous evaluation (Main_Diblock_Melt/main_diblock.f90:387),
+ * the first iteration comparing with adscf_scf_set_density.  Default: kind 0, measure 0. */
 int adscf_scf_set_residual(adscf_ctx* ctx, int kind, int measure, const double* target);
+/* the densities (nx,ny,nz,ntypes) the next evaluation's measure 2 compares with (the driver's
+ * density before Calculate_Saddle_Point, main_diblock.f90:398-412); after adscf_scf_set_field */
+int adscf_scf_set_density(adscf_ctx* ctx, const double* density);
 
 /* Mixing of the loop adscf_scf_iterate runs (Mix_Fields' mixing_type, main_multiblock.f90:738-747):
  * 1 Mixing_Simple (General_Routines/mod_iterate.f90:27-38, step lam_mix), 2 Mixing_Lambda

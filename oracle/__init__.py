@@ -117,6 +117,8 @@ def _load():
                                    C.c_double, C.c_int, _ip]),
         "orc_blend_scf": (C.c_int, [G, C.POINTER(_Blend), C.POINTER(_FH), C.c_int, _dp, _dp, C.c_int, C.c_int,
                                     C.c_double, C.c_double, C.c_int, _dp]),
+        "orc_diblock_saddle": (C.c_int, [G, C.POINTER(_Chain), C.POINTER(_FH), C.c_int, _dp, _dp, C.c_int, C.c_int,
+                                         C.c_double, C.c_double, C.c_int, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -532,3 +534,21 @@ def blend_scf(g: Grid, blend: Blend, fh: FH, field, m=2, mixing_type=2, lam=0.5,
     it = lib().orc_blend_scf(C.byref(g._g), C.byref(blend._b), C.byref(fh._fh), f.shape[0], _p(f), _p(d), int(m),
                              int(mixing_type), float(lam), float(thresh), int(maxit), C.byref(acc))
     return dict(field=f, density=d, iters=it, accuracy=acc.value)
+
+
+def diblock_saddle(g: Grid, block_types, block_segs, fh: FH, field, density, m=2, mixing_type=2, lam=0.1,
+                   tol=1e-12, maxit=100000, ensemble=2, rho=1.0, mu=0.0):
+    """Calculate_Saddle_Point of Main_Diblock_Melt (main_diblock.f90:398-451).  Returns
+    dict(field (the last newfield), mixed_field, density (the last newdensity), iters, accuracy,
+    mu, rho, energy)."""
+    bt = np.ascontiguousarray(block_types, dtype=np.int32)
+    bs = np.ascontiguousarray(block_segs, dtype=np.int32)
+    f = np.array(field, dtype=np.float64, order="C")
+    d = np.array(density, dtype=np.float64, order="C")
+    mf = np.empty_like(f)
+    acc = C.c_double()
+    ch = _Chain(len(bt), _pi(bt), _pi(bs), int(ensemble), float(rho), float(mu), 0.0, 0.0)
+    it = lib().orc_diblock_saddle(C.byref(g._g), C.byref(ch), C.byref(fh._fh), f.shape[0], _p(f), _p(d), int(m),
+                                  int(mixing_type), float(lam), float(tol), int(maxit), C.byref(acc), _p(mf))
+    return dict(field=f, mixed_field=mf, density=d, iters=it, accuracy=acc.value, mu=ch.mu, rho=ch.rho,
+                energy=ch.energy)

@@ -1096,3 +1096,50 @@ int orc_epd_step(const orc_grid* g, orc_blend* b, const orc_fh* fh, int ntypes, 
   free(nabla);
   return rc;
 }
+
+/* ======================================================================
+ * Main_Diblock_Melt/main_diblock.f90:398-451
+ * ====================================================================== */
+int orc_diblock_saddle(const orc_grid* g, orc_chain* c, const orc_fh* fh, int ntypes, double* field, double* density,
+                       int m, int mixing_type, double lam, double tol, int maxit, double* accuracy,
+                       double* mixed_field) {
+  long ng = (long)g->nx * g->ny * g->nz;
+  long nvar = ng * ntypes;
+  double* var = (double*)calloc((size_t)nvar * (size_t)(m + 1), sizeof(double));
+  double* dvar = (double*)calloc((size_t)nvar * (size_t)(m + 1), sizeof(double));
+  double* newdensity = (double*)malloc(sizeof(double) * (size_t)nvar);
+  double* newfield = (double*)calloc((size_t)nvar, sizeof(double));
+  double acc = 0.0;
+  int iter;
+  for (iter = 1; iter <= maxit; ++iter) {
+    if (orc_species_densities(g, c, ntypes, field, newdensity) != 0) { iter = -1; break; } /* :412 */
+    acc = 0.0;                                                                             /* :414, :387 */
+    for (long i = 0; i < nvar; ++i) {
+      double d = density[i] - newdensity[i];
+      acc += d * d;
+    }
+    if (!(acc > tol)) break;                                                               /* :421 */
+    orc_mean_fields(g, fh, newdensity, newfield);                                         /* :425 */
+    for (long i = 0; i < nvar; ++i) dvar[i] = newfield[i] - field[i];                      /* :426 */
+    memcpy(var, field, sizeof(double) * (size_t)nvar);                                     /* :427 */
+    if (mixing_type == 1)
+      orc_mixing_simple(var, dvar, nvar, m, lam);
+    else if (mixing_type == 2)
+      orc_mixing_lambda(var, dvar, nvar, m, lam, iter);
+    else
+      orc_mixing_anderson(var, dvar, nvar, m, lam, iter);
+    memcpy(field, var, sizeof(double) * (size_t)nvar);
+    memcpy(density, newdensity, sizeof(double) * (size_t)nvar);                            /* :428 */
+  }
+  *accuracy = acc;
+  if (mixed_field) memcpy(mixed_field, field, sizeof(double) * (size_t)nvar);
+  if (iter > 0) {
+    memcpy(density, newdensity, sizeof(double) * (size_t)nvar);                            /* :447 */
+    memcpy(field, newfield, sizeof(double) * (size_t)nvar);                                /* :448 */
+  }
+  free(newfield);
+  free(newdensity);
+  free(dvar);
+  free(var);
+  return iter;
+}

@@ -121,6 +121,17 @@ int orc_scf_anderson(const orc_grid* g, orc_chain* c, const orc_fh* fh, int ntyp
                      double* density, int m, double lam, double tol, int maxit, double* accuracy,
                      double* residual_history);
 
+/* Calculate_Saddle_Point of Main_Diblock_Melt (main_diblock.f90:398-451) with Mix_Fields
+ * (:315-372) and Check_Accuracy (:380-390): per iteration the densities of the field, accuracy
+ * sum((density-newdensity)**2) against the previous iteration's densities (density on entry
+ * for the first), exit when .not.(accuracy > tol), else the mean fields, dfield = newfield -
+ * field, mixing_type 1/2/3 with lam (Mixing_Lambda's lammax) and iter_status = iter, density =
+ * newdensity.  On return density holds the last newdensity and field the last newfield
+ * (:447-448), the mixed field in mixed_field (may be NULL).  Returns the loop index at exit. */
+int orc_diblock_saddle(const orc_grid* g, orc_chain* c, const orc_fh* fh, int ntypes, double* field, double* density,
+                       int m, int mixing_type, double lam, double tol, int maxit, double* accuracy,
+                       double* mixed_field);
+
 /* ---- DDFT (Main_Multiblock/main_multiblock.f90) ------------------------ */
 /* Read_Mobilities, table part (main_multiblock.f90:486-504): for every half-spectrum
  * point, |q| = sqrt(-laplace), iq = int(|q|/dq), the reference's interpolation of

@@ -260,3 +260,47 @@ def test_autotune_keeps_parity(amd, orc):
         got = ctx.propagate_chain([1, 2], [25, 25], E)
     ref = orc.propagate_chain(g, [1, 2], [25, 25], E)
     assert _rel(got.density, ref.density) < FP64_RTOL
+
+
+def test_diblock_saddle_loop_matches_oracle(orc):
+    """Main_Diblock_Melt's own loop (main_diblock.f90:398-451) on the device: lambda mixing
+    (lammax 0.1), the accuracy sum((density - newdensity)**2) against the previous iteration's
+    densities (measure 2; the first iteration against the driver's guessed density), grand
+    canonical f_A = 0.7 from the zero field.  Lambda mixing amplifies rounding slowly (see
+    tests/test_gpu_blend.py), so the trajectory is compared after 40 iterations; the run to
+    convergence by the oracle's evaluation of its final state."""
+    import adscf_amd as amd
+
+    nx, ny, nz, box, ds = 32, 1, 40, (3.6, 1.0, 4.5), 0.01
+    bs = [orc.block_segments(0.7, ds), orc.block_segments(0.3, ds)]
+    g = orc.Grid(nx, ny, nz, *box, ds)
+    fh = orc.FH(25.0, [[0.0, 20.0], [20.0, 0.0]], [1.0, 1.0])
+    mu, rho = 2.5, 1.0
+    x = np.arange(1, nx + 1)
+    den0 = np.empty((2, nz, ny, nx))
+    den0[0] = (rho * 0.7 * (1.0 + 0.5 * np.cos(x / nx * np.pi)))[None, None, :]  # main_diblock.f90:128-129
+    den0[1] = (rho * 0.3 * (1.0 - 0.5 * np.cos(x / nx * np.pi)))[None, None, :]
+    w0 = np.zeros_like(den0)
+    ref = orc.diblock_saddle(g, [1, 2], bs, fh, w0, den0, m=2, mixing_type=2, lam=0.1, tol=-1.0, maxit=40,
+                             ensemble=2, rho=rho, mu=mu)
+    with amd.Context(nx, ny, nz, *box, ds, precision=amd.FP64) as ctx:
+        ctx.set_interactions(25.0, [[0.0, 20.0], [20.0, 0.0]], [1.0, 1.0])
+        ctx.scf_setup([1, 2], bs, ensemble=2, rho=rho, mu=mu, mixing_dim=2, lam_mix=0.1)
+        ctx.scf_set_mixing(2)
+        ctx.scf_set_residual(kind=0, measure=2)
+        ctx.scf_set_field(w0)
+        ctx.scf_set_density(den0)
+        st = ctx.scf_iterate(40, tol=-1.0)
+        field, phi = ctx.scf_get()
+        print(f"40 iterations: accuracy GPU {st.accuracy:.12e} oracle {ref['accuracy']:.12e}")
+        assert st.iters == 40
+        assert abs(st.accuracy - ref["accuracy"]) <= 1e-8 * ref["accuracy"]
+        assert np.max(np.abs(field - ref["mixed_field"])) <= 1e-9 * np.max(np.abs(ref["mixed_field"]))
+        st = ctx.scf_iterate(20000, tol=1e-12)
+        field, phi = ctx.scf_get()
+    assert st.converged
+    # at exit the field is the one whose densities converged (the loop exits before mixing)
+    d, _ = orc.species_densities(g, [1, 2], bs, field, ensemble=2, rho=rho, mu=mu)
+    print(f"converged after {st.iters} iterations, accuracy {st.accuracy:.3e}")
+    assert st.accuracy <= 1e-12
+    assert np.max(np.abs(d - phi)) <= 1e-12 * np.max(d)
