@@ -96,6 +96,8 @@ SIGNATURES = {
     "adscf_autotune": (C.c_int, [_ctxp, C.c_int, C.c_char_p, C.c_int]),
     "adscf_get_schedule": (C.c_int, [_ctxp, C.c_int, _ip, _ip, _ip, _ip]),
     "adscf_set_schedule": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "adscf_set_middle_axis": (C.c_int, [_ctxp, C.c_int]),
+    "adscf_get_middle_axis": (C.c_int, [_ctxp, _ip]),
     "adscf_batch_create": (C.c_int, [C.POINTER(_ctxp), C.c_int, _ip, _dp, _dp, _ip, _ip, _ip, _dp, _dp, _dp,
                                      C.c_int, C.c_double, C.c_int, C.c_double, C.c_int]),
     "adscf_batch_destroy": (C.c_int, [_ctxp]),
@@ -502,6 +504,15 @@ class Context:
         n, a, b, w = (C.c_int() for _ in range(4))
         self._chk(self._lib.adscf_get_schedule(self._h, int(axis), C.byref(n), C.byref(a), C.byref(b), C.byref(w)))
         return dict(n=n.value, N1=a.value, N2=b.value, waves=w.value)
+
+    def set_middle_axis(self, axis=-1):
+        """Axis with one line pass per contour step (-1: the default, y)."""
+        self._chk(self._lib.adscf_set_middle_axis(self._h, int(axis)))
+
+    def middle_axis(self):
+        a = C.c_int()
+        self._chk(self._lib.adscf_get_middle_axis(self._h, C.byref(a)))
+        return a.value
 
     def set_schedule(self, axis, n1, n2, waves=1):
         self._chk(self._lib.adscf_set_schedule(self._h, int(axis), int(n1), int(n2), int(waves)))

@@ -129,6 +129,7 @@ struct adscf_ctx {
   int resid_kind = 0, measure = 0, graph_mode = -1;
   int user_kind = 0, user_measure = 0;  // the loop adscf_scf_iterate runs (adscf_scf_set_residual)
   int user_mixing = 3, mixing = 3;       // 1 simple, 2 lambda, 3 Anderson (adscf_scf_set_mixing)
+  int mid_axis = -1;  // axis of the single-operator passes of a 3-D sweep (-1: y); the others merge
   DevBuf<double> gsum_w;                 // lambda mixing: Gram sums of the field ring
   DevBuf<double> target;             // target densities (nt, ng): the DDFT density
   DevBuf<double> mob;                // mobility_qq (nk, nt, nt)
@@ -455,7 +456,16 @@ std::vector<PassArgs<T>> build_sweep(adscf_ctx* c, const ChainDesc& cd, const T*
                                      const int* dexp2, const int* done) {
   Engine<T>& En = eng<T>(c);
   const long ng = c->ng;
-  const std::vector<int> A = active_axes(c);
+  // Steps alternate the axis order (a, b, c then c, b, a), so the operators of the outer axes
+  // of consecutive steps meet and share a pass; the middle axis gets one pass per step.  The
+  // line operators of different axes commute, so any order gives the same propagator.
+  std::vector<int> A = active_axes(c);
+  if (A.size() == 3 && c->mid_axis >= 0 && c->mid_axis != A[1]) {
+    std::vector<int> o;
+    for (int a : A)
+      if (a != c->mid_axis) o.push_back(a);
+    A = {o[0], c->mid_axis, o[1]};
+  }
   const int ns = cd.ns;
   auto Ep = [&](int chain, int s) { return E + (long)step_type(cd, chain, s) * ng; };
   std::vector<PassArgs<T>> passes;
@@ -1311,6 +1321,23 @@ std::string autotune_impl(adscf_ctx* c, int reps) {
     ax = best;
     snprintf(line, sizeof line, "axis %d chosen: %s waves=%d (%.4f ms)\n", a, shape_name(best.N1, best.N2).c_str(),
              best.waves, best_ms);
+    rep += line;
+  }
+  if (active_axes(c).size() == 3) {  // which axis takes the single-operator passes
+    int best_mid = c->mid_axis;
+    double best_ms = 1e300;
+    for (int mid = 0; mid < 3; ++mid) {
+      c->mid_axis = mid;
+      const double ms = sweep_timing<T>(c, reps);
+      snprintf(line, sizeof line, "middle axis %d sweep=%.4f ms\n", mid, ms);
+      rep += line;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_mid = mid;
+      }
+    }
+    c->mid_axis = best_mid;
+    snprintf(line, sizeof line, "middle axis chosen: %d (%.4f ms)\n", best_mid, best_ms);
     rep += line;
   }
   if (c->graph) {
@@ -2967,6 +2994,21 @@ int adscf_autotune(adscf_ctx* c, int reps, char* report, int report_cap) {
     }
     return ADSCF_OK;
   });
+}
+
+int adscf_set_middle_axis(adscf_ctx* c, int axis) {
+  return guarded(c, [&]() -> int {
+    if (axis < -1 || axis > 2) throw ArgError{"axis must be -1, 0, 1 or 2"};
+    c->mid_axis = axis;
+    invalidate_tables(c);
+    return ADSCF_OK;
+  });
+}
+
+int adscf_get_middle_axis(const adscf_ctx* c, int* axis) {
+  if (!c || !axis) return ADSCF_ERR_ARG;
+  *axis = c->mid_axis < 0 ? 1 : c->mid_axis;
+  return ADSCF_OK;
 }
 
 int adscf_get_schedule(const adscf_ctx* c, int axis, int* n, int* n1, int* n2, int* waves) {

@@ -239,6 +239,11 @@ int adscf_autotune(adscf_ctx* ctx, int reps, char* report, int report_cap);
 /* current schedule of an axis (0 x, 1 y, 2 z): n1 x n2 four-step register engine, n1 = 0 the
  * Stockham kernel, n2 = 0 the n1^3 cube engine (n = n1^3 points, n1^2 threads per line) */
 int adscf_get_schedule(const adscf_ctx* ctx, int axis, int* n, int* n1, int* n2, int* waves);
+/* The axis whose operator gets one line pass per contour step in a 3-D sweep; the operators
+ * of the other two meet across consecutive steps (the order alternates a,b,c / c,b,a) and
+ * share merged passes.  -1 restores the default (y).  adscf_autotune also chooses it. */
+int adscf_set_middle_axis(adscf_ctx* ctx, int axis);
+int adscf_get_middle_axis(const adscf_ctx* ctx, int* axis);
 int adscf_set_schedule(adscf_ctx* ctx, int axis, int n1, int n2, int waves);
 
 /* ---- batched independent instances (no counterpart in the reference) ---------- */
