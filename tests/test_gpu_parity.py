@@ -255,11 +255,18 @@ def test_autotune_keeps_parity(amd, orc):
         ctx.scf_set_field(w)
         rep = ctx.autotune(reps=2)
         print(rep)
-        assert "chosen" in rep
+        assert "chosen" in rep and "middle axis chosen" in rep
         E = orc.expfield(g, w)
         got = ctx.propagate_chain([1, 2], [25, 25], E)
+        # every choice of the single-pass axis gives the same propagator (the axis operators commute)
+        mids = []
+        for mid in (0, 1, 2):
+            ctx.set_middle_axis(mid)
+            mids.append(ctx.propagate_chain([1, 2], [25, 25], E))
     ref = orc.propagate_chain(g, [1, 2], [25, 25], E)
     assert _rel(got.density, ref.density) < FP64_RTOL
+    for r in mids:
+        assert _rel(r.density, ref.density) < FP64_RTOL and abs(r.Qsum - ref.Qsum) < FP64_RTOL * ref.Qsum
 
 
 def test_diblock_saddle_loop_matches_oracle(orc):
