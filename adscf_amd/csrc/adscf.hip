@@ -968,7 +968,7 @@ void enqueue_iteration(adscf_ctx* c, bool passes_too = true) {
   s.gsum_w = c->gsum_w.p;
   s.st = c->st.p;
   hipLaunchKernelGGL(k_solve, dim3(1), dim3(kThreads), 0, c->stream, s);
-  hipLaunchKernelGGL(k_mix<T>, dim3(nb), dim3(kThreads), 0, c->stream, c->wring.p, (const double*)c->dring.p, ng,
+  hipLaunchKernelGGL(k_mix<T>, dim3(nb, c->nt), dim3(kThreads), 0, c->stream, c->wring.p, (const double*)c->dring.p, ng,
                      c->nt, m, (const ScfState*)c->st.p, c->ds, En.E.p, c->e2part.p);
   hipLaunchKernelGGL(k_schedule, dim3(1), dim3(kThreads), 0, c->stream, c->cd, (const double*)c->e2part.p, nb, ng, c->nt,
                      c->e1.p, c->e2.p, c->d1.p, c->d2.p, c->st.p, 1, m, done);

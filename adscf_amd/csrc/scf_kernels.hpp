@@ -435,19 +435,38 @@ __global__ void __launch_bounds__(kThreads) k_gram(const double* __restrict__ dr
   __syncthreads();
   const long ntile = (nvar + kGramTile - 1) / kGramTile;
   const double* d0 = dring + (long)head * nvar;
-  for (long tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+  // The next tile's history values are loaded into registers before the current tile's
+  // products, so each block keeps its loads in flight behind its arithmetic.
+  constexpr int kPre = ((kMaxHist + 1) * kGramTile + kThreads - 1) / kThreads;
+  double pre0[kPre], prei[kPre];
+  auto fetch = [&](long tile) {
     const long k0 = tile * kGramTile;
-    for (int e = tid; e < nv * kGramTile; e += blockDim.x) {
-      const int i = e / kGramTile, c = e % kGramTile;
-      const long k = k0 + c;
-      double y = 0.0;
-      if (k < nvar) {
-        const double x0 = d0[k];
-        y = i == 0 ? x0 : x0 - slot[i][k];
+#pragma unroll
+    for (int q = 0; q < kPre; ++q) {
+      const int e = tid + q * kThreads;
+      pre0[q] = prei[q] = 0.0;
+      if (e < nv * kGramTile) {
+        const int i = e / kGramTile, c = e % kGramTile;
+        const long k = k0 + c;
+        if (k < nvar) {
+          pre0[q] = d0[k];
+          if (i != 0) prei[q] = slot[i][k];
+        }
       }
-      ys[i * ld + c] = y;
+    }
+  };
+  if ((long)blockIdx.x < ntile) fetch(blockIdx.x);
+  for (long tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+#pragma unroll
+    for (int q = 0; q < kPre; ++q) {
+      const int e = tid + q * kThreads;
+      if (e < nv * kGramTile) {
+        const int i = e / kGramTile, c = e % kGramTile;
+        ys[i * ld + c] = i == 0 ? pre0[q] : pre0[q] - prei[q];
+      }
     }
     __syncthreads();
+    if (tile + gridDim.x < ntile) fetch(tile + gridDim.x);
     if (worker) {
       for (int c = r * sub; c < (r + 1) * sub; ++c) {
         double yi[kGramBlk], yj[kGramBlk];
@@ -523,10 +542,20 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// The loops over k, i and j run in unrolled groups of kLuBatch with no branch per
+// step: the range test is part of each lane's predicate, and the reads of a(lane, k)
+// for a group are issued together (rows beyond m read row 0 and columns are clamped;
+// those values are never used).  Padding lanes take values that leave a step without
+// effect: -1 in the pivot scan (never >= the running maximum) and 0 in the backward
+// chain (s - 0 = s).  The pivot column stays in registers: the row interchange swaps
+// the other columns in LDS and column j in registers, and lane j keeps indx(j).
+constexpr int kLuBatch = 4;
+
 __device__ int lu_solve_wave(double* a, double* b, int m, int* indx) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & 63;
   const bool row = lane < m;
+  const int rl = row ? lane : 0;
   double vv = 0.0;
   int zero_row = 0;
   if (row) {
@@ -539,80 +568,99 @@ __device__ int lu_solve_wave(double* a, double* b, int m, int* indx) {
     vv = 1.0 / aamax;
   }
   const int singular = __ballot(zero_row) != 0;
-  int imax = 0;
+  int imax = 0, piv_lane = 0;
   for (int j = 0; j < m; ++j) {
     double c = row ? a[lane + j * m] : 0.0;
-    double nxt = (row && j > 0) ? a[lane] : 0.0;  // a(lane, k) one step ahead
-    for (int k = 0; k < j; ++k) {
-      const double aik = nxt;
-      if (row && k + 1 < j) nxt = a[lane + (k + 1) * m];
-      const double akj = lane_bcast(c, k);  // final: lane k has applied every k' < k
-      if (row && lane > k) c = c - aik * akj;
-    }
-    // pivot search of ludcmp (:263-270): the scan over i = j..n in order, '.ge.'
-    const double dum_l = row ? vv * fabs(c) : 0.0;
-    double aamax = 0.0;
-    for (int i = j; i < m; ++i) {
-      const double dum = lane_bcast(dum_l, i);
-      if (dum >= aamax) {
-        imax = i;
-        aamax = dum;
+    for (int k0 = 0; k0 < j; k0 += kLuBatch) {
+      double aik[kLuBatch];
+#pragma unroll
+      for (int u = 0; u < kLuBatch; ++u) aik[u] = a[rl + min(k0 + u, j - 1) * m];
+#pragma unroll
+      for (int u = 0; u < kLuBatch; ++u) {
+        const int k = k0 + u;
+        const double akj = lane_bcast(c, k);  // final: lane k has applied every k' < k
+        if (row && lane > k && k < j) c = c - aik[u] * akj;
       }
     }
-    if (row) a[lane + j * m] = c;
-    wave_lds_sync();
+    // pivot search of ludcmp (:263-270): the scan over i = j..n in order, '.ge.'
+    const double dum_l = row ? vv * fabs(c) : -1.0;
+    double aamax = 0.0;
+    for (int i0 = j; i0 < m; i0 += kLuBatch) {
+#pragma unroll
+      for (int u = 0; u < kLuBatch; ++u) {
+        const double dum = lane_bcast(dum_l, i0 + u);
+        const bool ge = dum >= aamax;
+        imax = ge ? i0 + u : imax;
+        aamax = ge ? dum : aamax;
+      }
+    }
     if (j != imax) {
-      if (lane < m) {
+      // interchange rows j and imax: column j in registers, the others in LDS (lane = column)
+      const double cj = lane_bcast(c, j), ci = lane_bcast(c, imax);
+      if (row && lane != j) {
         const double t = a[imax + lane * m];
         a[imax + lane * m] = a[j + lane * m];
         a[j + lane * m] = t;
       }
+      if (lane == j) c = ci;
+      if (lane == imax) c = cj;
       const double vj = lane_bcast(vv, j);
       if (lane == imax) vv = vj;
-      wave_lds_sync();
     }
-    if (lane == 0) {
-      indx[j] = imax + 1;
-      if (a[j + j * m] == 0.0) a[j + j * m] = 1.0e-20;
+    if (lane == j) piv_lane = imax + 1;
+    double piv = lane_bcast(c, j);
+    if (piv == 0.0) {
+      piv = 1.0e-20;
+      if (lane == j) c = piv;
     }
-    wave_lds_sync();
     if (j != m - 1) {
-      const double dinv = 1.0 / a[j + j * m];
-      if (row && lane > j) a[lane + j * m] = a[lane + j * m] * dinv;
+      const double dinv = 1.0 / piv;
+      if (row && lane > j) c = c * dinv;
     }
+    if (row) a[lane + j * m] = c;
     wave_lds_sync();
   }
+  if (row) indx[lane] = piv_lane;
   // forward substitution: the interchanges first (each touches entries >= i only,
   // so applying them up front gives every step the values lubksb reads), then
   // y_i -= a(i,j) y_j for j = ii, ii+1, ... with ii the first nonzero entry
-  if (lane == 0) {
-    for (int i = 0; i < m; ++i) {
-      const int ll = indx[i] - 1;
-      const double t = b[ll];
-      b[ll] = b[i];
-      b[i] = t;
-    }
-  }
-  wave_lds_sync();
   double y = row ? b[lane] : 0.0;
+  for (int i = 0; i < m; ++i) {
+    const int ll = __builtin_amdgcn_readlane(piv_lane, i) - 1;
+    const double tl = lane_bcast(y, ll), ti = lane_bcast(y, i);
+    if (lane == ll) y = ti;
+    if (lane == i) y = tl;
+  }
   int first = m;
   for (int i = 0; i < m; ++i)
     if (lane_bcast(y, i) != 0.0) {
       first = i;
       break;
     }
-  for (int j = first; j < m - 1; ++j) {
-    const double yj = lane_bcast(y, j);
-    if (row && lane > j) y = y - a[lane + j * m] * yj;
+  for (int j0 = first; j0 < m - 1; j0 += kLuBatch) {
+    double aj[kLuBatch];
+#pragma unroll
+    for (int u = 0; u < kLuBatch; ++u) aj[u] = a[rl + min(j0 + u, m - 2) * m];
+#pragma unroll
+    for (int u = 0; u < kLuBatch; ++u) {
+      const int j = j0 + u;
+      const double yj = lane_bcast(y, j);
+      if (row && lane > j && j < m - 1) y = y - aj[u] * yj;
+    }
   }
   // backward substitution (:329-335): the products a(i,j) x_j in parallel (lane j),
   // then the reference's increasing-j chain of subtractions
+  const double diag = row ? a[lane + lane * m] : 1.0;
   double x = 0.0;
+  double arow = row ? a[(m - 1) + lane * m] : 0.0;  // a(i, lane), one row ahead
   for (int i = m - 1; i >= 0; --i) {
-    const double t = (lane > i && row) ? a[i + lane * m] * x : 0.0;
+    const double t = (lane > i && row) ? arow * x : 0.0;
+    if (i > 0 && row) arow = a[(i - 1) + lane * m];
     double sacc = lane_bcast(y, i);
-    for (int j = i + 1; j < m; ++j) sacc = sacc - lane_bcast(t, j);
-    const double xi = sacc / a[i + i * m];
+    for (int j0 = i + 1; j0 < m; j0 += kLuBatch)
+#pragma unroll
+      for (int u = 0; u < kLuBatch; ++u) sacc = sacc - lane_bcast(t, j0 + u);
+    const double xi = sacc / lane_bcast(diag, i);
     if (lane == i) x = xi;
   }
   if (row) b[lane] = x;
@@ -741,10 +789,12 @@ __global__ void k_solve(SolveArgs a) {
   }
 }
 
+constexpr int kMixBatch = 8;
+
 // Mixing update (mod_iterate.f90:125-148) over the ring: the new field goes
 // to the slot of the discarded oldest entry, which becomes logical index 0
 // after the rotation in k_schedule.  E of the new field and its E^2 partials
-// are produced in the same pass.
+// are produced in the same pass.  Grid row y covers the types y, y + gridDim.y, ...
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_mix(double* __restrict__ wring, const double* __restrict__ dring, long ng,
                                                   int nt, int m, const ScfState* st, double ds, T* __restrict__ E,
@@ -765,17 +815,29 @@ __global__ void __launch_bounds__(kThreads) k_mix(double* __restrict__ wring, co
   const double* w0 = wring + (long)head * nvar;
   const double* d0 = dring + (long)head * nvar;
   double* wn = wring + (long)dst * nvar;
-  for (int t = 0; t < nt; ++t) {
+  for (int t = blockIdx.y; t < nt; t += gridDim.y) {
     double acc = 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ng; i += (long)gridDim.x * blockDim.x) {
       const long k = (long)t * ng + i;
       const double W0 = w0[k], D0 = d0[k];
       double f = W0, d = D0;
       if (mode == 1) {
-        for (int j = 1; j <= m; ++j) {
-          const long o = off[j] + k;
-          f = f - x[j - 1] * (W0 - wring[o]);
-          d = d - x[j - 1] * (D0 - dring[o]);
+        // kMixBatch history entries are loaded before their updates (entries past m
+        // re-read entry m and are not used)
+        for (int j0 = 1; j0 <= m; j0 += kMixBatch) {
+          double wv[kMixBatch], dv[kMixBatch];
+#pragma unroll
+          for (int u = 0; u < kMixBatch; ++u) {
+            const long o = off[min(j0 + u, m)] + k;
+            wv[u] = wring[o];
+            dv[u] = dring[o];
+          }
+#pragma unroll
+          for (int u = 0; u < kMixBatch; ++u)
+            if (j0 + u <= m) {
+              f = f - x[j0 + u - 1] * (W0 - wv[u]);
+              d = d - x[j0 + u - 1] * (D0 - dv[u]);
+            }
         }
       }
       const double wnew = f + lam * d;
