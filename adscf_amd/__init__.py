@@ -68,6 +68,7 @@ SIGNATURES = {
     "adscf_mixing_lambda": (C.c_int, [_ctxp, _dp, _dp, C.c_long, C.c_int, C.c_double, C.c_int]),
     "adscf_mixing_anderson": (C.c_int, [_ctxp, _dp, _dp, C.c_long, C.c_int, C.c_double, C.c_int]),
     "adscf_ludcmp": (C.c_int, [_dp, C.c_int, _ip, _dp]),
+    "adscf_device_lu_solve": (C.c_int, [_ctxp, _dp, C.c_int, _dp, _dp, _ip, _ip]),
     "adscf_lubksb": (C.c_int, [_dp, C.c_int, _ip, _dp]),
     "adscf_solve_lu": (C.c_int, [_dp, _dp, _dp, C.c_int]),
     "adscf_solve_cholesky": (C.c_int, [_dp, _dp, _dp, C.c_int]),
@@ -355,6 +356,18 @@ class Context:
     def mixing_anderson(self, field, dfield, lam, it):
         self._chk(self._lib.adscf_mixing_anderson(self._h, _p(field), _p(dfield), field.shape[1],
                                                   field.shape[0] - 1, float(lam), int(it)))
+
+    def device_lu_solve(self, a, b):
+        """The Anderson step's device LU (ludcmp + lubksb) on one (n,n) system, n <= 40;
+        returns (x, lu, indx, singular) in the layout of ludcmp() and solve_lu()."""
+        n = a.shape[0]
+        af = np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)
+        x = _f64(b).copy()
+        lu = np.empty((n, n))
+        indx = np.zeros(n, dtype=np.int32)
+        sing = C.c_int()
+        self._chk(self._lib.adscf_device_lu_solve(self._h, _p(af), n, _p(x), _p(lu), _pi(indx), C.byref(sing)))
+        return x, lu.T.copy(), indx, bool(sing.value)
 
     # -- device-resident SCF ------------------------------------------------
     def scf_setup(self, block_types, block_segments, ensemble=1, rho=1.0, mu=0.0, mixing_dim=10, lam_mix=0.5):

@@ -2519,6 +2519,30 @@ int adscf_mixing_anderson(adscf_ctx* c, double* field, double* dfield, long fiel
   });
 }
 
+int adscf_device_lu_solve(adscf_ctx* c, const double* a, int n, double* b, double* lu, int* indx, int* singular) {
+  return guarded(c, [&] {
+    if (!a || !b || n < 1 || n > kMaxHist) throw ArgError{"adscf_device_lu_solve: need a, b and 1 <= n <= 40"};
+    c->io.ensure((size_t)n * n + n);
+    c->io2.ensure((size_t)n * n + n);
+    DevBuf<int> piv;
+    piv.ensure(n + 1);
+    HIPCK(hipMemcpyAsync(c->io.p, a, (size_t)n * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipMemcpyAsync(c->io.p + (size_t)n * n, b, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_lu_check, dim3(1), dim3(kThreads), 0, c->stream, (const double*)c->io.p,
+                       (const double*)c->io.p + (size_t)n * n, n, c->io2.p, piv.p);
+    HIPCK(hipGetLastError());
+    std::vector<int> hp(n + 1);
+    HIPCK(hipMemcpyAsync(b, c->io2.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (lu)
+      HIPCK(hipMemcpyAsync(lu, c->io2.p + n, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipMemcpyAsync(hp.data(), piv.p, (n + 1) * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (indx) std::copy(hp.begin(), hp.begin() + n, indx);
+    if (singular) *singular = hp[n];
+    return ADSCF_OK;
+  });
+}
+
 int adscf_ludcmp(double* a, int n, int* indx, double* d) {
   if (!a || !indx || !d || n < 1) return ADSCF_ERR_ARG;
   std::vector<double> vv(n + 1);

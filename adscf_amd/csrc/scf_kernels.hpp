@@ -789,6 +789,31 @@ __global__ void k_solve(SolveArgs a) {
   }
 }
 
+// The wave LU of k_solve on one n x n system (column-major a, right-hand side b), for
+// checking it against the host ludcmp / lubksb: out = [x (n), factors (n*n)],
+// ipiv = [indx (n, 1-based), singular].
+__global__ void k_lu_check(const double* __restrict__ a_in, const double* __restrict__ b_in, int n,
+                           double* __restrict__ out, int* __restrict__ ipiv) {
+  __shared__ double A[kMaxHist * kMaxHist];
+  __shared__ double B[kMaxHist];
+  __shared__ int indx[kMaxHist];
+  __shared__ int sing;
+  for (int i = threadIdx.x; i < n * n; i += blockDim.x) A[i] = a_in[i];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) B[i] = b_in[i];
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int s = lu_solve_wave(A, B, n, indx);
+    if (threadIdx.x == 0) sing = s;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    out[i] = B[i];
+    ipiv[i] = indx[i];
+  }
+  for (int i = threadIdx.x; i < n * n; i += blockDim.x) out[n + i] = A[i];
+  if (threadIdx.x == 0) ipiv[n] = sing;
+}
+
 constexpr int kMixBatch = 8;
 
 // Mixing update (mod_iterate.f90:125-148) over the ring: the new field goes

@@ -129,6 +129,11 @@ int adscf_ludcmp(double* a, int n, int* indx, double* d);                       
 int adscf_lubksb(const double* a, int n, const int* indx, double* b);              /* :302-337 */
 int adscf_solve_lu(const double* a, const double* b, double* x, int n);            /* :203-224 */
 int adscf_solve_cholesky(const double* a, const double* b, double* x, int n);      /* :160-195 */
+/* The device wavefront LU that the Anderson step runs (ludcmp + lubksb, mod_iterate.f90:232-337)
+   on one n x n system, 1 <= n <= 40, for checking it against adscf_ludcmp / adscf_lubksb:
+   a column-major (unchanged), b overwritten by the solution; lu (n*n) receives the factors and
+   indx (n) the 1-based pivots when not null, singular (when not null) 1 if a row of a is zero. */
+int adscf_device_lu_solve(adscf_ctx* ctx, const double* a, int n, double* b, double* lu, int* indx, int* singular);
 
 /* ---- device-resident SCF loop ------------------------------------------ */
 /* The loop of Find_SCF_Saddle_Point (Main_Multiblock/main_multiblock.f90:648-700):

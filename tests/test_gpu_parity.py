@@ -150,6 +150,38 @@ def test_mixing_anderson(amd, orc, m, it):
     assert _rel(f1, f2) < 1e-11  # random histories are well conditioned
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 20, 21, 33, 40])
+def test_device_lu_matches_host_lu_bitwise(amd, orc, n):
+    """The wavefront LU of the Anderson step gives the bits of the host ludcmp/lubksb (lu.hpp),
+    which the oracle's ludcmp matches bit for bit: general, symmetric Gram, badly scaled, equal-row
+    (zero pivot -> 1e-20) and zero-row (singular) matrices, pivoting included."""
+    rng = np.random.default_rng(100 + n)
+    a0 = rng.uniform(-0.5, 0.5, (n, n))
+    v = rng.standard_normal((n, 3 * n + 1))
+    cases = [a0, v @ v.T, a0 * 10.0 ** rng.integers(-6, 7, (n, n))]
+    if n > 2:
+        eq = a0.copy()
+        eq[2] = eq[0]
+        zr = a0.copy()
+        zr[1] = 0.0
+        cases += [eq, zr]
+    pivots = 0
+    with amd.Context(4, 4, 4, 1.0, 1.0, 1.0, 0.01) as ctx:
+        for a in cases:
+            b = rng.uniform(-1.0, 1.0, n)
+            x, lu, indx, sing = ctx.device_lu_solve(a, b)
+            lu_h, indx_h, _ = amd.ludcmp(a)
+            x_h = amd.solve_lu(a, b)
+            lu_o, indx_o, _, sing_o = orc.ludcmp(a)
+            assert np.array_equal(indx, indx_h) and np.array_equal(indx, indx_o)
+            assert np.array_equal(lu, lu_h, equal_nan=True) and np.array_equal(lu, lu_o, equal_nan=True)
+            assert np.array_equal(x, x_h, equal_nan=True)
+            assert sing == bool(sing_o)
+            pivots += int(np.sum(indx != np.arange(1, n + 1)))
+    if n > 2:
+        assert pivots > 0
+
+
 def test_mixing_lambda_and_simple(amd, orc):
     rng = np.random.default_rng(9)
     F = rng.standard_normal((3, 500))
