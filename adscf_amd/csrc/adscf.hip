@@ -918,29 +918,17 @@ int c2r_impl(adscf_ctx* c, const double* in, double* prop) {
 // ---------------------------------------------------------------------------
 // device-resident SCF iteration
 // ---------------------------------------------------------------------------
+// The mixing half of an iteration: Gram sums, convergence test and solve, the mix with E
+// of the new field, the rescale schedule and the ring rotation.  resume: the loop of an
+// earlier call stopped at convergence; the test uses that evaluation's accuracy against the
+// current tolerance (no new evaluation), so a continued loop is the one that never stopped.
 template <typename T>
-void enqueue_iteration(adscf_ctx* c, bool passes_too = true) {
+void enqueue_mix_tail(adscf_ctx* c, int nbc, bool resume) {
   Engine<T>& En = eng<T>(c);
   const long ng = c->ng;
   const long nvar = ng * c->nt;
   const int* done = &c->st.p->done;
   const int nb = grid_for(ng);
-  int nbc;
-  if (!c->bl_cd.empty()) {
-    nbc = enqueue_blend_densities<T>(c, done, true);
-  } else {
-    run_chain<T>(c, c->cd, c->st.p, done, c->rho, c->mu, passes_too);
-    ContourArgs<T> a = contour_args<T>(c, c->cd, c->st.p);
-    a.mode = CONTOUR_SCF;
-    a.phi = c->phi.p;
-    a.dring = c->dring.p;
-    a.done = done;
-    a.w0 = c->wring.p;  // ring base; the kernel reads slot `head` from the device state
-    a.target = c->resid_kind ? c->target.p : nullptr;
-    a.measure = c->measure;
-    nbc = launch_contour<T>(c->stream, a);
-    HIPCK(hipGetLastError());
-  }
   const int m = c->m;
   if (c->mixing == 2) {
     // Mixing_Lambda needs <D0-D1, D0-D1> and <W0-W1, W0-W1> from iteration 2 on
@@ -967,12 +955,36 @@ void enqueue_iteration(adscf_ctx* c, bool passes_too = true) {
   s.mixing = c->mixing;
   s.gsum_w = c->gsum_w.p;
   s.st = c->st.p;
+  s.resume = resume ? 1 : 0;
   hipLaunchKernelGGL(k_solve, dim3(1), dim3(kThreads), 0, c->stream, s);
   hipLaunchKernelGGL(k_mix<T>, dim3(nb, c->nt), dim3(kThreads), 0, c->stream, c->wring.p, (const double*)c->dring.p, ng,
                      c->nt, m, (const ScfState*)c->st.p, c->ds, En.E.p, c->e2part.p);
   hipLaunchKernelGGL(k_schedule, dim3(1), dim3(kThreads), 0, c->stream, c->cd, (const double*)c->e2part.p, nb, ng, c->nt,
                      c->e1.p, c->e2.p, c->d1.p, c->d2.p, c->st.p, 1, m, done);
   HIPCK(hipGetLastError());
+}
+
+
+template <typename T>
+void enqueue_iteration(adscf_ctx* c, bool passes_too = true) {
+  const int* done = &c->st.p->done;
+  int nbc;
+  if (!c->bl_cd.empty()) {
+    nbc = enqueue_blend_densities<T>(c, done, true);
+  } else {
+    run_chain<T>(c, c->cd, c->st.p, done, c->rho, c->mu, passes_too);
+    ContourArgs<T> a = contour_args<T>(c, c->cd, c->st.p);
+    a.mode = CONTOUR_SCF;
+    a.phi = c->phi.p;
+    a.dring = c->dring.p;
+    a.done = done;
+    a.w0 = c->wring.p;  // ring base; the kernel reads slot `head` from the device state
+    a.target = c->resid_kind ? c->target.p : nullptr;
+    a.measure = c->measure;
+    nbc = launch_contour<T>(c->stream, a);
+    HIPCK(hipGetLastError());
+  }
+  enqueue_mix_tail<T>(c, nbc, false);
 }
 
 template <typename T>
@@ -1088,8 +1100,26 @@ void build_graph(adscf_ctx* c) {
 template <typename T>
 int scf_iterate_impl(adscf_ctx* c, int niter, double tol, int* iters, double* acc, int* conv) {
   if (c->use_graph) build_graph<T>(c);
-  HIPCK(hipMemcpyAsync(&c->st.p->tol, &tol, sizeof(double), hipMemcpyHostToDevice, c->stream));
   ScfState st{};
+  HIPCK(hipMemcpyAsync(&c->st.p->tol, &tol, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  if (st.done && st.iter > 0 && niter > 0) {
+    // an earlier call stopped at convergence: continue its loop under the current
+    // tolerance from the mixing step of its last iteration
+    const int zero = 0;
+    HIPCK(hipMemcpyAsync(&c->st.p->done, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+    enqueue_mix_tail<T>(c, 0, true);
+    HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (st.done) {
+      c->phi_fresh = true;
+      if (iters) *iters = st.iter;
+      if (acc) *acc = st.accuracy;
+      if (conv) *conv = 1;
+      return ADSCF_OK;
+    }
+  }
   int done_iters = 0;
   float ms_total = 0.f;
   while (done_iters < niter) {

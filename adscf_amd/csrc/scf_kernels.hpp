@@ -685,6 +685,7 @@ struct SolveArgs {
   int mixing;          // 3 Anderson (default 0 reads as 3), 1 simple, 2 lambda
   const double* gsum_w;  // lambda mixing: Gram sums of the field ring
   ScfState* st;
+  int resume;          // test st->accuracy of the last evaluation again (no new one), then mix
 };
 
 // Convergence test (main_multiblock.f90:676-685), grand canonical rho/energy
@@ -702,6 +703,14 @@ __global__ void k_solve(SolveArgs a) {
   int it;
   if (a.iter_override > 0) {
     it = a.iter_override;
+  } else if (a.resume) {
+    it = st->iter;
+    if (tid == 0) {
+      done_now = !(st->accuracy > st->tol);
+      if (done_now) st->done = 1;
+    }
+    __syncthreads();
+    if (done_now) return;
   } else {
     double r = 0.0;
     if (a.measure) {

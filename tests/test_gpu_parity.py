@@ -256,6 +256,43 @@ def test_scf_converges_to_oracle_saddle_point(amd, orc):
     assert np.max(np.abs(den - den_ref)) < 1e-6
 
 
+def test_scf_continues_after_convergence(amd, orc):
+    """A loop that stopped at a loose tolerance continues under a tighter one as if it had
+    never stopped (same iterations, fields and residuals, bit for bit), and the FP32 phase
+    followed by an FP64 refinement converges to the oracle's saddle point."""
+    g, w0, chi = _scf_case(orc)
+
+    def ctx_for(prec):
+        ctx = amd.Context(8, 8, 8, 3.5, 3.5, 3.5, 0.05, precision=prec)
+        ctx.set_interactions(25.0, chi, [1.0, 1.0])
+        ctx.scf_setup([1, 2], [10, 10], ensemble=1, rho=1.0, mixing_dim=10, lam_mix=0.1)
+        ctx.scf_set_field(w0)
+        return ctx
+
+    with ctx_for(amd.FP64) as a, ctx_for(amd.FP64) as b:
+        s1 = a.scf_iterate(2000, tol=1e-4)
+        assert s1.converged and s1.accuracy <= 1e-4
+        again = a.scf_iterate(2000, tol=1e-3)  # already below: returns at once
+        assert again.converged and again.iters == s1.iters
+        s2 = a.scf_iterate(2000, tol=1e-9)
+        ref = b.scf_iterate(2000, tol=1e-9)
+        assert s2.converged and s2.iters == ref.iters > s1.iters and s2.accuracy == ref.accuracy
+        fa, pa = a.scf_get()
+        fb, pb = b.scf_get()
+        assert np.array_equal(fa, fb) and np.array_equal(pa, pb)
+        assert np.array_equal(a.scf_residuals(), b.scf_residuals())
+    with ctx_for(amd.FP32) as c:
+        s32 = c.scf_iterate(2000, tol=1e-4)
+        assert s32.converged
+        c.set_precision(amd.FP64)
+        s64 = c.scf_iterate(2000, tol=1e-9)
+        assert s64.converged and s64.iters > s32.iters and s64.accuracy <= 1e-9
+        f, _ = c.scf_get()
+    fh = orc.FH(25.0, chi, (1.0, 1.0))
+    den, _ = orc.species_densities(g, [1, 2], [10, 10], f)
+    assert np.max(np.abs(orc.mean_fields(g, fh, den) - f)) < 2e-9
+
+
 @pytest.mark.parametrize("n,shape", [(32, (4, 8)), (32, (8, 4)), (32, (2, 16)), (48, (8, 6)), (48, (6, 8)),
                                      (48, (16, 3)), (64, (8, 8)), (64, (4, 16)), (64, (16, 4)), (24, (8, 3)),
                                      (24, (4, 6)), (128, (8, 16)), (128, (16, 8)), (16, (1, 16)), (64, (0, 0)),
