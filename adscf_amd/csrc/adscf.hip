@@ -3189,6 +3189,28 @@ int adscf_batch_iterate(adscf_batch* b, int niter, double tol, int* iters, doubl
       HIPCK(hipMemcpyAsync(&c->st.p->tol, &tol, sizeof(double), hipMemcpyHostToDevice, b->stream));
     const int n = (int)b->inst.size();
     std::vector<ScfState> st(n);
+    for (int i = 0; i < n; ++i)
+      HIPCK(hipMemcpyAsync(&st[i], b->inst[i]->st.p, sizeof(ScfState), hipMemcpyDeviceToHost, b->stream));
+    HIPCK(hipStreamSynchronize(b->stream));
+    // instances that stopped at convergence in an earlier call continue their loops under the
+    // current tolerance from the mixing step of their last evaluation (as adscf_scf_iterate)
+    const int zero = 0;
+    for (int i = 0; i < n && niter > 0; ++i) {
+      if (!(st[i].done && st[i].iter > 0)) continue;
+      adscf_ctx* c = b->inst[i];
+      HIPCK(hipMemcpyAsync(&c->st.p->done, &zero, sizeof zero, hipMemcpyHostToDevice, b->stream));
+      hipStream_t own = c->stream;
+      c->stream = b->stream;
+      try {
+        if (b->prec == ADSCF_FP64) enqueue_mix_tail<double>(c, 0, true);
+        else enqueue_mix_tail<float>(c, 0, true);
+      } catch (...) {
+        c->stream = own;
+        throw;
+      }
+      c->stream = own;
+    }
+    HIPCK(hipStreamSynchronize(b->stream));
     int done_iters = 0;
     float total = 0.f;
     while (done_iters < niter) {

@@ -66,3 +66,26 @@ def test_batch_autotune_keeps_results():
         b.iterate(5, tol=0.0)
         for i in range(len(inst)):
             assert np.max(np.abs(b.get(i)[1] - f_ref[i])) < 1e-11 * np.max(np.abs(f_ref[i]))
+
+
+def test_batch_continues_after_convergence():
+    """Instances that stop at a loose tolerance continue under a tighter one exactly as a
+    single call with the tight tolerance (iterations, fields and densities bit for bit)."""
+    import adscf_amd
+
+    inst, w = _instances()
+    runs = {}
+    for mode in ("two calls", "one call"):
+        with adscf_amd.Batch(inst, mixing_dim=4, lam_mix=0.2, precision=adscf_amd.FP64) as b:
+            b.set_fields(w)
+            if mode == "two calls":
+                it1, _, conv1 = b.iterate(3000, tol=1e-3)
+                assert conv1.all()
+            it, acc, conv = b.iterate(3000, tol=1e-7)
+            runs[mode] = (it, acc, conv, [b.get(i) for i in range(len(inst))])
+    (it2, acc2, conv2, g2), (it1c, acc1c, conv1c, g1c) = runs["two calls"], runs["one call"]
+    print("iterations:", it1, "then", it2, "single call", it1c)
+    assert conv2.all() and conv1c.all()
+    assert np.array_equal(it2, it1c) and np.array_equal(acc2, acc1c) and np.all(it2 > it1)
+    for a, c in zip(g2, g1c):
+        assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
