@@ -111,6 +111,7 @@ SIGNATURES = {
     "adscf_batch_timing": (C.c_int, [_ctxp, _dp, _dp]),
     "adscf_batch_autotune": (C.c_int, [_ctxp, C.c_int, C.c_char_p, C.c_int]),
     "adscf_hipfft_step_time": (C.c_int, [_ctxp, C.c_int, _dp]),
+    "adscf_hbm_bandwidth": (C.c_int, [_ctxp, C.c_long, C.c_int, _dp, _dp]),
     "adscf_adjoint": (C.c_int, [_ctxp, C.c_int, _ip, _ip, C.c_double, _dp, C.c_int, _dp, C.c_double, C.c_int,
                                 _dp, _dp, _dp, _dp, _dp]),
 }
@@ -506,6 +507,12 @@ class Context:
         v = C.c_double()
         self._chk(self._lib.adscf_hipfft_step_time(self._h, int(reps), C.byref(v)))
         return v.value
+
+    def hbm_bandwidth(self, nbytes=1 << 30, reps=7):
+        """Measured (copy GB/s counting read + write, read GB/s) of the device."""
+        cp, rd = C.c_double(), C.c_double()
+        self._chk(self._lib.adscf_hbm_bandwidth(self._h, int(nbytes), int(reps), C.byref(cp), C.byref(rd)))
+        return cp.value, rd.value
 
     # -- schedules ------------------------------------------------------------
     def autotune(self, reps=3):

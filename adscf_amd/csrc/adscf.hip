@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -3317,6 +3318,45 @@ int adscf_batch_autotune(adscf_batch* b, int reps, char* report, int report_cap)
       memcpy(report, rep.data(), k);
       report[k] = 0;
     }
+    return ADSCF_OK;
+  });
+}
+
+int adscf_hbm_bandwidth(adscf_ctx* c, long bytes, int reps, double* copy_GBs, double* read_GBs) {
+  return guarded(c, [&]() -> int {
+    if (bytes < (1L << 20) || reps < 1) throw ArgError{"adscf_hbm_bandwidth: bytes >= 1 MiB and reps >= 1"};
+    const long n = bytes / (long)sizeof(double);
+    DevBuf<double> a, b, part;
+    a.ensure(n);
+    b.ensure(n);
+    part.ensure(kRedBlocks * 4);
+    hipLaunchKernelGGL(k_fill<double>, dim3(kRedBlocks * 4), dim3(kThreads), 0, c->stream, a.p, 1.0, n);
+    HIPCK(hipGetLastError());
+    auto median_ms = [&](const std::function<void()>& f) {
+      f();
+      std::vector<float> ms(reps);
+      for (int r = 0; r < reps; ++r) {
+        HIPCK(hipEventRecord(c->ev0, c->stream));
+        f();
+        HIPCK(hipEventRecord(c->ev1, c->stream));
+        HIPCK(hipEventSynchronize(c->ev1));
+        HIPCK(hipEventElapsedTime(&ms[r], c->ev0, c->ev1));
+      }
+      std::sort(ms.begin(), ms.end());
+      return (double)ms[reps / 2];
+    };
+    const double nb = (double)n * sizeof(double);
+    const double tc = median_ms([&] {
+      HIPCK(hipMemcpyAsync(b.p, a.p, (size_t)nb, hipMemcpyDeviceToDevice, c->stream));
+    });
+    const double tr = median_ms([&] {
+      hipLaunchKernelGGL(k_read_sum, dim3(kRedBlocks * 4), dim3(kThreads), 0, c->stream, (const double*)a.p, n,
+                         part.p);
+      HIPCK(hipGetLastError());
+    });
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (copy_GBs) *copy_GBs = 2.0 * nb / (tc * 1e-3) / 1e9;
+    if (read_GBs) *read_GBs = nb / (tr * 1e-3) / 1e9;
     return ADSCF_OK;
   });
 }

@@ -141,6 +141,23 @@ __global__ void __launch_bounds__(kThreads) k_schedule(ChainDesc cd, const doubl
   if (rotate && st && threadIdx.x == 0) st->head = (st->head + m) % (m + 1);
 }
 
+// Streaming read of n doubles (16-byte loads, two in flight per thread and step) into
+// per-block sums: the read bandwidth probe of adscf_hbm_bandwidth.
+__global__ void __launch_bounds__(kThreads) k_read_sum(const double* __restrict__ x, long n, double* __restrict__ part) {
+  __shared__ double sh[kThreads];
+  const double2* x2 = reinterpret_cast<const double2*>(x);
+  const long n2 = n / 2, stride = (long)gridDim.x * blockDim.x;
+  double acc = 0.0;
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  for (; i + stride < n2; i += 2 * stride) {
+    const double2 u = x2[i], v = x2[i + stride];
+    acc += (u.x + u.y) + (v.x + v.y);
+  }
+  for (; i < n2; i += stride) acc += x2[i].x + x2[i].y;
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
 // Update_Solvent (mod_solvent.f90:97-98): distribution = exp(-field), partial sums
 __global__ void __launch_bounds__(kThreads) k_solvent_dist(const double* __restrict__ w, long ng,
                                                            double* __restrict__ dist, double* __restrict__ part) {

@@ -269,6 +269,15 @@ def copy_probe(mib=256, reps=3):
     del x, y, z
 
 
+def measured_hbm(adscf_amd, nbytes=1 << 30, reps=7):
+    """Device bandwidth beside the 8 TB/s peak: a 1 GiB device-to-device copy (read + write)
+    and a 1 GiB streaming read, median of reps (adscf_hbm_bandwidth, HIP events)."""
+    with adscf_amd.Context(8, 8, 8, 1.0, 1.0, 1.0, 0.1) as ctx:
+        cp, rd = ctx.hbm_bandwidth(nbytes, reps)
+    return {"copy_GBs": cp, "read_GBs": rd, "bytes": nbytes,
+            "method": "hipMemcpyAsync device-to-device and a streaming-read kernel over 1 GiB, median of %d" % reps}
+
+
 def run_large(adscf_amd, fields, steps, warmup):
     """The large mixed-precision configuration, 128^3 x 400 ABA triblock (0.25/0.5/0.25,
     chiN = 25, box 6.0, BCC seed, m = 20): iterations/s and the line-pass roofline for
@@ -405,6 +414,9 @@ def main():
         line["epd"] = run_epd(adscf_amd, fields)
     if not args.no_converge:
         line["convergence"] = converge(adscf_amd, fields, w0, args.converge_max_iter)
+    hbm = measured_hbm(adscf_amd)
+    hbm["line_pass_frac_of_copy"] = ach32 / hbm["copy_GBs"]
+    line["hbm_measured"] = hbm
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(fields, w0)
     print(json.dumps(line))

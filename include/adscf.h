@@ -283,6 +283,10 @@ int adscf_batch_autotune(adscf_batch* batch, int reps, char* report, int report_
  * Propagate_Step (E*q, r2c, K/Ng, c2r, E*q) with hipFFT plans on this context's grid and
  * precision.  For timing comparisons only; the library's own path does not use hipFFT. */
 int adscf_hipfft_step_time(adscf_ctx* ctx, int reps, double* us_per_step);
+/* Measured device bandwidth beside the 8 TB/s peak: a device-to-device copy of `bytes`
+ * (copy_GBs counts read + write) and a streaming read of `bytes` (read_GBs), median of reps
+ * timed with HIP events on the context stream; bytes >= 1 MiB.  Diagnostic only. */
+int adscf_hbm_bandwidth(adscf_ctx* ctx, long bytes, int reps, double* copy_GBs, double* read_GBs);
 
 /* ---- adjoint (no counterpart in the reference) ---------------------------- */
 /* Reverse-mode derivative of one canonical density evaluation of the chain
