@@ -380,3 +380,14 @@ def test_diblock_saddle_loop_matches_oracle(orc):
     print(f"converged after {st.iters} iterations, accuracy {st.accuracy:.3e}")
     assert st.accuracy <= 1e-12
     assert np.max(np.abs(d - phi)) <= 1e-12 * np.max(d)
+
+
+def test_hbm_bandwidth_probe(amd):
+    """The bandwidth diagnostic returns plausible rates and rejects sizes under 1 MiB (512 MiB
+    exceeds the 256 MiB last-level cache, so the rates are HBM rates below the 8 TB/s peak)."""
+    with amd.Context(8, 8, 8, 1.0, 1.0, 1.0, 0.1) as ctx:
+        cp, rd = ctx.hbm_bandwidth(512 << 20, 3)
+        print(f"copy {cp:.0f} GB/s, read {rd:.0f} GB/s")
+        assert 500.0 < cp < 8000.0 and 500.0 < rd < 8000.0
+        with pytest.raises(amd.AdscfError):
+            ctx.hbm_bandwidth(1000, 3)
