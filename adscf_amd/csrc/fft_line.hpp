@@ -165,9 +165,12 @@ ADSCF_HD void stockham_pass(const C2<T>* src, C2<T>* dst, int n, int ldn, int L,
   }
 }
 
-// Generic radix R (any factor): one output per work item, O(R) each.  The twiddle
-// indices advance by addition (r*k*tstep < n needs no reduction; (r*q) mod R wraps once
-// per step), so the inner loop is loads and complex multiply-adds only.
+// Generic radix R (an odd prime above 5): one output per work item.  Inputs r and R - r
+// meet conjugate kernel factors, w^(rq) and conj(w^(rq)), so each output folds them into
+// one pair: a w + b conj(w) = (a + b) Re w + i (a - b) Im w, (R - 1)/2 steps of four
+// multiply-adds.  The stage twiddles of the inputs (Ns > 1) advance by addition; (r q) mod R
+// wraps once per step.  Leftover primes come first in the plan, where Ns = 1 and the inputs
+// carry no stage twiddle.
 template <typename T>
 ADSCF_HD void stockham_pass_generic(const C2<T>* src, C2<T>* dst, int n, int ldn, int L, int Ns, int R, const C2<T>* tw,
                                     int dir, int tid, int nthr) {
@@ -175,6 +178,7 @@ ADSCF_HD void stockham_pass_generic(const C2<T>* src, C2<T>* dst, int n, int ldn
   const int tstep = n / (Ns * R);
   const int rstep = n / R;
   const int total = L * n;
+  const int half = (R - 1) / 2;
   for (int o = tid; o < total; o += nthr) {
     const int line = o / n;
     const int idx = o - line * n;
@@ -182,17 +186,23 @@ ADSCF_HD void stockham_pass_generic(const C2<T>* src, C2<T>* dst, int n, int ldn
     const int q = (idx / Ns) % R;
     const int j = (idx / (Ns * R)) * Ns + k;
     const C2<T>* s = src + line * ldn;
-    C2<T> acc = s[j];
-    int i1 = 0, rq = 0;
-    for (int r = 1; r < R; ++r) {
-      i1 += k * tstep;
+    const int step = k * tstep;  // stage twiddle index step per input
+    T ax = s[j].x, ay = s[j].y;
+    int rq = 0;
+    for (int r = 1; r <= half; ++r) {
       rq += q;
       if (rq >= R) rq -= R;
-      C2<T> v = s[j + r * nb];
-      if (Ns > 1) v = cmul(v, twid(tw, i1, dir));
-      acc = cadd(acc, cmul(v, twid(tw, rq * rstep, dir)));
+      C2<T> u = s[j + r * nb], v = s[j + (R - r) * nb];
+      if (Ns > 1) {
+        u = cmul(u, twid(tw, r * step, dir));
+        v = cmul(v, twid(tw, (R - r) * step, dir));
+      }
+      const C2<T> w = twid(tw, rq * rstep, dir);
+      const T sx = u.x + v.x, sy = u.y + v.y, dx = u.x - v.x, dy = u.y - v.y;
+      ax += sx * w.x - dy * w.y;
+      ay += sy * w.x + dx * w.y;
     }
-    dst[line * ldn + idx] = acc;
+    dst[line * ldn + idx] = C2<T>{ax, ay};
   }
 }
 
@@ -204,20 +214,26 @@ struct LinePlan {
   int radix[kMaxPasses];  // radix of each pass, product = n
 };
 
-// Factor n into radices: 8s first, then 4, 2, 3, 5, then leftover primes (generic).
+// Factor n into radices: leftover primes above 5 first (generic passes, which then run
+// without stage twiddles), then 8s, 4, 2, 3, 5.
 inline LinePlan make_line_plan(int n) {
   LinePlan p{};
   p.n = n;
   int m = n;
   int np = 0;
+  {
+    int r = m;
+    for (int f : {2, 3, 5})
+      while (r % f == 0) r /= f;
+    for (int f = 7; r > 1 && np < kMaxPasses; f += 2) {
+      while (r % f == 0 && np < kMaxPasses) { p.radix[np++] = f; r /= f; m /= f; }
+    }
+  }
   while (m % 8 == 0 && np < kMaxPasses) { p.radix[np++] = 8; m /= 8; }
   while (m % 4 == 0 && np < kMaxPasses) { p.radix[np++] = 4; m /= 4; }
   while (m % 2 == 0 && np < kMaxPasses) { p.radix[np++] = 2; m /= 2; }
   while (m % 3 == 0 && np < kMaxPasses) { p.radix[np++] = 3; m /= 3; }
   while (m % 5 == 0 && np < kMaxPasses) { p.radix[np++] = 5; m /= 5; }
-  for (int f = 7; m > 1 && np < kMaxPasses; f += 2) {
-    while (m % f == 0 && np < kMaxPasses) { p.radix[np++] = f; m /= f; }
-  }
   p.npass = (m == 1) ? np : -1;  // -1: too many factors
   return p;
 }
