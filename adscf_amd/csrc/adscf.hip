@@ -424,7 +424,7 @@ int launch_pass(adscf_ctx* c, const PassArgs<T>& p) {
   if (p.axis == 0) items = (int)(((long)c->ny * c->nz + p.L - 1) / p.L);
   else if (p.axis == 1) items = c->nz * p.gx;
   else items = c->ny * p.gx;
-  const size_t smem = stockham_lds_bytes<T>(p.L, p.ldn, p.plan.n);
+  const size_t smem = line_pass_lds_bytes<T>(p.L, p.ldn, p.plan.n);
   hipLaunchKernelGGL(k_line_pass<T>, dim3(items), dim3(kThreads), smem, c->stream, p);
   HIPCK(hipGetLastError());
   return items;

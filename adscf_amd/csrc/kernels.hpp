@@ -136,7 +136,7 @@ __device__ __forceinline__ T exp2i(int e) {
 }
 
 template <typename T>
-__device__ C2<T>* pass_line_op(const PassArgs<T>& a, C2<T>* b0, C2<T>* b1, const C2<T>* tw) {
+__device__ C2<T>* pass_line_op(const PassArgs<T>& a, C2<T>* b0, C2<T>* b1, const C2<T>* tw, const T* kf) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int n = a.plan.n;
   auto sync = [] { __syncthreads(); };
@@ -144,7 +144,7 @@ __device__ C2<T>* pass_line_op(const PassArgs<T>& a, C2<T>* b0, C2<T>* b1, const
   const int tot = a.L * n;
   for (int e = tid; e < tot; e += nthr) {
     const int l = e / n, k = e - l * n;
-    const T f = a.kf[k];
+    const T f = kf[k];
     C2<T> v = s[l * a.ldn + k];
     s[l * a.ldn + k] = {v.x * f, v.y * f};
   }
@@ -159,6 +159,11 @@ template <typename T>
 __host__ __device__ constexpr size_t stockham_lds_bytes(int L, int ldn, int n) {
   return (2 * (size_t)L * ldn + (size_t)n) * sizeof(C2<T>);
 }
+// k_line_pass also keeps the axis's n K multipliers in LDS
+template <typename T>
+__host__ __device__ constexpr size_t line_pass_lds_bytes(int L, int ldn, int n) {
+  return stockham_lds_bytes<T>(L, ldn, n) + (size_t)n * sizeof(T);
+}
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_line_pass(PassArgs<T> a) {
@@ -168,10 +173,14 @@ __global__ void __launch_bounds__(kThreads) k_line_pass(PassArgs<T> a) {
   C2<T>* b1 = b0 + a.L * a.ldn;
   C2<T>* twl = b1 + a.L * a.ldn;  // the axis twiddles: every butterfly reads them
   const int n = a.plan.n;
+  T* kfl = reinterpret_cast<T*>(twl + n);  // the K multipliers, read between the transforms
   const int tot = a.L * n;
   const int item = blockIdx.x;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  for (int e = tid; e < n; e += nthr) twl[e] = a.tw[e];
+  for (int e = tid; e < n; e += nthr) {
+    twl[e] = a.tw[e];
+    kfl[e] = a.kf[e];
+  }
   // ---- load
   for (int e = tid; e < tot; e += nthr) {
     int l, j;
@@ -194,7 +203,7 @@ __global__ void __launch_bounds__(kThreads) k_line_pass(PassArgs<T> a) {
     b0[l * a.ldn + j] = v;
   }
   __syncthreads();
-  C2<T>* r = pass_line_op(a, b0, b1, twl);
+  C2<T>* r = pass_line_op(a, b0, b1, twl, kfl);
   if (a.nops == 2) {
     const T sc1 = exp2i<T>(a.dexp1[a.sidx]);
     const T sc2 = exp2i<T>(a.dexp2[a.sidx]);
@@ -216,7 +225,7 @@ __global__ void __launch_bounds__(kThreads) k_line_pass(PassArgs<T> a) {
     }
     __syncthreads();
     C2<T>* other = (r == b0) ? b1 : b0;
-    r = pass_line_op(a, r, other, twl);
+    r = pass_line_op(a, r, other, twl, kfl);
   }
   // ---- store
   if (a.out_kind == OUT_TMP) {
