@@ -189,7 +189,29 @@ ADSCF_HD void stockham_pass_generic(const C2<T>* src, C2<T>* dst, int n, int ldn
     const int step = k * tstep;  // stage twiddle index step per input
     T ax = s[j].x, ay = s[j].y;
     int rq = 0;
-    for (int r = 1; r <= half; ++r) {
+    auto fold = [&](C2<T> u, C2<T> v, C2<T> w) {
+      const T sx = u.x + v.x, sy = u.y + v.y, dx = u.x - v.x, dy = u.y - v.y;
+      ax += sx * w.x - dy * w.y;
+      ay += sy * w.x + dx * w.y;
+    };
+    int r = 1;
+    if (Ns == 1) {
+      // groups of four steps: the twelve LDS reads of a group are issued before its folds
+      for (; r + 3 <= half; r += 4) {
+        C2<T> u[4], v[4], w[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          rq += q;
+          if (rq >= R) rq -= R;
+          u[t] = s[j + (r + t) * nb];
+          v[t] = s[j + (R - r - t) * nb];
+          w[t] = twid(tw, rq * rstep, dir);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) fold(u[t], v[t], w[t]);
+      }
+    }
+    for (; r <= half; ++r) {
       rq += q;
       if (rq >= R) rq -= R;
       C2<T> u = s[j + r * nb], v = s[j + (R - r) * nb];
@@ -197,10 +219,7 @@ ADSCF_HD void stockham_pass_generic(const C2<T>* src, C2<T>* dst, int n, int ldn
         u = cmul(u, twid(tw, r * step, dir));
         v = cmul(v, twid(tw, (R - r) * step, dir));
       }
-      const C2<T> w = twid(tw, rq * rstep, dir);
-      const T sx = u.x + v.x, sy = u.y + v.y, dx = u.x - v.x, dy = u.y - v.y;
-      ax += sx * w.x - dy * w.y;
-      ay += sy * w.x + dx * w.y;
+      fold(u, v, twid(tw, rq * rstep, dir));
     }
     dst[line * ldn + idx] = C2<T>{ax, ay};
   }
