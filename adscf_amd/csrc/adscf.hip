@@ -131,6 +131,7 @@ struct adscf_ctx {
   int user_kind = 0, user_measure = 0;  // the loop adscf_scf_iterate runs (adscf_scf_set_residual)
   int user_mixing = 3, mixing = 3;       // 1 simple, 2 lambda, 3 Anderson (adscf_scf_set_mixing)
   int mid_axis = -1;  // axis of the single-operator passes of a 3-D sweep (-1: y); the others merge
+  int done_loop = -1;  // loop kind ((mixing*2 + resid_kind)*4 + measure) of the call that converged
   DevBuf<double> gsum_w;                 // lambda mixing: Gram sums of the field ring
   DevBuf<double> target;             // target densities (nt, ng): the DDFT density
   DevBuf<double> mob;                // mobility_qq (nk, nt, nt)
@@ -1105,12 +1106,14 @@ int scf_iterate_impl(adscf_ctx* c, int niter, double tol, int* iters, double* ac
   HIPCK(hipMemcpyAsync(&c->st.p->tol, &tol, sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
+  const int loop = (c->mixing * 2 + c->resid_kind) * 4 + c->measure;
   if (st.done && st.iter > 0 && niter > 0) {
-    // an earlier call stopped at convergence: continue its loop under the current
-    // tolerance from the mixing step of its last iteration
+    // an earlier call stopped at convergence: the same loop continues under the current
+    // tolerance from the mixing step of its last iteration; another loop starts with a
+    // fresh evaluation of the current field
     const int zero = 0;
     HIPCK(hipMemcpyAsync(&c->st.p->done, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
-    enqueue_mix_tail<T>(c, 0, true);
+    if (loop == c->done_loop) enqueue_mix_tail<T>(c, 0, true);
     HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
     if (st.done) {
@@ -1143,6 +1146,7 @@ int scf_iterate_impl(adscf_ctx* c, int niter, double tol, int* iters, double* ac
     if (st.done) break;
   }
   c->last_ms_iter = done_iters > 0 ? ms_total / done_iters : 0.0;
+  c->done_loop = st.done ? loop : -1;
   c->phi_fresh = st.done != 0;
   if (iters) *iters = st.iter;
   if (acc) *acc = st.accuracy;
