@@ -148,7 +148,8 @@ int adscf_scf_set_field(adscf_ctx* ctx, const double* field);
 /* run up to niter iterations; stops early when .not.(accuracy > tol).  iters_total is the
  * reference's loop index so far, accuracy the last max|dfield|, converged 0/1.  After a call
  * that stopped at convergence, a call with a smaller tol continues the same loop (the mixing
- * step of the last evaluation, then further iterations) exactly as one call with that tol. */
+ * step of the last evaluation, then further iterations) exactly as one call with that tol;
+ * a call with another residual, measure or mixing type starts with a fresh evaluation. */
 int adscf_scf_iterate(adscf_ctx* ctx, int niter, double tol, int* iters_total, double* accuracy, int* converged);
 /* recompute densities of the current field (main_multiblock.f90:319) without mixing */
 int adscf_scf_evaluate(adscf_ctx* ctx);
