@@ -72,6 +72,9 @@ def test_propagate_step_fp64(amd, orc, dims):
     ((86, 1, 75), (17.2, 1.0, 1.0), 0.01, [1, 2], [10, 10], 1),          # radix 43 and 3*5*5
     ((9, 5, 7), (2.0, 2.0, 2.0), 0.05, [2, 1], [7, 13], 1),
     ((1, 1, 24), (1.0, 1.0, 5.0), 0.01, [1, 2], [30, 30], 1),            # single active axis
+    ((12, 10, 8), (3.0, 3.0, 3.0), 0.02, [1, 2, 1], [10, 0, 12], 1),     # empty middle block
+    ((12, 10, 8), (3.0, 3.0, 3.0), 0.02, [1, 2, 1], [0, 20, 3], 1),      # empty first block
+    ((12, 10, 8), (3.0, 3.0, 3.0), 0.02, [1, 2, 1], [1, 19, 0], 2),      # one-segment and empty last block
 ])
 def test_propagate_chain_fp64(amd, orc, case):
     dims, L, ds, bt, bs, ens = case
