@@ -307,20 +307,34 @@ def run_large(adscf_amd, fields, steps, warmup):
     return out
 
 
-def cpu_baseline(fields, w0):
-    """The CPU FP64 oracle (single thread) on one SCF iteration of the same workload."""
+def cpu_baseline(adscf_amd, fields, w0):
+    """The CPU FP64 oracle (single thread) on one SCF iteration of the same workload; its
+    result also checks the GPU's FP64 and FP32-mixed iteration from the same fields."""
+    import numpy as np
     import oracle
 
     segs = fields.block_segments([FA, 1 - FA], NS)
     g = oracle.Grid(N, N, N, L, L, L, 1.0 / NS)
     fh = oracle.FH(KAPPA, [[0.0, CHIN], [CHIN, 0.0]], [1.0, 1.0])
     t0 = time.perf_counter()
-    oracle.scf_anderson(g, [1, 2], segs, fh, w0, M, lam=LAM, tol=-1.0, maxit=1)
+    ref = oracle.scf_anderson(g, [1, 2], segs, fh, w0, M, lam=LAM, tol=-1.0, maxit=1)
     t = time.perf_counter() - t0
+    parity = {}
+    for name, prec in (("fp64", adscf_amd.FP64), ("fp32mixed", adscf_amd.FP32)):
+        with adscf_amd.Context(N, N, N, L, L, L, 1.0 / NS, precision=prec) as ctx:
+            ctx.set_interactions(KAPPA, [[0.0, CHIN], [CHIN, 0.0]], [1.0, 1.0])
+            ctx.scf_setup([1, 2], segs, ensemble=1, rho=1.0, mixing_dim=M, lam_mix=LAM)
+            ctx.scf_set_field(w0)
+            st = ctx.scf_iterate(1, tol=-1.0)
+            field, den = ctx.scf_get()
+        scale = lambda a: float(np.max(np.abs(a)))
+        parity[name] = {"phi_max_rel": float(np.max(np.abs(den - ref["density"]))) / scale(ref["density"]),
+                        "field_max_rel": float(np.max(np.abs(field - ref["field"]))) / scale(ref["field"]),
+                        "residual_rel": abs(st.accuracy - ref["accuracy"]) / ref["accuracy"]}
     return dict(value=1.0 / t, unit="SCF iters/s", cores=1, kind="port",
                 sample="1 SCF iteration (2x200 propagator steps, contour integral, mean fields, mixing) of the "
                        "64^3 workload by the single-threaded C FP64 oracle (oracle/scft_oracle.c)",
-                seconds=t)
+                seconds=t, gpu_vs_oracle_same_iteration=parity)
 
 
 def main():
@@ -418,7 +432,7 @@ def main():
     hbm["line_pass_frac_of_copy"] = ach32 / hbm["copy_GBs"]
     line["hbm_measured"] = hbm
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(fields, w0)
+        line["cpu_baseline"] = cpu_baseline(adscf_amd, fields, w0)
     print(json.dumps(line))
 
 
