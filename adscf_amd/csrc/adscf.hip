@@ -991,7 +991,6 @@ void enqueue_iteration(adscf_ctx* c, bool passes_too = true) {
 
 template <typename T>
 void enqueue_evaluate(adscf_ctx* c) {
-  const long ng = c->ng;
   if (!c->bl_cd.empty()) {
     enqueue_blend_densities<T>(c, nullptr, false);
     return;

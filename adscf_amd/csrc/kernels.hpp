@@ -95,8 +95,9 @@ __device__ __forceinline__ void block_dot_out(double acc, double* out) {
   const int tid = threadIdx.x;
   red[tid] = acc;
   __syncthreads();
-  for (int s = 1; s < blockDim.x; s <<= 1) {
-    if ((tid % (2 * s)) == 0 && tid + s < blockDim.x) red[tid] += red[tid + s];
+  const int nthr = (int)blockDim.x;
+  for (int s = 1; s < nthr; s <<= 1) {
+    if ((tid % (2 * s)) == 0 && tid + s < nthr) red[tid] += red[tid + s];
     __syncthreads();
   }
   if (tid == 0) out[blockIdx.x] = red[0];
