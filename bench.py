@@ -356,8 +356,8 @@ def main():
                     help="also run a 256 MiB device-to-device copy (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the SCFT path on a single MI355X; "
-              "multi-GPU runs are not part of it", file=sys.stderr)
+        print("bench.py measures one GPU: the SCFT path runs on a single MI355X; "
+              "runs on more than one GPU are not supported", file=sys.stderr)
         sys.exit(2)
     import adscf_amd
     from adscf_amd import fields
