@@ -74,6 +74,44 @@ struct AxisTables {
   int waves = 1;  // wavefronts per workgroup of the register kernel
 };
 
+// Captured graphs live until the context (or batch) is torn down: an executable graph that
+// is replaced (new buffers, another loop, a tuned schedule) is retired rather than destroyed,
+// and the sweep graphs of the timing entry points are kept per schedule and reused.  No
+// graph object is destroyed while the process may still hold work or tool records of it,
+// and a new executable graph never takes the address of one destroyed earlier in the
+// context's life.
+struct GraphPair {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t e = nullptr;
+};
+struct GraphKeep {
+  std::vector<GraphPair> retired;
+  std::vector<std::pair<std::string, GraphPair>> sweeps;  // timing sweeps by schedule key
+  void retire(hipGraph_t& g, hipGraphExec_t& e) {
+    if (g || e) retired.push_back({g, e});
+    g = nullptr;
+    e = nullptr;
+  }
+  GraphPair* find(const std::string& key) {
+    for (auto& kv : sweeps)
+      if (kv.first == key) return &kv.second;
+    return nullptr;
+  }
+  // after the owning stream has been synchronised
+  void release() {
+    for (GraphPair& p : retired) {
+      if (p.e) (void)hipGraphExecDestroy(p.e);
+      if (p.g) (void)hipGraphDestroy(p.g);
+    }
+    for (auto& kv : sweeps) {
+      if (kv.second.e) (void)hipGraphExecDestroy(kv.second.e);
+      if (kv.second.g) (void)hipGraphDestroy(kv.second.g);
+    }
+    retired.clear();
+    sweeps.clear();
+  }
+};
+
 // Per-precision engine: the propagator arrays and FFT tables in T.
 template <typename T>
 struct Engine {
@@ -119,6 +157,7 @@ struct adscf_ctx {
   double lam_mix = 0.5;
   double rho = 1.0, mu = 0.0;
   hipGraphExec_t graph = nullptr;
+  GraphKeep keep;         // retired and timing graphs, destroyed at adscf_finalize
   bool use_graph = true;  // ADSCF_GRAPH=0 in the environment: stream launches (profiling aid)
   hipGraph_t graph_src = nullptr;
   int graph_prec = -1;
@@ -132,6 +171,7 @@ struct adscf_ctx {
   int user_mixing = 3, mixing = 3;       // 1 simple, 2 lambda, 3 Anderson (adscf_scf_set_mixing)
   int mid_axis = -1;  // axis of the single-operator passes of a 3-D sweep (-1: y); the others merge
   int done_loop = -1;  // loop kind ((mixing*2 + resid_kind)*4 + measure) of the call that converged
+  int last_loop = -1;  // loop kind of the last call since adscf_scf_set_field (-1: none yet)
   DevBuf<double> gsum_w;                 // lambda mixing: Gram sums of the field ring
   DevBuf<double> target;             // target densities (nt, ng): the DDFT density
   DevBuf<double> mob;                // mobility_qq (nk, nt, nt)
@@ -267,14 +307,7 @@ void upload_tables(adscf_ctx* c) {
 
 void invalidate_tables(adscf_ctx* c) {
   c->tables64 = c->tables32 = false;
-  if (c->graph) {
-    (void)hipGraphExecDestroy(c->graph);
-    c->graph = nullptr;
-  }
-  if (c->graph_src) {
-    (void)hipGraphDestroy(c->graph_src);
-    c->graph_src = nullptr;
-  }
+  c->keep.retire(c->graph_src, c->graph);
 }
 
 template <typename T>
@@ -1037,6 +1070,11 @@ void scf_set_field_impl(adscf_ctx* c, const double* field) {
   HIPCK(hipStreamSynchronize(c->stream));
 }
 
+// The loop an iteration runs: mixing type (1..3), residual kind (0..1) and accuracy
+// measure (0..2), one distinct code each.  It keys the captured iteration graph (the
+// three are baked into its kernel arguments) and the continuation of a converged loop.
+int loop_kind(const adscf_ctx* c) { return (c->mixing * 2 + c->resid_kind) * 4 + c->measure; }
+
 template <typename T>
 std::vector<const void*> graph_key(adscf_ctx* c) {
   Engine<T>& En = eng<T>(c);
@@ -1072,16 +1110,9 @@ template <typename T>
 void build_graph(adscf_ctx* c) {
   ensure_tables<T>(c);
   std::vector<const void*> key = graph_key<T>(c);
-  const int mode = (c->mixing * 2 + c->resid_kind) * 2 + c->measure;
+  const int mode = loop_kind(c);
   if (c->graph && c->graph_prec == c->prec && c->graph_mode == mode && key == c->graph_key) return;
-  if (c->graph) {
-    (void)hipGraphExecDestroy(c->graph);
-    c->graph = nullptr;
-  }
-  if (c->graph_src) {
-    (void)hipGraphDestroy(c->graph_src);
-    c->graph_src = nullptr;
-  }
+  c->keep.retire(c->graph_src, c->graph);
   HIPCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   try {
     enqueue_iteration<T>(c);
@@ -1105,7 +1136,16 @@ int scf_iterate_impl(adscf_ctx* c, int niter, double tol, int* iters, double* ac
   HIPCK(hipMemcpyAsync(&c->st.p->tol, &tol, sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
-  const int loop = (c->mixing * 2 + c->resid_kind) * 4 + c->measure;
+  const int loop = loop_kind(c);
+  if (c->last_loop >= 0 && loop != c->last_loop) {
+    // another loop than the last call's: it starts at iteration 1 on the current field, as
+    // each saddle-point or conjugate-field loop of the reference starts its count
+    hipLaunchKernelGGL(k_scf_restart, dim3(1), dim3(64), 0, c->stream, c->st.p);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+  }
+  c->last_loop = loop;
   if (st.done && st.iter > 0 && niter > 0) {
     // an earlier call stopped at convergence: the same loop continues under the current
     // tolerance from the mixing step of its last iteration; another loop starts with a
@@ -1297,26 +1337,56 @@ int epd_step_impl(adscf_ctx* c, double dt, double thresh, int itermax, int* iter
   return conjugate_impl<T>(c, thresh, itermax, 1, iters, &acc);  // :663
 }
 
+// the schedule and buffers a captured sweep bakes in
+template <typename T>
+std::string sweep_key(adscf_ctx* c) {
+  std::string k = std::to_string(sizeof(T)) + "/" + std::to_string(c->mid_axis) + "/" + std::to_string(c->cd.ns);
+  for (int b = 0; b < c->cd.nblocks; ++b) k += "/" + std::to_string(c->cd.type[b]) + ":" + std::to_string(c->cd.seg[b]);
+  for (const AxisTables& a : c->ax)
+    k += "/" + std::to_string(a.N1) + "," + std::to_string(a.N2) + "," + std::to_string(a.waves) + "," +
+         std::to_string(a.L);
+  char buf[32];
+  for (const void* p : graph_key<T>(c)) {
+    snprintf(buf, sizeof buf, "/%p", p);
+    k += buf;
+  }
+  return k;
+}
+
 template <typename T>
 double sweep_timing(adscf_ctx* c, int reps) {
-  // time the propagator sweep alone (the line passes of one SCF iteration)
-  Engine<T>& En = eng<T>(c);
-  auto passes = build_sweep<T>(c, c->cd, En.E.p, En.h1.p, En.h2.p, c->d1.p, c->d2.p, nullptr);
-  hipGraph_t g = nullptr;
-  hipGraphExec_t ge = nullptr;
-  HIPCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  for (auto& p : passes) launch_pass<T>(c, p);
-  HIPCK(hipStreamEndCapture(c->stream, &g));
-  HIPCK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-  HIPCK(hipGraphLaunch(ge, c->stream));
+  // time the propagator sweep alone (the line passes of one SCF iteration); the captured
+  // sweep of each schedule is kept and replayed by later calls
+  const std::string key = sweep_key<T>(c);
+  GraphPair* gp = c->keep.find(key);
+  if (!gp) {
+    Engine<T>& En = eng<T>(c);
+    auto passes = build_sweep<T>(c, c->cd, En.E.p, En.h1.p, En.h2.p, c->d1.p, c->d2.p, nullptr);
+    GraphPair np;
+    HIPCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      for (auto& p : passes) launch_pass<T>(c, p);
+    } catch (...) {
+      (void)hipStreamEndCapture(c->stream, &np.g);
+      if (np.g) (void)hipGraphDestroy(np.g);
+      throw;
+    }
+    HIPCK(hipStreamEndCapture(c->stream, &np.g));
+    const hipError_t e = hipGraphInstantiate(&np.e, np.g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(np.g);
+      throw HipError{e, "hipGraphInstantiate(sweep)", __LINE__};
+    }
+    c->keep.sweeps.push_back({key, np});
+    gp = &c->keep.sweeps.back().second;
+  }
+  HIPCK(hipGraphLaunch(gp->e, c->stream));
   HIPCK(hipEventRecord(c->ev0, c->stream));
-  for (int r = 0; r < reps; ++r) HIPCK(hipGraphLaunch(ge, c->stream));
+  for (int r = 0; r < reps; ++r) HIPCK(hipGraphLaunch(gp->e, c->stream));
   HIPCK(hipEventRecord(c->ev1, c->stream));
   HIPCK(hipEventSynchronize(c->ev1));
   float ms = 0.f;
   HIPCK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-  (void)hipGraphExecDestroy(ge);
-  (void)hipGraphDestroy(g);
   return ms / reps;
 }
 
@@ -1374,14 +1444,7 @@ std::string autotune_impl(adscf_ctx* c, int reps) {
     snprintf(line, sizeof line, "middle axis chosen: %d (%.4f ms)\n", best_mid, best_ms);
     rep += line;
   }
-  if (c->graph) {
-    (void)hipGraphExecDestroy(c->graph);
-    c->graph = nullptr;
-  }
-  if (c->graph_src) {
-    (void)hipGraphDestroy(c->graph_src);
-    c->graph_src = nullptr;
-  }
+  c->keep.retire(c->graph_src, c->graph);
   return rep;
 }
 
@@ -1865,6 +1928,7 @@ struct adscf_batch {
   bool tables_ok = false;
   hipGraph_t graph_src = nullptr;
   hipGraphExec_t graph = nullptr;
+  GraphKeep keep;  // retired and timing graphs, destroyed at adscf_batch_destroy
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms_iter = 0;
   std::string err;
@@ -2022,12 +2086,7 @@ void batch_enqueue_lanes(adscf_batch* b, bool post) {
   }
 }
 
-void batch_drop_graph(adscf_batch* b) {
-  if (b->graph) (void)hipGraphExecDestroy(b->graph);
-  if (b->graph_src) (void)hipGraphDestroy(b->graph_src);
-  b->graph = nullptr;
-  b->graph_src = nullptr;
-}
+void batch_drop_graph(adscf_batch* b) { b->keep.retire(b->graph_src, b->graph); }
 
 template <typename T>
 void batch_build_graph(adscf_batch* b) {
@@ -2049,27 +2108,36 @@ void batch_build_graph(adscf_batch* b) {
 template <typename T>
 double batch_sweep_timing(adscf_batch* b, int reps) {
   batch_build_tables<T>(b);
-  hipGraph_t g = nullptr;
-  hipGraphExec_t ge = nullptr;
-  HIPCK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  try {
-    batch_enqueue_lanes<T>(b, false);
-  } catch (...) {
-    (void)hipStreamEndCapture(b->stream, &g);
-    if (g) (void)hipGraphDestroy(g);
-    throw;
+  // the captured sweep of each set of schedules is kept and replayed by later calls
+  std::string key = std::to_string(sizeof(T));
+  for (adscf_ctx* c : b->inst) key += "|" + sweep_key<T>(c);
+  GraphPair* gp = b->keep.find(key);
+  if (!gp) {
+    GraphPair np;
+    HIPCK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      batch_enqueue_lanes<T>(b, false);
+    } catch (...) {
+      (void)hipStreamEndCapture(b->stream, &np.g);
+      if (np.g) (void)hipGraphDestroy(np.g);
+      throw;
+    }
+    HIPCK(hipStreamEndCapture(b->stream, &np.g));
+    const hipError_t e = hipGraphInstantiate(&np.e, np.g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(np.g);
+      throw HipError{e, "hipGraphInstantiate(batch sweep)", __LINE__};
+    }
+    b->keep.sweeps.push_back({key, np});
+    gp = &b->keep.sweeps.back().second;
   }
-  HIPCK(hipStreamEndCapture(b->stream, &g));
-  HIPCK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-  HIPCK(hipGraphLaunch(ge, b->stream));
+  HIPCK(hipGraphLaunch(gp->e, b->stream));
   HIPCK(hipEventRecord(b->ev0, b->stream));
-  for (int r = 0; r < reps; ++r) HIPCK(hipGraphLaunch(ge, b->stream));
+  for (int r = 0; r < reps; ++r) HIPCK(hipGraphLaunch(gp->e, b->stream));
   HIPCK(hipEventRecord(b->ev1, b->stream));
   HIPCK(hipEventSynchronize(b->ev1));
   float ms = 0.f;
   HIPCK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-  (void)hipGraphExecDestroy(ge);
-  (void)hipGraphDestroy(g);
   return ms / reps;
 }
 
@@ -2155,9 +2223,9 @@ int adscf_init(adscf_ctx** out, int nx, int ny, int nz, int ntypes, double lx, d
 
 int adscf_finalize(adscf_ctx* c) {
   if (!c) return ADSCF_ERR_ARG;
-  if (c->graph) (void)hipGraphExecDestroy(c->graph);
-  if (c->graph_src) (void)hipGraphDestroy(c->graph_src);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->keep.retire(c->graph_src, c->graph);
+  c->keep.release();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   hipStream_t s = c->stream;
@@ -2666,6 +2734,7 @@ int adscf_scf_set_field(adscf_ctx* c, const double* field) {
     else scf_set_field_impl<float>(c, field);
     c->field_set = true;
     c->phi_fresh = false;
+    c->last_loop = -1;
     return ADSCF_OK;
   });
 }
@@ -2677,6 +2746,8 @@ int adscf_scf_iterate(adscf_ctx* c, int niter, double tol, int* iters, double* a
       return ADSCF_ERR_STATE;
     }
     if (niter < 0) throw ArgError{"niter must be >= 0"};
+    if (c->user_measure == 2 && c->bl_cd.size() > 1)
+      throw ArgError{"measure 2 (change of the densities) is not available for a blend of several polymers"};
     c->resid_kind = c->user_kind;
     c->measure = c->user_measure;
     c->mixing = c->user_mixing;
@@ -3138,8 +3209,9 @@ int adscf_batch_create(adscf_batch** out, int ninst, const int* grid, const doub
 
 int adscf_batch_destroy(adscf_batch* b) {
   if (!b) return ADSCF_ERR_ARG;
-  batch_drop_graph(b);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  b->keep.release();
   for (adscf_ctx* c : b->inst) {
     c->stream = nullptr;
     adscf_finalize(c);

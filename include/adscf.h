@@ -148,8 +148,12 @@ int adscf_scf_set_field(adscf_ctx* ctx, const double* field);
 /* run up to niter iterations; stops early when .not.(accuracy > tol).  iters_total is the
  * reference's loop index so far, accuracy the last max|dfield|, converged 0/1.  After a call
  * that stopped at convergence, a call with a smaller tol continues the same loop (the mixing
- * step of the last evaluation, then further iterations) exactly as one call with that tol;
- * a call with another residual, measure or mixing type starts with a fresh evaluation. */
+ * step of the last evaluation, then further iterations) exactly as one call with that tol.
+ * A call with another residual, measure or mixing type than the previous call (or a
+ * conjugate-field solve in between) starts a new loop on the current field: the count
+ * restarts at 1, as each loop of the reference does (Mixing_Lambda's first step,
+ * Anderson's lam00 steps up to iteration m), and the older history entries are never read
+ * (Anderson first solves at iteration m+1, when every entry is the new loop's). */
 int adscf_scf_iterate(adscf_ctx* ctx, int niter, double tol, int* iters_total, double* accuracy, int* converged);
 /* recompute densities of the current field (main_multiblock.f90:319) without mixing */
 int adscf_scf_evaluate(adscf_ctx* ctx);
@@ -171,7 +175,9 @@ int adscf_scf_timing(adscf_ctx* ctx, double* ms_per_iter, double* ms_sweep);
  * uploaded when non-NULL.  measure 0: accuracy = maxval(abs(dfield)) (main_multiblock.f90:780);
  * 1: sum(abs(dfield))*dvol (Main_Blend/main_blend.f90:567); 2: sum((density-newdensity)**2),
  * the change of the densities since the previous evaluation (Main_Diblock_Melt/main_diblock.f90:387),
- * the first iteration comparing with adscf_scf_set_density.  Default: kind 0, measure 0. */
+ * the first iteration comparing with adscf_scf_set_density; not available for a blend of
+ * several homopolymers (adscf_scf_iterate returns ADSCF_ERR_ARG: the densities are summed
+ * over the polymers in place).  Default: kind 0, measure 0. */
 int adscf_scf_set_residual(adscf_ctx* ctx, int kind, int measure, const double* target);
 /* the densities (nx,ny,nz,ntypes) the next evaluation's measure 2 compares with (the driver's
  * density before Calculate_Saddle_Point, main_diblock.f90:398-412); after adscf_scf_set_field */

@@ -67,3 +67,45 @@ def test_metric_fp32_error(setup):
     ed = np.max(np.abs(r32.density - r64.density)) / np.max(r64.density)
     print(f"64^3 x 200 FP32-mixed vs FP64: Q {eq:.2e}, phi {ed:.2e}")
     assert eq < 1e-5 and ed < 1e-4
+
+
+_ORACLE = {}  # the oracle's iteration, shared by both precisions
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_metric_iteration_against_oracle(setup, orc, prec):
+    """One SCF iteration of the metric workload at full size (64^3 x 200, blocks 60/140, chiN
+    20, kappa 25, Anderson m = 20, lam_mix = 0.5) from the seeded cylinder field against the
+    oracle's Find_SCF_Saddle_Point iteration (main_multiblock.f90:648-700): densities, Q,
+    F (Calculate_Energy, main_multiblock.f90:548-557) and the mixed field to 1e-10 in FP64;
+    FP32-mixed to its stated bounds (densities 2e-5, Q 1e-5, F 1e-7, mixed field 1e-7)."""
+    amd, w = setup
+    ds = 1.0 / NS
+    chi = [[0.0, 20.0], [20.0, 0.0]]
+    g = orc.Grid(N, N, N, L, L, L, ds)
+    fh = orc.FH(25.0, chi, (1.0, 1.0))
+    if "ref" not in _ORACLE:
+        print("metric iteration: oracle", flush=True)
+        _ORACLE["ref"] = orc.scf_anderson(g, [1, 2], SEGS, fh, w, 20, lam=0.5, tol=-1.0, maxit=1)
+    ref = _ORACLE["ref"]
+    _, dvol = g.volume()
+    F_ref = ref["energy"] - dvol * float(np.sum(ref["density"] * w)) + orc.interaction_energy(g, fh, ref["density"])
+    P = amd.FP64 if prec == "fp64" else amd.FP32
+    with amd.Context(N, N, N, L, L, L, ds, precision=P) as ctx:
+        ctx.set_interactions(25.0, chi, [1.0, 1.0])
+        ctx.scf_setup([1, 2], SEGS, ensemble=1, rho=1.0, mixing_dim=20, lam_mix=0.5)
+        ctx.scf_set_field(w)
+        ctx.scf_evaluate()
+        en = ctx.scf_energy()  # Q and F of the start field, the evaluation the iteration makes
+        st = ctx.scf_iterate(1, tol=-1.0)
+        field, den = ctx.scf_get()  # the mixed field and the densities of that evaluation
+    e = dict(phi=float(np.max(np.abs(den - ref["density"])) / np.max(np.abs(ref["density"]))),
+             Q=abs(en["Qsum"] - ref["Qsum"]) / ref["Qsum"],
+             F=abs(en["F"] - F_ref) / abs(F_ref),
+             field=float(np.max(np.abs(field - ref["field"])) / np.max(np.abs(ref["field"]))),
+             accuracy=abs(st.accuracy - ref["accuracy"]) / ref["accuracy"])
+    print(f"metric iteration {prec}: " + " ".join(f"{k} {v:.2e}" for k, v in e.items()))
+    if prec == "fp64":
+        assert all(v < 1e-10 for v in e.values()), e
+    else:
+        assert e["phi"] < 2e-5 and e["Q"] < 1e-5 and e["F"] < 1e-7 and e["field"] < 1e-7 and e["accuracy"] < 1e-4, e
