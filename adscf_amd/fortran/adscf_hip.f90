@@ -271,4 +271,33 @@ contains
     stop 1
   end subroutine adscf_check
 
+  ! The chain set-up shared by the Initialize_* replacements: the announcement line of a
+  ! species, the contour segments of a block of the given length, and the uniform starting
+  ! density of a species carrying `amount` (rho times its contour length) on npoints points.
+  subroutine adscf_announce(species, label, ensemble)
+    character(*), intent(in) :: species
+    integer, intent(in)      :: label, ensemble
+    select case (ensemble)
+    case (1)
+      print*, species, ' ', label, ' is initialized (canonical ensemble).'
+    case (2)
+      print*, species, ' ', label, ' is initialized (grand canonical ensemble).'
+    case default
+      print*, species, ' ', label, ': invalid ensemble', ensemble
+    end select
+  end subroutine adscf_announce
+
+  ! (not pure or elemental: the drivers build with -fno-automatic, which saves every local)
+  function adscf_contour_segments(lengths, step) result(segments)
+    double precision, intent(in) :: lengths(:), step
+    integer :: segments(size(lengths))
+    segments = nint(lengths / step)
+  end function adscf_contour_segments
+
+  double precision function adscf_start_density(amount, box_volume, cell_volume, npoints)
+    double precision, intent(in) :: amount, box_volume, cell_volume
+    integer, intent(in)          :: npoints
+    adscf_start_density = amount * box_volume / cell_volume / dble(npoints)
+  end function adscf_start_density
+
 end module adscf_hip

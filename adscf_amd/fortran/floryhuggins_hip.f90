@@ -23,31 +23,32 @@ module Interactions_FloryHuggins
 
 contains
 
-  ! mod_FloryHuggins.f90:37-66
+  ! mod_FloryHuggins.f90:37-66.  input_interactions holds, one value per record: kappa, a
+  ! separator record, chi(i,j) for i < j in row order (i outer), a separator record, and the
+  ! monomer volumes.  The matrix is stored symmetric with a zero diagonal.
   subroutine Initialize_Interactions
-    character*30 :: file_name
-    integer      :: io_error
-    integer      :: component, component2
-    file_name = 'input_interactions'
-    open(unit=20, file=file_name, status='old', action='read', iostat=io_error)
-    if (io_error == 0) then
-      read(20,*) kappa
-      read(20,*)
-      do component = 1, components
-        chi(component,component) = 0.0d0
-        do component2 = component+1, components
-          read(20,*) chi(component,component2)
-          chi(component2,component) = chi(component,component2)
+    integer :: u, ios, i, j
+    kappa = 0.0d0
+    chi = 0.0d0
+    monomer_volume = 0.0d0
+    open(newunit=u, file='input_interactions', status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+      print*, 'Error', ios, ' while trying to open input_interactions'
+    else
+      read(u,*) kappa
+      read(u,*)
+      do i = 1, components - 1
+        do j = i + 1, components
+          read(u,*) chi(i,j)
         end do
       end do
-      read(20,*)
-      do component = 1, components
-        read(20,*) monomer_volume(component)
+      chi = chi + transpose(chi)
+      read(u,*)
+      do i = 1, components
+        read(u,*) monomer_volume(i)
       end do
-    else
-      print*, 'Error', io_error, ' while trying to open', file_name
+      close(u)
     end if
-    close(unit=20)
     call adscf_check(adscf_set_interactions(adscf_context, kappa, chi, monomer_volume), 'Initialize_Interactions')
   end subroutine Initialize_Interactions
 

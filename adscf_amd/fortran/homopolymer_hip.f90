@@ -37,23 +37,15 @@ contains
     integer, intent(in)           :: label, ensemble, monomer_type
     double precision, intent(in)  :: length, rho, mu
     type(homopolymer), intent(out):: polymer
-    character*15                  :: string_ensemble
-    if (ensemble .eq. 1) then
-      string_ensemble = 'canonical'
-    else if (ensemble .eq. 2) then
-      string_ensemble = 'grand canonical'
-    else
-      print*, 'Invalid Ensemble'
-    end if
     polymer%label        = label
-    polymer%length       = length
     polymer%monomer_type = monomer_type
+    polymer%length       = length
+    polymer%segments     = sum(adscf_contour_segments([length], ds))
     polymer%ensemble     = ensemble
     polymer%rho          = rho
     polymer%mu           = mu
-    polymer%segments     = nint(polymer%length/ds)
-    print*, 'Homopolymer ', polymer%label, ' is initialized (', string_ensemble, ' ensemble).'
-    polymer%density = polymer%rho * polymer%segments*ds * volume / dvol / dble(gridx*gridy*gridz)
+    call adscf_announce('Homopolymer', label, ensemble)
+    polymer%density = adscf_start_density(rho * polymer%segments * ds, volume, dvol, gridx*gridy*gridz)
   end subroutine Initialize_Homopolymer
 
   ! mod_homopolymer.f90:90-141
@@ -103,21 +95,13 @@ contains
     integer, intent(in)          :: label, ensemble, monomer_type
     double precision, intent(in) :: rho, mu
     type(monomeric_solvent), intent(out):: solvent
-    character*15                  :: string_ensemble
-    if (ensemble .eq. 1) then
-      string_ensemble = 'canonical'
-    else if (ensemble .eq. 2) then
-      string_ensemble = 'grand canonical'
-    else
-      print*, 'Invalid Ensemble'
-    end if
     solvent%label        = label
     solvent%monomer_type = monomer_type
     solvent%ensemble     = ensemble
     solvent%rho          = rho
     solvent%mu           = mu
-    print*, 'Solvent ', solvent%label, ' is initialized (', string_ensemble, ' ensemble).'
-    solvent%density = solvent%rho * volume / dvol / dble(gridx*gridy*gridz)
+    call adscf_announce('Solvent', label, ensemble)
+    solvent%density = adscf_start_density(rho, volume, dvol, gridx*gridy*gridz)
   end subroutine Initialize_Solvent
 
   ! mod_solvent.f90:85-108

@@ -40,28 +40,20 @@ contains
     double precision, intent(in)                          :: rho, mu
     double precision, dimension(block_number), intent(in) :: block_length
     integer, dimension(block_number), intent(in)          :: monomer_type
-    type(diblock_copolymer), intent(out)                  :: copolymer
-    character*15 :: string_ensemble
-    integer      :: block
-    if (ensemble .eq. 1) then
-      string_ensemble = 'canonical'
-    else if (ensemble .eq. 2) then
-      string_ensemble = 'grand canonical'
-    else
-      print*, 'Invalid Ensemble'
-    end if
+    type(diblock_copolymer), intent(out)                 :: copolymer
+    integer :: b
     copolymer%label          = label
-    copolymer%block_length   = block_length
     copolymer%monomer_type   = monomer_type
+    copolymer%block_length   = block_length
+    copolymer%block_segments = adscf_contour_segments(block_length, ds)
+    copolymer%segments       = sum(copolymer%block_segments)
     copolymer%ensemble       = ensemble
     copolymer%rho            = rho
     copolymer%mu             = mu
-    copolymer%block_segments = nint(copolymer%block_length/ds)
-    copolymer%segments       = sum(copolymer%block_segments)
-    print*, 'Diblock Copolymer ', copolymer%label, ' is initialized (', string_ensemble, ' ensemble).'
-    do block = 1, block_number
-      copolymer%density(:,:,:,block) = copolymer%rho * copolymer%block_segments(block)*ds &
-                                       * volume / dvol / dble(gridx*gridy*gridz)
+    call adscf_announce('Diblock Copolymer', label, ensemble)
+    do b = 1, block_number
+      copolymer%density(:,:,:,b) = adscf_start_density(rho * copolymer%block_segments(b) * ds, volume, dvol, &
+                                                       gridx*gridy*gridz)
     end do
   end subroutine Initialize_Diblock
 
@@ -121,27 +113,19 @@ contains
     double precision, dimension(block_number), intent(in) :: block_length
     integer, dimension(block_number), intent(in)          :: monomer_type
     type(triblock_copolymer), intent(out)                 :: copolymer
-    character*15 :: string_ensemble
-    integer      :: block
-    if (ensemble .eq. 1) then
-      string_ensemble = 'canonical'
-    else if (ensemble .eq. 2) then
-      string_ensemble = 'grand canonical'
-    else
-      print*, 'Invalid Ensemble'
-    end if
+    integer :: b
     copolymer%label          = label
-    copolymer%block_length   = block_length
     copolymer%monomer_type   = monomer_type
+    copolymer%block_length   = block_length
+    copolymer%block_segments = adscf_contour_segments(block_length, ds)
+    copolymer%segments       = sum(copolymer%block_segments)
     copolymer%ensemble       = ensemble
     copolymer%rho            = rho
     copolymer%mu             = mu
-    copolymer%block_segments = nint(copolymer%block_length/ds)
-    copolymer%segments       = sum(copolymer%block_segments)
-    print*, 'Triblock Copolymer ', copolymer%label, ' is initialized (', string_ensemble, ' ensemble).'
-    do block = 1, block_number
-      copolymer%density(:,:,:,block) = copolymer%rho * copolymer%block_segments(block)*ds &
-                                       * volume / dvol / dble(gridx*gridy*gridz)
+    call adscf_announce('Triblock Copolymer', label, ensemble)
+    do b = 1, block_number
+      copolymer%density(:,:,:,b) = adscf_start_density(rho * copolymer%block_segments(b) * ds, volume, dvol, &
+                                                       gridx*gridy*gridz)
     end do
   end subroutine Initialize_Triblock
 
@@ -200,33 +184,22 @@ contains
     double precision, dimension(:), intent(in) :: block_length
     integer, dimension(:), intent(in)          :: monomer_type
     type(multiblock_copolymer), intent(out)    :: copolymer
-    character*15 :: string_ensemble
-    integer      :: block, iAllocStatus
-    allocate(copolymer%monomer_type(nblocks), stat=iAllocStatus)
-    allocate(copolymer%block_length(nblocks), stat=iAllocStatus)
-    allocate(copolymer%block_segments(nblocks), stat=iAllocStatus)
-    allocate(copolymer%distribution(gridx,gridy,gridz,nblocks), stat=iAllocStatus)
-    allocate(copolymer%density(gridx,gridy,gridz,nblocks), stat=iAllocStatus)
-    if (ensemble .eq. 1) then
-      string_ensemble = 'canonical'
-    else if (ensemble .eq. 2) then
-      string_ensemble = 'grand canonical'
-    else
-      print*, 'Invalid Ensemble'
-    end if
-    copolymer%label          = label
-    copolymer%block_number   = nblocks
-    copolymer%monomer_type   = monomer_type
-    copolymer%block_length   = block_length
-    copolymer%ensemble       = ensemble
-    copolymer%rho            = rho
-    copolymer%mu             = mu
-    copolymer%block_segments = nint(copolymer%block_length/ds)
+    integer :: b
+    copolymer%label        = label
+    copolymer%block_number = nblocks
+    copolymer%ensemble     = ensemble
+    copolymer%rho          = rho
+    copolymer%mu           = mu
+    ! allocation on assignment sizes the per-block arrays
+    copolymer%monomer_type   = monomer_type(1:nblocks)
+    copolymer%block_length   = block_length(1:nblocks)
+    copolymer%block_segments = adscf_contour_segments(copolymer%block_length, ds)
     copolymer%segments       = sum(copolymer%block_segments)
-    print*, 'Multiblock Copolymer ', copolymer%label, ' is initialized (', string_ensemble, ' ensemble).'
-    do block = 1, nblocks
-      copolymer%density(:,:,:,block) = copolymer%rho * copolymer%block_segments(block)*ds &
-                                       * volume / dvol / dble(gridx*gridy*gridz)
+    allocate(copolymer%distribution(gridx,gridy,gridz,nblocks), copolymer%density(gridx,gridy,gridz,nblocks))
+    call adscf_announce('Multiblock Copolymer', label, ensemble)
+    do b = 1, nblocks
+      copolymer%density(:,:,:,b) = adscf_start_density(rho * copolymer%block_segments(b) * ds, volume, dvol, &
+                                                       gridx*gridy*gridz)
     end do
   end subroutine Initialize_Multiblock
 

@@ -72,37 +72,88 @@ def run_gpu(adscf_amd, fields, precision, steps, warmup, w0, hipfft=True, tune=F
     return ctx, out
 
 
+def _metric_ctx(adscf_amd, fields, prec, w0):
+    ctx = adscf_amd.Context(N, N, N, L, L, L, 1.0 / NS, precision=prec)
+    ctx.set_interactions(KAPPA, [[0.0, CHIN], [CHIN, 0.0]], [1.0, 1.0])
+    ctx.scf_setup([1, 2], fields.block_segments([FA, 1 - FA], NS), ensemble=1, rho=1.0, mixing_dim=M, lam_mix=LAM)
+    ctx.scf_set_field(w0)
+    return ctx
+
+
+def iterations_to_dF(adscf_amd, fields, prec, w0, F_ref, max_iter, thresh=1e-6, chunk=5):
+    """First iteration count k after which |F per chain - F_ref| < thresh holds at every probe
+    (every `chunk` iterations, for 3 probes running); then a fresh timed run of exactly k
+    iterations (graph captured beforehand).  Returns dict(iters, seconds, dF) or None."""
+    k = None
+    below = 0
+    with _metric_ctx(adscf_amd, fields, prec, w0) as ctx:
+        it = 0
+        while it < max_iter:
+            st = ctx.scf_iterate(chunk, tol=-1.0)
+            it = st.iters
+            dF = abs(ctx.scf_energy()["F_per_chain"] - F_ref)
+            if dF < thresh:
+                below += 1
+                if below == 1:
+                    k, dF_k = it, dF
+                if below == 3:
+                    break
+            else:
+                below, k = 0, None
+    if k is None or below < 3:
+        return None
+    with _metric_ctx(adscf_amd, fields, prec, w0) as ctx:
+        ctx.scf_iterate(0, tol=-1.0)  # captures the iteration graph outside the timed region
+        t0 = time.perf_counter()
+        st = ctx.scf_iterate(k, tol=-1.0)
+        t = time.perf_counter() - t0
+        dF = abs(ctx.scf_energy()["F_per_chain"] - F_ref)
+    return dict(iters=k, seconds=t, iters_per_s=k / t, dF_per_chain=dF)
+
+
 def converge(adscf_amd, fields, w0, max_iter):
-    """FP64 run and FP32-mixed run (FP64 refinement once max|dfield| < 1e-4) to tol 1e-8."""
-    segs = fields.block_segments([FA, 1 - FA], NS)
+    """The FP64 saddle point (max|dfield| < 1e-8) gives the reference F per chain.  Then, per
+    path, the iterations and seconds until |dF| < 1e-6 per chain against it: plain FP64, and
+    FP32-mixed (FP32 propagators and FFTs, FP64 Q/F/phi sums and Anderson) without refinement.
+    Last, FP32-mixed with FP64 refinement (switch once max|dfield| < 1e-4 or the FP32 residual
+    stops improving) to 1e-8, as the full solve."""
     res = {}
+    with _metric_ctx(adscf_amd, fields, adscf_amd.FP64, w0) as ctx:
+        ctx.scf_iterate(0, tol=1e-8)
+        t0 = time.perf_counter()
+        st = ctx.scf_iterate(max_iter, tol=1e-8)
+        t = time.perf_counter() - t0
+        F_ref = ctx.scf_energy()["F_per_chain"]
+        res["fp64"] = dict(iters=st.iters, accuracy=st.accuracy, converged=st.converged, F_per_chain=F_ref,
+                           seconds=t)
+    dF = {"threshold": 1e-6, "F_ref_per_chain": F_ref}
     for name, prec in (("fp64", adscf_amd.FP64), ("fp32mixed", adscf_amd.FP32)):
-        with adscf_amd.Context(N, N, N, L, L, L, 1.0 / NS, precision=prec) as ctx:
-            ctx.set_interactions(KAPPA, [[0.0, CHIN], [CHIN, 0.0]], [1.0, 1.0])
-            ctx.scf_setup([1, 2], segs, ensemble=1, rho=1.0, mixing_dim=M, lam_mix=LAM)
-            ctx.scf_set_field(w0)
-            t0 = time.perf_counter()
-            if prec == adscf_amd.FP32:
-                # FP32 phase until max|dfield| < 1e-4 or it stops improving (FP32 rounding of
-                # phi, amplified by kappa, sets a residual floor near 3e-5 here), then the FP64
-                # refinement continues the same loop and history
-                best, it32 = float("inf"), 0
-                while it32 < max_iter:
-                    st = ctx.scf_iterate(50, tol=1e-4)
-                    it32 = st.iters
-                    r = ctx.scf_residuals()[-50:]
-                    if st.converged or r.min() > 0.7 * best:
-                        break
-                    best = min(best, r.min())
-                res["fp32_residual_at_switch"] = float(st.accuracy)
-                ctx.set_precision(adscf_amd.FP64)
-            st = ctx.scf_iterate(max_iter, tol=1e-8)
-            t = time.perf_counter() - t0
-            en = ctx.scf_energy()
-            res[name] = dict(iters=st.iters, accuracy=st.accuracy, converged=st.converged,
-                             F_per_chain=en["F_per_chain"], seconds=t)
-            if prec == adscf_amd.FP32:
-                res[name]["fp32_iters"] = it32
+        dF[name] = iterations_to_dF(adscf_amd, fields, prec, w0, F_ref, max_iter)
+    if dF["fp64"] and dF["fp32mixed"]:
+        dF["speedup_fp32mixed_vs_fp64_time"] = dF["fp64"]["seconds"] / dF["fp32mixed"]["seconds"]
+        dF["speedup_fp32mixed_vs_fp64_iters_per_s"] = dF["fp32mixed"]["iters_per_s"] / dF["fp64"]["iters_per_s"]
+    res["to_dF_1e-6"] = dF
+    with _metric_ctx(adscf_amd, fields, adscf_amd.FP32, w0) as ctx:
+        ctx.scf_iterate(0, tol=1e-4)
+        t0 = time.perf_counter()
+        # FP32 phase until max|dfield| < 1e-4 or it stops improving (FP32 rounding of phi,
+        # amplified by kappa, sets a residual floor near 3e-5 here), then the FP64 refinement
+        # continues the same loop and history
+        best, it32 = float("inf"), 0
+        while it32 < max_iter:
+            st = ctx.scf_iterate(50, tol=1e-4)
+            it32 = st.iters
+            r = ctx.scf_residuals()[-50:]
+            if st.converged or r.min() > 0.7 * best:
+                break
+            best = min(best, r.min())
+        res["fp32_residual_at_switch"] = float(st.accuracy)
+        ctx.set_precision(adscf_amd.FP64)
+        st = ctx.scf_iterate(max_iter, tol=1e-8)
+        t = time.perf_counter() - t0
+        en = ctx.scf_energy()
+        res["fp32mixed"] = dict(iters=st.iters, accuracy=st.accuracy, converged=st.converged,
+                                F_per_chain=en["F_per_chain"], seconds=t, fp32_iters=it32)
     res["dF_per_chain"] = abs(res["fp32mixed"]["F_per_chain"] - res["fp64"]["F_per_chain"])
     return res
 
@@ -307,9 +358,51 @@ def run_large(adscf_amd, fields, steps, warmup):
     return out
 
 
+def host_cpu():
+    """CPU model and thread count of this host, and whether the reference's own CPU path
+    could be built here: a Fortran compiler (gfortran, or flang) and FFTW3's libraries."""
+    import ctypes.util
+    import shutil
+
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(),
+            "gfortran": shutil.which("gfortran"), "flang": shutil.which("flang") or
+            (shutil.which("/opt/rocm/llvm/bin/flang") if os.path.exists("/opt/rocm/llvm/bin/flang") else None),
+            "libfftw3": ctypes.util.find_library("fftw3"), "libfftw3f": ctypes.util.find_library("fftw3f")}
+
+
+_ORACLE_CHILD = """
+import sys, time
+import numpy as np
+sys.path.insert(0, %(root)r)
+import oracle
+w0 = np.load(%(path)r)
+g = oracle.Grid(%(n)d, %(n)d, %(n)d, %(L)r, %(L)r, %(L)r, %(ds)r)
+fh = oracle.FH(%(kappa)r, [[0.0, %(chin)r], [%(chin)r, 0.0]], [1.0, 1.0])
+t0 = time.perf_counter()
+oracle.scf_anderson(g, [1, 2], %(segs)r, fh, w0, %(m)d, lam=%(lam)r, tol=-1.0, maxit=1)
+print(time.perf_counter() - t0)
+"""
+
+
 def cpu_baseline(adscf_amd, fields, w0):
-    """The CPU FP64 oracle (single thread) on one SCF iteration of the same workload; its
-    result also checks the GPU's FP64 and FP32-mixed iteration from the same fields."""
+    """The CPU FP64 oracle on one SCF iteration of the same workload (bounded sample): on one
+    thread, and as nproc independent single-threaded processes at once (the all-core
+    throughput of independent SCF problems, the reference's single-threaded path run once
+    per core).  The one-thread result also checks the GPU's FP64 and FP32-mixed iteration from
+    the same fields.  The oracle is a plain C restatement (its FFT a recursive radix-2/3/5 /
+    generic DFT, not FFTW): the reference itself needs FFTW3, which this host lacks."""
+    import subprocess
+    import tempfile
+
     import numpy as np
     import oracle
 
@@ -319,22 +412,102 @@ def cpu_baseline(adscf_amd, fields, w0):
     t0 = time.perf_counter()
     ref = oracle.scf_anderson(g, [1, 2], segs, fh, w0, M, lam=LAM, tol=-1.0, maxit=1)
     t = time.perf_counter() - t0
+    host = host_cpu()
+    nproc = int(host["nproc"] or 1)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "w0.npy")
+        np.save(path, np.ascontiguousarray(w0))
+        code = _ORACLE_CHILD % dict(root=ROOT, path=path, n=N, L=L, ds=1.0 / NS, kappa=KAPPA, chin=CHIN,
+                                    segs=list(segs), m=M, lam=LAM)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        t0 = time.perf_counter()
+        procs = [subprocess.Popen([sys.executable, "-c", code], stdout=subprocess.PIPE, text=True, env=env)
+                 for _ in range(nproc)]
+        outs = [p.communicate()[0] for p in procs]
+        wall = time.perf_counter() - t0
+        ok = all(p.returncode == 0 for p in procs)
     parity = {}
     for name, prec in (("fp64", adscf_amd.FP64), ("fp32mixed", adscf_amd.FP32)):
-        with adscf_amd.Context(N, N, N, L, L, L, 1.0 / NS, precision=prec) as ctx:
-            ctx.set_interactions(KAPPA, [[0.0, CHIN], [CHIN, 0.0]], [1.0, 1.0])
-            ctx.scf_setup([1, 2], segs, ensemble=1, rho=1.0, mixing_dim=M, lam_mix=LAM)
-            ctx.scf_set_field(w0)
+        with _metric_ctx(adscf_amd, fields, prec, w0) as ctx:
             st = ctx.scf_iterate(1, tol=-1.0)
             field, den = ctx.scf_get()
         scale = lambda a: float(np.max(np.abs(a)))
         parity[name] = {"phi_max_rel": float(np.max(np.abs(den - ref["density"]))) / scale(ref["density"]),
                         "field_max_rel": float(np.max(np.abs(field - ref["field"]))) / scale(ref["field"]),
                         "residual_rel": abs(st.accuracy - ref["accuracy"]) / ref["accuracy"]}
+    all_core = None
+    if ok:
+        per = [float(o.strip().splitlines()[-1]) for o in outs]
+        all_core = {"value": nproc / wall, "unit": "SCF iters/s", "cores": nproc, "processes": nproc,
+                    "wall_seconds": wall, "seconds_per_process_median": float(np.median(per))}
     return dict(value=1.0 / t, unit="SCF iters/s", cores=1, kind="port",
                 sample="1 SCF iteration (2x200 propagator steps, contour integral, mean fields, mixing) of the "
-                       "64^3 workload by the single-threaded C FP64 oracle (oracle/scft_oracle.c)",
-                seconds=t, gpu_vs_oracle_same_iteration=parity)
+                       "64^3 workload by the single-threaded C FP64 oracle (oracle/scft_oracle.c; recursive "
+                       "mixed-radix DFT, not FFTW)",
+                seconds=t, host=host, all_cores=all_core,
+                reference_buildable=bool((host["gfortran"] or host["flang"]) and host["libfftw3"]),
+                gpu_vs_oracle_same_iteration=parity)
+
+
+def profile_run(adscf_amd, fields, what, iters):
+    """A workload without timing legs, for rocprofv3 kernel traces and counter runs:
+    metric32 / metric64: `iters` SCF iterations of the 64^3 x 200 diblock workload above
+    (the first m + 1 = 21 iterations mix without the Anderson solve; later ones run every
+    kernel of the steady-state iteration); large32 / large64: `iters` iterations of the
+    128^3 x 400 ABA triblock (lamellar seed); batch: `iters` batched iterations of the
+    64-instance sweep (FP32-mixed).  With ADSCF_GRAPH=0 in the environment the iterations
+    are stream launches instead of graph replays."""
+    if what in ("metric32", "metric64"):
+        prec = adscf_amd.FP32 if what == "metric32" else adscf_amd.FP64
+        w0 = fields.cylinder_seed(N, L, CHIN, sigma=0.2, seed=2007)
+        with adscf_amd.Context(N, N, N, L, L, L, 1.0 / NS, precision=prec) as ctx:
+            ctx.set_interactions(KAPPA, [[0.0, CHIN], [CHIN, 0.0]], [1.0, 1.0])
+            ctx.scf_setup([1, 2], fields.block_segments([FA, 1 - FA], NS), ensemble=1, rho=1.0, mixing_dim=M,
+                          lam_mix=LAM)
+            ctx.scf_set_field(w0)
+            st = ctx.scf_iterate(iters, tol=-1.0)
+    elif what in ("large32", "large64"):
+        n, Lb, ns = 128, 6.0, 400
+        prec = adscf_amd.FP32 if what == "large32" else adscf_amd.FP64
+        with adscf_amd.Context(n, n, n, Lb, Lb, Lb, 1.0 / ns, precision=prec) as ctx:
+            ctx.set_interactions(KAPPA, [[0.0, 25.0], [25.0, 0.0]], [1.0, 1.0])
+            ctx.scf_setup([1, 2, 1], fields.block_segments([0.25, 0.5, 0.25], ns), ensemble=1, rho=1.0,
+                          mixing_dim=M, lam_mix=LAM)
+            ctx.scf_set_field(fields.lamellar_seed(n, Lb, 25.0, sigma=0.1, seed=2007))
+            st = ctx.scf_iterate(iters, tol=-1.0)
+    elif what == "batch":
+        inst, w = fields.batched_instances(seed=2007)
+        with adscf_amd.Batch(inst, mixing_dim=M, lam_mix=LAM, precision=adscf_amd.FP32) as b:
+            b.set_fields(w)
+            st = b.iterate(iters, tol=-1.0)
+    else:
+        raise SystemExit("unknown --profile-run workload %r" % what)
+    print("done", what, iters, st, flush=True)
+
+
+def pmc_summary(db, pattern="."):
+    """Per-kernel summary of the hardware counters in a rocprofv3 database (rocpd, SQLite):
+    for every kernel whose name matches `pattern`, the dispatch count and the median and
+    maximum of each counter over its dispatches (summed over hardware instances per dispatch,
+    as rocprofv3 reports *_sum counters).  Turns `rocprofv3 --pmc FETCH_SIZE ...` and
+    `--pmc WRITE_SIZE ...` runs into per-kernel traffic figures."""
+    import collections
+    import re
+    import sqlite3
+    import statistics
+
+    con = sqlite3.connect("file:%s?mode=ro" % db, uri=True)
+    rows = con.execute("select name, dispatch_id, counter_name, counter_value from pmc_events").fetchall()
+    per = collections.defaultdict(lambda: collections.defaultdict(lambda: collections.defaultdict(float)))
+    rx = re.compile(pattern)
+    for name, did, cname, val in rows:
+        if rx.search(name):
+            per[name][cname][did] += val
+    for name in sorted(per):
+        for cname, vals in sorted(per[name].items()):
+            v = list(vals.values())
+            print("%s\tdispatches=%d\tmedian=%.1f\tmax=%.1f\t%s" % (cname, len(v), statistics.median(v), max(v),
+                                                                      name[:100]))
 
 
 def main():
@@ -354,7 +527,16 @@ def main():
                     help="re-tune the line-pass schedules first (the defaults are the tuner's MI355X choice)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 256 MiB device-to-device copy (counter calibration under rocprofv3 --pmc)")
+    ap.add_argument("--profile-run", metavar="WORKLOAD",
+                    help="only run WORKLOAD (metric32, metric64, large32, large64, batch) for --iters iterations, "
+                         "for rocprofv3 traces and counters")
+    ap.add_argument("--iters", type=int, default=30, help="iterations of --profile-run")
+    ap.add_argument("--pmc-summary", metavar="DB", help="summarise the counters of a rocprofv3 database and exit")
+    ap.add_argument("--pmc-regex", default=".", help="kernel-name filter of --pmc-summary")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary, args.pmc_regex)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the SCFT path runs on a single MI355X; "
               "runs on more than one GPU are not supported", file=sys.stderr)
@@ -362,6 +544,11 @@ def main():
     import adscf_amd
     from adscf_amd import fields
 
+    if args.profile_run:
+        if args.copy_probe:
+            copy_probe()
+        profile_run(adscf_amd, fields, args.profile_run, args.iters)
+        return
     w0 = fields.cylinder_seed(N, L, CHIN, sigma=0.2, seed=2007)
     ng = N ** 3
     if args.copy_probe:

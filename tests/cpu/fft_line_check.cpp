@@ -78,7 +78,10 @@ static double check(int n, int L, double tol) {
 }
 
 int main() {
-  int sizes[] = {2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 13, 15, 16, 17, 24, 25, 27, 30, 32, 34, 36, 38, 40, 43, 48, 57, 60, 64, 75, 86, 96, 97, 100, 128, 256, 300};
+  // 49, 77, 91, 121, 143, 169, 221: two generic-radix stages in a row (stage twiddles on a
+  // generic pass that follows another)
+  int sizes[] = {2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 13, 15, 16, 17, 24, 25, 27, 30, 32, 34, 36, 38, 40, 43, 48, 49, 57,
+                 60, 64, 75, 77, 86, 91, 96, 97, 100, 121, 128, 143, 169, 221, 256, 300};
   int bad = 0;
   for (int n : sizes) {
     double e64 = check<double>(n, 3, 0);
