@@ -2108,32 +2108,30 @@ void batch_build_graph(adscf_batch* b) {
 template <typename T>
 double batch_sweep_timing(adscf_batch* b, int reps) {
   batch_build_tables<T>(b);
-  // the captured sweep of each set of schedules is kept and replayed by later calls
-  std::string key = std::to_string(sizeof(T));
-  for (adscf_ctx* c : b->inst) key += "|" + sweep_key<T>(c);
-  GraphPair* gp = b->keep.find(key);
-  if (!gp) {
-    GraphPair np;
-    HIPCK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    try {
-      batch_enqueue_lanes<T>(b, false);
-    } catch (...) {
-      (void)hipStreamEndCapture(b->stream, &np.g);
-      if (np.g) (void)hipGraphDestroy(np.g);
-      throw;
-    }
-    HIPCK(hipStreamEndCapture(b->stream, &np.g));
-    const hipError_t e = hipGraphInstantiate(&np.e, np.g, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-      (void)hipGraphDestroy(np.g);
-      throw HipError{e, "hipGraphInstantiate(batch sweep)", __LINE__};
-    }
-    b->keep.sweeps.push_back({key, np});
-    gp = &b->keep.sweeps.back().second;
+  // Captured afresh on every call: the graph bakes in the batch tables (per-block codes and
+  // per-instance pass arguments), which batch_build_tables rewrites in place for another set
+  // of schedules, so a sweep graph is valid only until the next rebuild.  It is retired (not
+  // destroyed) after timing and released with the batch.
+  GraphPair np;
+  HIPCK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    batch_enqueue_lanes<T>(b, false);
+  } catch (...) {
+    (void)hipStreamEndCapture(b->stream, &np.g);
+    if (np.g) (void)hipGraphDestroy(np.g);
+    throw;
   }
-  HIPCK(hipGraphLaunch(gp->e, b->stream));
+  HIPCK(hipStreamEndCapture(b->stream, &np.g));
+  const hipError_t e = hipGraphInstantiate(&np.e, np.g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(np.g);
+    throw HipError{e, "hipGraphInstantiate(batch sweep)", __LINE__};
+  }
+  hipGraphExec_t ge = np.e;
+  b->keep.retire(np.g, np.e);
+  HIPCK(hipGraphLaunch(ge, b->stream));
   HIPCK(hipEventRecord(b->ev0, b->stream));
-  for (int r = 0; r < reps; ++r) HIPCK(hipGraphLaunch(gp->e, b->stream));
+  for (int r = 0; r < reps; ++r) HIPCK(hipGraphLaunch(ge, b->stream));
   HIPCK(hipEventRecord(b->ev1, b->stream));
   HIPCK(hipEventSynchronize(b->ev1));
   float ms = 0.f;
