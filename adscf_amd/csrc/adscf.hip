@@ -15,6 +15,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -319,6 +320,13 @@ void ensure_tables(adscf_ctx* c) {
   }
 }
 
+// the transform tables of the context's precision, uploaded at set-up so that the
+// no-allocation entry point finds them
+void ensure_current_tables(adscf_ctx* c) {
+  if (c->prec == ADSCF_FP64) ensure_tables<double>(c);
+  else ensure_tables<float>(c);
+}
+
 template <typename T>
 PassArgs<T> base_pass(adscf_ctx* c, int axis) {
   Engine<T>& E = eng<T>(c);
@@ -488,8 +496,8 @@ int step_type(const ChainDesc& cd, int chain, int s) {
 // Build the line passes of a forward + complementary sweep.
 template <typename T>
 std::vector<PassArgs<T>> build_sweep(adscf_ctx* c, const ChainDesc& cd, const T* E, T* h1, T* h2, const int* dexp1,
-                                     const int* dexp2, const int* done) {
-  Engine<T>& En = eng<T>(c);
+                                     const int* dexp2, const int* done, C2<T>* tmp = nullptr) {
+  C2<T>* const tmpp = tmp ? tmp : eng<T>(c).tmp.p;  // packed line buffer between passes
   const long ng = c->ng;
   // Steps alternate the axis order (a, b, c then c, b, a), so the operators of the outer axes
   // of consecutive steps meet and share a pass; the middle axis gets one pass per step.  The
@@ -533,7 +541,7 @@ std::vector<PassArgs<T>> build_sweep(adscf_ctx* c, const ChainDesc& cd, const T*
         PassArgs<T>& p = passes.back();  // last op of step s-1, same axis
         p.nops = 2;
         p.out_kind = OUT_TMP;
-        p.tmp_out = En.tmp.p;
+        p.tmp_out = tmpp;
         p.h1 = h1 + (long)(s - 1) * ng;
         p.h2 = h2 + (long)(s - 1) * ng;
         p.Ea1 = Ep(0, s - 1);
@@ -555,7 +563,7 @@ std::vector<PassArgs<T>> build_sweep(adscf_ctx* c, const ChainDesc& cd, const T*
         p.h2 = h2;  // marks the second chain present for IN_ONES
       } else {
         p.in_kind = IN_TMP;
-        p.tmp_in = En.tmp.p;
+        p.tmp_in = tmpp;
       }
       if (idx + 1 == order.size() && s == ns) {
         p.out_kind = OUT_PLANAR;
@@ -568,7 +576,7 @@ std::vector<PassArgs<T>> build_sweep(adscf_ctx* c, const ChainDesc& cd, const T*
         p.sidx = ns;
       } else {
         p.out_kind = OUT_TMP;
-        p.tmp_out = En.tmp.p;
+        p.tmp_out = tmpp;
       }
       passes.push_back(p);
     }
@@ -1133,7 +1141,8 @@ template <typename T>
 int scf_iterate_impl(adscf_ctx* c, int niter, double tol, int* iters, double* acc, int* conv) {
   if (c->use_graph) build_graph<T>(c);
   ScfState st{};
-  HIPCK(hipMemcpyAsync(&c->st.p->tol, &tol, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_state_update, dim3(1), dim3(64), 0, c->stream, c->st.p, tol, 1, 0);
+  HIPCK(hipGetLastError());
   HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
   const int loop = loop_kind(c);
@@ -1150,8 +1159,8 @@ int scf_iterate_impl(adscf_ctx* c, int niter, double tol, int* iters, double* ac
     // an earlier call stopped at convergence: the same loop continues under the current
     // tolerance from the mixing step of its last iteration; another loop starts with a
     // fresh evaluation of the current field
-    const int zero = 0;
-    HIPCK(hipMemcpyAsync(&c->st.p->done, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_state_update, dim3(1), dim3(64), 0, c->stream, c->st.p, 0.0, 0, 1);
+    HIPCK(hipGetLastError());
     if (loop == c->done_loop) enqueue_mix_tail<T>(c, 0, true);
     HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
@@ -2167,6 +2176,7 @@ int bcheck(adscf_batch* b, adscf_ctx* c, int rc) {
   }
   return rc;
 }
+
 }  // namespace
 
 // ===========================================================================
@@ -2208,6 +2218,7 @@ int adscf_init(adscf_ctx** out, int nx, int ny, int nz, int ntypes, double lx, d
     c->prec = precision;
     c->fh.nt = ntypes;
     reset_axes(c);
+    ensure_current_tables(c);
     return ADSCF_OK;
   });
   if (rc != ADSCF_OK) {
@@ -2241,6 +2252,7 @@ int adscf_set_box(adscf_ctx* c, double lx, double ly, double lz) {
     c->volume = lx * ly * lz;
     c->dvol = c->volume / (double)(c->nx * c->ny * c->nz);
     invalidate_tables(c);
+    ensure_current_tables(c);
     return ADSCF_OK;
   });
 }
@@ -2250,6 +2262,7 @@ int adscf_set_ds(adscf_ctx* c, double ds) {
     if (!(ds > 0)) throw ArgError{"ds must be positive"};
     c->ds = ds;
     invalidate_tables(c);
+    ensure_current_tables(c);
     return ADSCF_OK;
   });
 }
@@ -2259,6 +2272,7 @@ int adscf_set_precision(adscf_ctx* c, int precision) {
     if (precision != ADSCF_FP64 && precision != ADSCF_FP32) throw ArgError{"precision must be 0 (FP64) or 1 (FP32)"};
     if (precision == c->prec) return ADSCF_OK;
     c->prec = precision;
+    ensure_current_tables(c);
     if (c->field_set) {  // continue the same SCF run (history kept) in the new precision
       if (precision == ADSCF_FP64) refresh_E<double>(c);
       else refresh_E<float>(c);
@@ -3260,7 +3274,8 @@ int adscf_batch_iterate(adscf_batch* b, int niter, double tol, int* iters, doubl
     if (b->prec == ADSCF_FP64) batch_build_graph<double>(b);
     else batch_build_graph<float>(b);
     for (adscf_ctx* c : b->inst)
-      HIPCK(hipMemcpyAsync(&c->st.p->tol, &tol, sizeof(double), hipMemcpyHostToDevice, b->stream));
+      hipLaunchKernelGGL(k_state_update, dim3(1), dim3(64), 0, b->stream, c->st.p, tol, 1, 0);
+    HIPCK(hipGetLastError());
     const int n = (int)b->inst.size();
     std::vector<ScfState> st(n);
     for (int i = 0; i < n; ++i)
@@ -3268,11 +3283,11 @@ int adscf_batch_iterate(adscf_batch* b, int niter, double tol, int* iters, doubl
     HIPCK(hipStreamSynchronize(b->stream));
     // instances that stopped at convergence in an earlier call continue their loops under the
     // current tolerance from the mixing step of their last evaluation (as adscf_scf_iterate)
-    const int zero = 0;
     for (int i = 0; i < n && niter > 0; ++i) {
       if (!(st[i].done && st[i].iter > 0)) continue;
       adscf_ctx* c = b->inst[i];
-      HIPCK(hipMemcpyAsync(&c->st.p->done, &zero, sizeof zero, hipMemcpyHostToDevice, b->stream));
+      hipLaunchKernelGGL(k_state_update, dim3(1), dim3(64), 0, b->stream, c->st.p, 0.0, 0, 1);
+      HIPCK(hipGetLastError());
       hipStream_t own = c->stream;
       c->stream = b->stream;
       try {

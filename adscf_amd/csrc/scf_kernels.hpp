@@ -966,6 +966,14 @@ __global__ void __launch_bounds__(kThreads) k_fill(T* __restrict__ a, T v, long 
 // again (Get_Conjugate_Fields restarts iter_status at 1, main_multiblock.f90:594).
 // The ring keeps its entries; Anderson reads history only from iteration m+1 on,
 // when every entry it reads belongs to the new solve.
+// the loop's tolerance (set_tol) and a cleared stop flag (clear_done), from kernel arguments
+__global__ void k_state_update(ScfState* st, double tol, int set_tol, int clear_done) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (set_tol) st->tol = tol;
+    if (clear_done) st->done = 0;
+  }
+}
+
 __global__ void k_scf_restart(ScfState* st) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     st->iter = 0;

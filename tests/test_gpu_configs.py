@@ -79,3 +79,48 @@ def test_config_128cubed_fp32_mixed_errors_against_fp64():
     print(f"128^3 x 400 FP32-mixed vs FP64 at that field: F {eF:.2e}, phi {ephi:.2e}, stress {est:.2e}; "
           f"stress FP64 {a['stress']}")
     assert eF < 1e-6 and ephi < 1e-4 and est < 1e-3
+
+
+def test_config_128cubed_refinement_switch():
+    """The FP64 refinement phase of the large configuration: the FP32-mixed loop (lam_mix 0.3)
+    runs until max|dfield| < 3e-2, then adscf_set_precision(FP64) continues the same loop and
+    history.  After the switch the loop computes in FP64: an evaluation of its field equals an
+    independent FP64 evaluation of the same field bit for bit (before the switch the two differ
+    by FP32 rounding), and the iteration count carries on.  (The reference's Anderson form
+    approaches the homogeneous state F = chiN/4 - 1 per chain from this seed too slowly to
+    reach 1e-5 in a test: 2.0e-3 after 3000 iterations at lam_mix 0.3.)"""
+    import adscf_amd
+    from adscf_amd import fields
+
+    n, L, ns, chin, m = 128, 6.0, 400, 25.0, 20
+    segs = fields.block_segments([0.25, 0.5, 0.25], ns)
+    chi = [[0.0, chin], [chin, 0.0]]
+
+    def ctx_for(prec):
+        ctx = adscf_amd.Context(n, n, n, L, L, L, 1.0 / ns, precision=prec)
+        ctx.set_interactions(25.0, chi, [1.0, 1.0])
+        ctx.scf_setup([1, 2, 1], segs, ensemble=1, rho=1.0, mixing_dim=m, lam_mix=0.3)
+        return ctx
+
+    with ctx_for(adscf_amd.FP32) as mixed:
+        mixed.scf_set_field(fields.bcc_seed(n, L, chin))
+        st = mixed.scf_iterate(2000, tol=3e-2)
+        print(f"FP32-mixed phase: {st.iters} iterations to residual {st.accuracy:.3e}", flush=True)
+        assert st.converged
+        w, _ = mixed.scf_get()
+        mixed.scf_evaluate()
+        _, d32 = mixed.scf_get()
+        mixed.set_precision(adscf_amd.FP64)
+        mixed.scf_evaluate()
+        _, d_after = mixed.scf_get()
+        st2 = mixed.scf_iterate(20, tol=-1.0)
+        assert st2.iters == st.iters + 20
+    with ctx_for(adscf_amd.FP64) as ref:
+        ref.scf_set_field(w)
+        ref.scf_evaluate()
+        _, d64 = ref.scf_get()
+    e32 = float(np.max(np.abs(d32 - d64)) / np.max(d64))
+    print(f"densities at the switch field: FP32-mixed vs FP64 {e32:.2e}; after the switch "
+          f"{float(np.max(np.abs(d_after - d64))):.1e}")
+    assert 0.0 < e32 < 1e-4
+    assert np.array_equal(d_after, d64)
