@@ -57,6 +57,8 @@ SIGNATURES = {
     "adscf_expfield": (C.c_int, [_ctxp, _dp, _dp]),
     "adscf_propagate_step": (C.c_int, [_ctxp, _dp, _dp]),
     "adscf_propagate_chain": (C.c_int, [_ctxp, C.c_int, _ip, _ip, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "adscf_propagate_chain_noalloc": (C.c_int, [_ctxp, C.c_int, _ip, _ip, C.c_int, C.c_double, C.c_double,
+                                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "adscf_propagate_homopolymer": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
                                               _dp]),
     "adscf_update_solvent": (C.c_int, [_ctxp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -284,6 +286,19 @@ class Context:
         e = _f64(expfield_type, self.shape)
         self._chk(self._lib.adscf_propagate_step(self._h, _p(q), _p(e)))
         return q
+
+    def propagate_chain_noalloc(self, block_types, block_segments, expfield_dev, distribution_dev, density_dev,
+                                scalars_dev, ensemble=1, rho=1.0, mu=0.0):
+        """adscf_propagate_chain_noalloc on device arrays given as integer addresses (allocated
+        with the HIP runtime the library links): expfield (ntypes,nz,ny,nx) float64 in;
+        distribution and density (nblocks,nz,ny,nx) float64 and scalars[4] = Qsum, mu, rho,
+        energy out.  The caller's own pending work on these arrays must be complete first."""
+        bt = np.ascontiguousarray(block_types, dtype=np.int32)
+        bs = np.ascontiguousarray(block_segments, dtype=np.int32)
+        self._chk(self._lib.adscf_propagate_chain_noalloc(self._h, len(bt), _pi(bt), _pi(bs), int(ensemble),
+                                                          float(rho), float(mu), C.c_void_p(int(expfield_dev)),
+                                                          C.c_void_p(int(distribution_dev)),
+                                                          C.c_void_p(int(density_dev)), C.c_void_p(int(scalars_dev))))
 
     def propagate_chain(self, block_types, block_segments, expfield, ensemble=1, rho=1.0, mu=0.0,
                         want_distribution=True, want_density=True):

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <type_traits>
 #include <string>
 #include <vector>
@@ -26,6 +27,14 @@
 #include "adjoint.hpp"
 
 using namespace adscf;
+
+// Device scratch of adscf_propagate_chain_noalloc: the propagator history, the packed line
+// buffer, rescale exponents, reduction partials and the chain state of one evaluation live
+// here instead of in allocations.  Each GPU the library runs on holds its own copy for the
+// life of the process.  Host code never passes this name to a kernel (on the host it is a
+// stand-in, not a device address): it uses the per-device address from hipGetSymbolAddress.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 
@@ -159,6 +168,7 @@ struct adscf_ctx {
   double rho = 1.0, mu = 0.0;
   hipGraphExec_t graph = nullptr;
   GraphKeep keep;         // retired and timing graphs, destroyed at adscf_finalize
+  unsigned char* scratch_dev = nullptr;  // this device's copy of `scratch` (adscf_init)
   bool use_graph = true;  // ADSCF_GRAPH=0 in the environment: stream launches (profiling aid)
   hipGraph_t graph_src = nullptr;
   int graph_prec = -1;
@@ -2177,6 +2187,125 @@ int bcheck(adscf_batch* b, adscf_ctx* c, int rc) {
   return rc;
 }
 
+// ---------------------------------------------------------------------------
+// no-allocation chain solve: every work array in the device scratch
+// ---------------------------------------------------------------------------
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[kMaxScratchDevices] = {};
+
+// The address of `scratch` on device dev (current), looked up once per device.
+unsigned char* scratch_address(int dev) {
+  if (dev < 0 || dev >= kMaxScratchDevices) throw ArgError{"device index out of range for the scratch table"};
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    HIPCK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw ArgError{"hipGetSymbolAddress returned no address for the scratch array"};
+    g_scratch_dev[dev] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch_dev[dev];
+}
+
+// Carves 256-byte aligned pieces off the scratch (or only measures them when base is null).
+struct ScratchCarve {
+  unsigned char* base;
+  size_t off = 0;
+  template <typename V>
+  V* take(size_t count) {
+    const size_t at = off;
+    off = (off + count * sizeof(V) + 255) & ~size_t(255);
+    return base ? reinterpret_cast<V*>(base + at) : nullptr;
+  }
+};
+
+template <typename T>
+struct ChainScratch {
+  ScfState* st;
+  double *qpart, *spart, *rpart, *e2part;
+  int *e1, *e2, *d1, *d2;
+  C2<T>* tmp;
+  T* E;  // FP32: E converted from the caller's FP64 array; FP64: unused
+  T *h1, *h2;
+  size_t bytes;
+};
+
+template <typename T>
+ChainScratch<T> carve_chain(unsigned char* base, long ng, int nt, int ns) {
+  ScratchCarve sc{base};
+  ChainScratch<T> w{};
+  w.st = sc.take<ScfState>(1);
+  w.qpart = sc.take<double>(kRedBlocks);
+  w.spart = sc.take<double>(kRedBlocks);
+  w.rpart = sc.take<double>(kRedBlocks);
+  w.e2part = sc.take<double>((size_t)kRedBlocks * kMaxTypes);
+  w.e1 = sc.take<int>(ns + 1);
+  w.e2 = sc.take<int>(ns + 1);
+  w.d1 = sc.take<int>(ns + 1);
+  w.d2 = sc.take<int>(ns + 1);
+  w.tmp = sc.take<C2<T>>(ng);
+  w.E = std::is_same<T, double>::value ? nullptr : sc.take<T>((size_t)nt * ng);
+  w.h1 = sc.take<T>((size_t)(ns + 1) * ng);
+  w.h2 = sc.take<T>((size_t)(ns + 1) * ng);
+  w.bytes = sc.off;
+  return w;
+}
+
+// Propagate_Multiblock (mod_multiblock.f90:112-188) of adscf_propagate_chain on device arrays,
+// enqueued with the same kernels in the same order, every work array in the scratch.
+template <typename T>
+void propagate_chain_noalloc_impl(adscf_ctx* c, const ChainDesc& cd, double rho, double mu, const double* expfield,
+                                  double* distribution, double* density, double* scalars) {
+  const long ng = c->ng;
+  const ChainScratch<T> need = carve_chain<T>(nullptr, ng, c->nt, cd.ns);
+  if (need.bytes > kScratchBytes) {
+    char buf[256];
+    snprintf(buf, sizeof buf,
+             "the chain needs %zu bytes of scratch (grid %ld points, %d contour steps); the no-allocation entry "
+             "point holds %zu",
+             need.bytes, ng, cd.ns, kScratchBytes);
+    throw ArgError{buf};
+  }
+  const bool ready = std::is_same<T, double>::value ? c->tables64 : c->tables32;
+  if (!ready) throw ArgError{"the transform tables of this precision are not on the device"};
+  const ChainScratch<T> w = carve_chain<T>(c->scratch_dev, ng, c->nt, cd.ns);
+  const size_t nE = (size_t)c->nt * ng;
+  const T* E;
+  if constexpr (std::is_same<T, double>::value) {
+    E = expfield;
+  } else {
+    hipLaunchKernelGGL(k_convert<T>, dim3(grid_for((long)nE)), dim3(kThreads), 0, c->stream, expfield, w.E, (long)nE);
+    E = w.E;
+  }
+  const int nb = grid_for(ng);
+  hipLaunchKernelGGL(k_e2<T>, dim3(nb), dim3(kThreads), 0, c->stream, E, ng, c->nt, w.e2part);
+  hipLaunchKernelGGL(k_schedule, dim3(1), dim3(kThreads), 0, c->stream, cd, (const double*)w.e2part, nb, ng, c->nt,
+                     w.e1, w.e2, w.d1, w.d2, (ScfState*)nullptr, 0, 0, (const int*)nullptr);
+  hipLaunchKernelGGL(k_fill<T>, dim3(nb), dim3(kThreads), 0, c->stream, w.h1, T(1), ng);
+  hipLaunchKernelGGL(k_fill<T>, dim3(nb), dim3(kThreads), 0, c->stream, w.h2, T(1), ng);
+  HIPCK(hipGetLastError());
+  for (auto& p : build_sweep<T>(c, cd, E, w.h1, w.h2, w.d1, w.d2, nullptr, w.tmp)) launch_pass<T>(c, p);
+  hipLaunchKernelGGL(k_qsum<T>, dim3(nb), dim3(kThreads), 0, c->stream, (const T*)(w.h1 + (long)cd.ns * ng), ng,
+                     w.qpart, (const int*)nullptr);
+  hipLaunchKernelGGL(k_qfinal, dim3(1), dim3(kThreads), 0, c->stream, (const double*)w.qpart, nb,
+                     (const int*)w.e1, cd.ns, c->dvol, c->volume, cd.ensemble, rho, mu, w.st, (const int*)nullptr);
+  ContourArgs<T> a = contour_args<T>(c, cd, w.st);
+  a.h1 = w.h1;
+  a.h2 = w.h2;
+  a.e1 = w.e1;
+  a.e2 = w.e2;
+  a.spart = w.spart;
+  a.rpart = nullptr;
+  a.mode = CONTOUR_BLOCKS;
+  a.dist = distribution;
+  a.dens = density;
+  const int nbc = launch_contour<T>(c->stream, a);
+  hipLaunchKernelGGL(k_chain_scalars, dim3(1), dim3(64), 0, c->stream, (const ScfState*)w.st, (const double*)w.spart,
+                     nbc, cd.ensemble, rho, c->dvol, c->volume, c->ds, cd.ns, scalars);
+  HIPCK(hipGetLastError());
+  HIPCK(hipStreamSynchronize(c->stream));
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2218,6 +2347,7 @@ int adscf_init(adscf_ctx** out, int nx, int ny, int nz, int ntypes, double lx, d
     c->prec = precision;
     c->fh.nt = ntypes;
     reset_axes(c);
+    c->scratch_dev = scratch_address(c->dev);
     ensure_current_tables(c);
     return ADSCF_OK;
   });
@@ -2380,6 +2510,26 @@ int adscf_propagate_chain(adscf_ctx* c, int nblocks, const int* monomer_type, co
     return c->prec == ADSCF_FP64
                ? propagate_chain_impl<double>(c, cd, rho, mu, expfield, distribution, density, qsum, energy)
                : propagate_chain_impl<float>(c, cd, rho, mu, expfield, distribution, density, qsum, energy);
+  });
+}
+
+int adscf_propagate_chain_noalloc(adscf_ctx* c, int nblocks, const int* monomer_type, const int* block_segments,
+                                  int ensemble, double rho, double mu, const double* expfield, double* distribution,
+                                  double* density, double* scalars) {
+  return guarded(c, [&]() -> int {
+    if (!monomer_type || !block_segments || !expfield || !distribution || !density || !scalars)
+      throw ArgError{"null argument"};
+    if (ensemble != 1 && ensemble != 2) {
+      c->err = "Invalid ensemble; Cannot calculate Density";  // mod_multiblock.f90:182
+      return ADSCF_ERR_ENSEMBLE;
+    }
+    if (!c->scratch_dev) throw ArgError{"the device scratch was not resolved at adscf_init"};
+    ChainDesc cd = make_chain(c, nblocks, monomer_type, block_segments, ensemble);
+    if (c->prec == ADSCF_FP64)
+      propagate_chain_noalloc_impl<double>(c, cd, rho, mu, expfield, distribution, density, scalars);
+    else
+      propagate_chain_noalloc_impl<float>(c, cd, rho, mu, expfield, distribution, density, scalars);
+    return ADSCF_OK;
   });
 }
 

@@ -966,6 +966,29 @@ __global__ void __launch_bounds__(kThreads) k_fill(T* __restrict__ a, T v, long 
 // again (Get_Conjugate_Fields restarts iter_status at 1, main_multiblock.f90:594).
 // The ring keeps its entries; Anderson reads history only from iteration m+1 on,
 // when every entry it reads belongs to the new solve.
+// The chain scalars of a Propagate_* evaluation into out[4] = {Qsum, mu, rho, energy}
+// (mod_multiblock.f90:155-186): canonical from the state k_qfinal wrote; grand canonical
+// rho = sum(density) dvol / V / (ds Ns) from the contour kernel's per-block sums, added in
+// block order as the host does, and energy = -rho V.
+__global__ void k_chain_scalars(const ScfState* __restrict__ st, const double* __restrict__ spart, int nblk,
+                                int ensemble, double rho_in, double dvol, double volume, double ds, int ns,
+                                double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  out[0] = st->qsum;
+  if (ensemble == 1) {
+    out[1] = st->mu;
+    out[2] = rho_in;
+    out[3] = st->energy;
+  } else {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += spart[b];
+    const double rho = s * dvol / volume / (ds * ns);
+    out[1] = st->mu;
+    out[2] = rho;
+    out[3] = -rho * volume;
+  }
+}
+
 // the loop's tolerance (set_tol) and a cleared stop flag (clear_done), from kernel arguments
 __global__ void k_state_update(ScfState* st, double tol, int set_tol, int clear_done) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {

@@ -89,6 +89,22 @@ int adscf_propagate_step(adscf_ctx* ctx, double* prop, const double* expfield_ty
 int adscf_propagate_chain(adscf_ctx* ctx, int nblocks, const int* monomer_type, const int* block_segments,
                           int ensemble, double* rho, double* mu, const double* expfield, double* distribution,
                           double* density, double* qsum, double* energy);
+/* No counterpart in the reference.  adscf_propagate_chain on DEVICE arrays with no device
+ * allocation, for callers that cannot allocate (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc): expfield (nx,ny,nz,ntypes) in, distribution and density
+ * (nx,ny,nz,nblocks) out, scalars[4] = {Qsum, mu, rho, energy} out, all device pointers; rho
+ * and mu are inputs (canonical: rho; grand canonical: mu).  Every work array lives in one
+ * 128 MiB scratch array per GPU, held until the process ends: the history of the chain needs
+ * 2 (Ns+1) nx ny nz P bytes (P = 8 in FP64, 4 in FP32-mixed) plus about nx ny nz (2P + 4P
+ * in FP32-mixed) and 64 KiB, and a chain that needs more returns ADSCF_ERR_ARG.  The
+ * context's transform tables are uploaded at adscf_init, adscf_set_box, adscf_set_ds and
+ * adscf_set_precision, which may allocate; this call does not.  Results equal
+ * adscf_propagate_chain's bit for bit.  All calls on one GPU share the scratch: calls that
+ * overlap on one GPU (two contexts, two threads) race.  Returns after the results are
+ * written (the context stream is synchronised). */
+int adscf_propagate_chain_noalloc(adscf_ctx* ctx, int nblocks, const int* monomer_type, const int* block_segments,
+                                  int ensemble, double rho, double mu, const double* expfield, double* distribution,
+                                  double* density, double* scalars);
 
 /* ---- Polymer_Modules: Propagate_Homopolymer, Update_Solvent ---------------- */
 
