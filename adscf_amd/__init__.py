@@ -107,6 +107,7 @@ SIGNATURES = {
     "adscf_batch_last_error": (C.c_char_p, [_ctxp]),
     "adscf_batch_set_fields": (C.c_int, [_ctxp, _dp]),
     "adscf_batch_set_precision": (C.c_int, [_ctxp, C.c_int]),
+    "adscf_batch_set_persistent": (C.c_int, [_ctxp, C.c_int, _ip]),
     "adscf_batch_iterate": (C.c_int, [_ctxp, C.c_int, C.c_double, _ip, _dp, _ip]),
     "adscf_batch_energy": (C.c_int, [_ctxp, _dp]),
     "adscf_batch_get": (C.c_int, [_ctxp, C.c_int, _dp, _dp]),
@@ -631,6 +632,13 @@ class Batch:
 
     def set_precision(self, precision):
         self._chk(self._lib.adscf_batch_set_precision(self._h, int(precision)))
+
+    def set_persistent(self, enable=True):
+        """One persistent launch for the whole sweep (True) or one launch per pass (False);
+        returns whether the persistent sweep is active for the current schedules."""
+        act = np.zeros(1, dtype=np.int32)
+        self._chk(self._lib.adscf_batch_set_persistent(self._h, int(bool(enable)), _pi(act)))
+        return bool(act[0])
 
     def iterate(self, niter, tol=0.0):
         it = np.zeros(self.n, dtype=np.int32)

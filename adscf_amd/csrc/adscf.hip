@@ -1887,6 +1887,14 @@ int adjoint_impl(adscf_ctx* c, const ChainDesc& cd, double rho, const double* fi
 // ---------------------------------------------------------------------------
 template <typename T>
 struct BatchTables {
+  // persistent sweep (k_batch_persist): one launch for every pass of every instance
+  bool persist = false;
+  DevBuf<PassArgs<T>> pargs;
+  DevBuf<int> pshape, pneed, pdone;
+  DevBuf<PersistItem> pitems;
+  int pnitems = 0, npass_total = 0, pgrid = 0;
+  size_t psmem = 0;
+  // per-pass launches (lanes), the path for shapes the persistent kernel does not carry
   DevBuf<PassArgs<T>> args;
   DevBuf<int> blk;
   struct Launch {
@@ -1948,6 +1956,8 @@ struct adscf_batch {
   hipGraph_t graph_src = nullptr;
   hipGraphExec_t graph = nullptr;
   GraphKeep keep;  // retired and timing graphs, destroyed at adscf_batch_destroy
+  bool persist_req = false;  // adscf_batch_set_persistent
+  DevBuf<int> perr;  // set by a persistent sweep whose bounded wait expired
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms_iter = 0;
   std::string err;
@@ -1961,6 +1971,136 @@ template <>
 BatchTables<double>& btab<double>(adscf_batch* b) { return b->t64; }
 template <>
 BatchTables<float>& btab<float>(adscf_batch* b) { return b->t32; }
+
+int persist_shape_id(int N1, int N2) {
+  for (int k = 0; k < kNumPersistShapes; ++k)
+    if (kPersistShapes[k][0] == N1 && kPersistShapes[k][1] == N2) return k;
+  return -1;
+}
+
+// the persistent sweep kernel of precision T over kPersistShapes
+template <typename T>
+constexpr auto persist_kernel() {
+  return &k_batch_persist<T, 4, 8, 8, 4, 8, 6, 8, 8, 4, 0>;
+}
+
+// whether every pass of every instance has a shape the persistent kernel carries and a tile
+// that fills the workgroup as the cube engine needs
+template <typename T>
+bool persist_shapes_ok(adscf_batch* b) {
+  for (adscf_ctx* c : b->inst)
+    for (int a : active_axes(c)) {
+      const AxisTables& ax = c->ax[a];
+      if (persist_shape_id(ax.N1, ax.N2) < 0) return false;
+      PassArgs<T> p = base_pass<T>(c, a);
+      int threads = 0;
+      prep_reg<T>(c, p, ax.N1, ax.N2, 4, &threads, nullptr);
+      if (threads > kPersistThreads || (ax.N2 == 0 && threads != kPersistThreads)) return false;
+    }
+  return true;
+}
+
+// Tables of the persistent sweep: every pass of every instance with a 256-thread tile
+// geometry, the items in pass-major order, and the grid the occupancy allows.  Leaves
+// tb.persist false (the per-pass launches run instead) unless adscf_batch_set_persistent
+// asked for it, every shape is one the kernel carries and every tile fills its workgroup.
+template <typename T>
+void batch_build_persist(adscf_batch* b) {
+  BatchTables<T>& tb = btab<T>(b);
+  tb.persist = false;
+  if (!b->persist_req || !persist_shapes_ok<T>(b)) return;
+  std::vector<PassArgs<T>> args;
+  std::vector<int> shape, need, tiles_of, base;
+  size_t smem = 0;
+  int maxpass = 0;
+  for (adscf_ctx* c : b->inst) {
+    Engine<T>& En = eng<T>(c);
+    auto ps = build_sweep<T>(c, c->cd, En.E.p, En.h1.p, En.h2.p, c->d1.p, c->d2.p, &c->st.p->done);
+    base.push_back((int)args.size());
+    int prev_tiles = 0;
+    for (size_t q = 0; q < ps.size(); ++q) {
+      PassArgs<T> p = ps[q];
+      const AxisTables& ax = c->ax[p.axis];
+      const int id = persist_shape_id(ax.N1, ax.N2);
+      if (id < 0) return;
+      int threads = 0;
+      size_t sm = 0;
+      const int tiles = prep_reg<T>(c, p, ax.N1, ax.N2, 4, &threads, &sm);
+      // the cube engine needs every thread of the workgroup on a line
+      if (ax.N2 == 0 && threads != kPersistThreads) return;
+      if (threads > kPersistThreads) return;
+      smem = std::max(smem, sm);
+      args.push_back(p);
+      shape.push_back(id);
+      need.push_back(q == 0 ? 0 : prev_tiles);
+      tiles_of.push_back(tiles);
+      prev_tiles = tiles;
+    }
+    maxpass = std::max(maxpass, (int)ps.size());
+  }
+  std::vector<PersistItem> items;
+  for (int q = 0; q < maxpass; ++q)
+    for (size_t i = 0; i < b->inst.size(); ++i) {
+      const int npass = (i + 1 < b->inst.size() ? base[i + 1] : (int)args.size()) - base[i];
+      if (q >= npass) continue;
+      const int f = base[i] + q;
+      for (int t = 0; t < tiles_of[f]; ++t) items.push_back(PersistItem{f, t});
+    }
+  int per_cu = 0, ncu = 0, dev = 0;
+  HIPCK(hipGetDevice(&dev));
+  HIPCK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persist_kernel<T>(), kPersistThreads, smem));
+  if (per_cu < 1 || ncu < 1) return;
+  tb.pargs.ensure(args.size());
+  tb.pshape.ensure(shape.size());
+  tb.pneed.ensure(need.size());
+  tb.pdone.ensure(args.size());
+  tb.pitems.ensure(items.size());
+  b->perr.ensure(1);
+  HIPCK(hipMemcpy(tb.pargs.p, args.data(), args.size() * sizeof(PassArgs<T>), hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(tb.pshape.p, shape.data(), shape.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(tb.pneed.p, need.data(), need.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(tb.pitems.p, items.data(), items.size() * sizeof(PersistItem), hipMemcpyHostToDevice));
+  const int zero = 0;
+  HIPCK(hipMemcpy(b->perr.p, &zero, sizeof zero, hipMemcpyHostToDevice));
+  tb.pnitems = (int)items.size();
+  tb.npass_total = (int)args.size();
+  tb.pgrid = std::min(tb.pnitems, per_cu * ncu);
+  tb.psmem = smem;
+  tb.persist = true;
+}
+
+template <typename T>
+void launch_batch_persist(adscf_batch* b) {
+  BatchTables<T>& tb = btab<T>(b);
+  hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(tb.npass_total)), dim3(kThreads), 0, b->stream, tb.pdone.p, 0,
+                     (long)tb.npass_total);
+  PersistArgs<T> pa{};
+  pa.args = tb.pargs.p;
+  pa.shape = tb.pshape.p;
+  pa.need = tb.pneed.p;
+  pa.items = tb.pitems.p;
+  pa.nitems = tb.pnitems;
+  pa.done = tb.pdone.p;
+  pa.err = b->perr.p;
+  pa.spin_limit = kPersistSpinLimit;
+  hipLaunchKernelGGL(persist_kernel<T>(), dim3(tb.pgrid), dim3(kPersistThreads), tb.psmem, b->stream, pa);
+  HIPCK(hipGetLastError());
+}
+
+// a persistent sweep that gave up waiting leaves *err set: report it (and clear it)
+void batch_check_persist(adscf_batch* b) {
+  if (!b->perr.p) return;
+  int e = 0;
+  HIPCK(hipMemcpy(&e, b->perr.p, sizeof e, hipMemcpyDeviceToHost));
+  if (e) {
+    const int zero = 0;
+    HIPCK(hipMemcpy(b->perr.p, &zero, sizeof zero, hipMemcpyHostToDevice));
+    throw ArgError{"the persistent batched sweep gave up waiting for a pass (not every workgroup was resident); "
+                   "results of this call are invalid (adscf_batch_set_persistent(batch, 0) selects per-pass "
+                   "launches)"};
+  }
+}
 
 template <typename T>
 void batch_build_tables(adscf_batch* b) {
@@ -2053,6 +2193,7 @@ void batch_build_tables(adscf_batch* b) {
   tb.blk.ensure(blk.size());
   HIPCK(hipMemcpy(tb.args.p, args.data(), args.size() * sizeof(PassArgs<T>), hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(tb.blk.p, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice));
+  batch_build_persist<T>(b);
 }
 
 template <typename T>
@@ -2078,12 +2219,16 @@ hipStream_t lane_stream(adscf_batch* b, int lane) { return lane == 0 ? b->stream
 template <typename T>
 void batch_enqueue_lanes(adscf_batch* b, bool post) {
   BatchTables<T>& tb = btab<T>(b);
+  // the persistent sweep runs alone on the batch stream (every workgroup resident), then the
+  // lanes fork for the post-sweep kernels
+  if (tb.persist) launch_batch_persist<T>(b);
   HIPCK(hipEventRecord(b->fork, b->stream));
   for (hipStream_t s : b->side) HIPCK(hipStreamWaitEvent(s, b->fork, 0));
   for (int lane = 0; lane < kBatchLanes; ++lane) {
     hipStream_t s = lane_stream(b, lane);
-    for (const auto& L : tb.launches)
-      if (L.lane == lane) launch_batch_group<T>(s, L, BatchGroup<T>{tb.args.p + L.arg_off, tb.blk.p + L.blk_off});
+    if (!tb.persist)
+      for (const auto& L : tb.launches)
+        if (L.lane == lane) launch_batch_group<T>(s, L, BatchGroup<T>{tb.args.p + L.arg_off, tb.blk.p + L.blk_off});
     HIPCK(hipGetLastError());
     if (!post) continue;
     for (size_t i = 0; i < b->inst.size(); ++i) {
@@ -2153,6 +2298,7 @@ double batch_sweep_timing(adscf_batch* b, int reps) {
   for (int r = 0; r < reps; ++r) HIPCK(hipGraphLaunch(ge, b->stream));
   HIPCK(hipEventRecord(b->ev1, b->stream));
   HIPCK(hipEventSynchronize(b->ev1));
+  batch_check_persist(b);
   float ms = 0.f;
   HIPCK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   return ms / reps;
@@ -3369,6 +3515,17 @@ int adscf_batch_create(adscf_batch** out, int ninst, const int* grid, const doub
   return ADSCF_OK;
 }
 
+int adscf_batch_set_persistent(adscf_batch* b, int enable, int* active) {
+  return bguard(b, [&]() -> int {
+    if (enable != 0 && enable != 1) throw ArgError{"enable must be 0 or 1"};
+    b->persist_req = enable != 0;
+    batch_drop_graph(b);
+    if (active)
+      *active = b->persist_req && (b->prec == ADSCF_FP64 ? persist_shapes_ok<double>(b) : persist_shapes_ok<float>(b));
+    return ADSCF_OK;
+  });
+}
+
 int adscf_batch_destroy(adscf_batch* b) {
   if (!b) return ADSCF_ERR_ARG;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
@@ -3460,6 +3617,7 @@ int adscf_batch_iterate(adscf_batch* b, int niter, double tol, int* iters, doubl
       for (int i = 0; i < n; ++i)
         HIPCK(hipMemcpyAsync(&st[i], b->inst[i]->st.p, sizeof(ScfState), hipMemcpyDeviceToHost, b->stream));
       HIPCK(hipStreamSynchronize(b->stream));
+      batch_check_persist(b);
       float ms = 0.f;
       HIPCK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
       total += ms;

@@ -363,7 +363,48 @@ __device__ __forceinline__ void sto(V* base, unsigned off, V x) {
   *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + off) = x;
 }
 
-template <typename T, class ENG, int MODE = PM_GENERIC>
+// Coherent access to the packed line buffer for the persistent sweep, where the pass that
+// wrote a line and the pass that reads it are workgroups of one kernel on different XCDs:
+// relaxed atomic loads and stores at agent scope, 64 bits at a time, which bypass the
+// non-coherent caches (an XCD's L2 may hold a stale copy of the line from an earlier pass)
+// and write through to the device-wide coherence point.
+template <typename T>
+__device__ __forceinline__ C2<T> ld_co(const C2<T>* base, unsigned off) {
+  const char* p = reinterpret_cast<const char*>(base) + off;
+  if constexpr (sizeof(T) == 4) {
+    const unsigned long long u =
+        __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    C2<T> v;
+    __builtin_memcpy(&v, &u, sizeof v);
+    return v;
+  } else {
+    const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
+    const unsigned long long ux = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long uy = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    C2<T> v;
+    __builtin_memcpy(&v.x, &ux, sizeof ux);
+    __builtin_memcpy(&v.y, &uy, sizeof uy);
+    return v;
+  }
+}
+template <typename T>
+__device__ __forceinline__ void st_co(C2<T>* base, unsigned off, C2<T> v) {
+  char* p = reinterpret_cast<char*>(base) + off;
+  if constexpr (sizeof(T) == 4) {
+    unsigned long long u;
+    __builtin_memcpy(&u, &v, sizeof u);
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    unsigned long long ux, uy;
+    __builtin_memcpy(&ux, &v.x, sizeof ux);
+    __builtin_memcpy(&uy, &v.y, sizeof uy);
+    unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+    __hip_atomic_store(q, ux, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, uy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <typename T, class ENG, int MODE = PM_GENERIC, bool CO = false>
 __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, unsigned char* smem_raw) {
   constexpr int NP = ENG::NP, STRIDE = ENG::STRIDE;
   constexpr bool G = MODE == PM_GENERIC;
@@ -412,7 +453,7 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
     ea1[j2] = ea2[j2] = eb1[j2] = eb2[j2] = T(1);
     if (act) {
       if (in_kind == IN_TMP) {
-        x = ldo(a.tmp_in, fc(j2));
+        x = CO ? ld_co(a.tmp_in, fc(j2)) : ldo(a.tmp_in, fc(j2));
       } else if (in_kind == IN_ONES) {
         x = {T(1), two ? T(1) : T(0)};
       } else {
@@ -493,7 +534,10 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
   if (!act) return;
   if (out_kind == OUT_TMP) {
 #pragma unroll
-    for (int j2 = 0; j2 < NP; ++j2) sto(a.tmp_out, fc(j2), v[j2]);
+    for (int j2 = 0; j2 < NP; ++j2) {
+      if constexpr (CO) st_co(a.tmp_out, fc(j2), v[j2]);
+      else sto(a.tmp_out, fc(j2), v[j2]);
+    }
   } else {
 #pragma unroll
     for (int j2 = 0; j2 < NP; ++j2) {
@@ -562,6 +606,92 @@ __global__ void __launch_bounds__(64) k_line_pass_batch(BatchGroup<T> g) {
   extern __shared__ unsigned char smem_raw[];
   reg_pass_body<T, PassEngine<T, N1, N2>>(a, item, smem_raw);
 }
+
+// ---------------------------------------------------------------------------
+// persistent batched sweep: every pass of every instance in one launch
+// ---------------------------------------------------------------------------
+// A work item is one 256-thread tile of one pass of one instance.  Items are ordered pass
+// by pass (then instance, then tile) and dealt to the resident workgroups round-robin; each
+// workgroup runs its items in that order.  An item waits only until every tile of the
+// previous pass of its own instance is done (a per-pass completion counter), so instances
+// advance independently and a workgroup never idles on a global barrier.  An item depends
+// only on items earlier in the order, and every workgroup is resident (the grid is sized by
+// the occupancy), so the earliest unfinished item can always run.  Waits are bounded: a
+// workgroup that waits past the spin limit records it in *err and leaves the kernel, and
+// the host reports the sweep as failed.
+constexpr int kPersistThreads = 256;
+constexpr int kPersistSpinLimit = 2000000;  // polls (~1 us each) before a wait gives up
+
+struct PersistItem {
+  int pass;  // flat pass index: instance base + pass number
+  int tile;  // tile of that pass
+};
+
+template <typename T>
+struct PersistArgs {
+  const PassArgs<T>* args;  // one per flat pass (256-thread tile geometry)
+  const int* shape;         // shape id of each flat pass (index into the kernel's shape list)
+  const int* need;          // tiles of the previous pass of the same instance (0: first pass)
+  const PersistItem* items;
+  int nitems;
+  int* done;                // tiles completed per flat pass; zeroed before each sweep
+  int* err;
+  int spin_limit;
+};
+
+template <typename T, int... S>
+struct PersistShapes;
+template <typename T>
+struct PersistShapes<T> {
+  static __device__ __forceinline__ void run(int, const PassArgs<T>&, int, unsigned char*) {}
+};
+template <typename T, int A, int B, int... R>
+struct PersistShapes<T, A, B, R...> {
+  static __device__ __forceinline__ void run(int id, const PassArgs<T>& a, int tile, unsigned char* sm) {
+    if (id == 0) reg_pass_body<T, PassEngine<T, A, B>, PM_GENERIC, true>(a, tile, sm);
+    else PersistShapes<T, R...>::run(id - 1, a, tile, sm);
+  }
+};
+
+template <typename T, int... S>
+__global__ void __launch_bounds__(kPersistThreads) k_batch_persist(PersistArgs<T> pa) {
+  extern __shared__ unsigned char smem_raw[];
+  __shared__ int s_fail;
+  for (int k = blockIdx.x; k < pa.nitems; k += gridDim.x) {
+    const PersistItem it = pa.items[k];
+    if (threadIdx.x == 0) {
+      int fail = 0;
+      const int need = pa.need[it.pass];
+      if (need > 0) {
+        int spins = 0;
+        while (__hip_atomic_load(pa.done + it.pass - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+          if (++spins > pa.spin_limit) {
+            fail = 1;
+            __hip_atomic_store(pa.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+      }
+      s_fail = fail;
+    }
+    __syncthreads();
+    if (s_fail) return;  // uniform: the previous pass never completed
+    // the line buffer is read and written with agent-scope atomics (ld_co / st_co), so the
+    // handshake needs no cache-wide fence: each wave waits until its stores are complete
+    // (written through to the coherence point) before the workgroup reports the tile done,
+    // and the next pass issues its loads only after it has seen the count
+    const PassArgs<T> a = pa.args[it.pass];
+    PersistShapes<T, S...>::run(pa.shape[it.pass], a, it.tile, smem_raw);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(pa.done + it.pass, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// the shapes one persistent kernel carries: (N1, N2) pairs, N2 == 0 the cube engine
+constexpr int kPersistShapes[][2] = {{4, 8}, {8, 4}, {8, 6}, {8, 8}, {4, 0}};
+constexpr int kNumPersistShapes = 5;
 
 constexpr int kBatchShapes[][2] = {{4, 8}, {8, 4}, {2, 16}, {8, 6}, {6, 8}, {16, 3}, {8, 8},
                                    {4, 16}, {16, 4}, {8, 16}, {16, 8}, {1, 16}, {1, 8}, {4, 0}};

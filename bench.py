@@ -171,6 +171,17 @@ def run_batched(adscf_amd, fields, steps, warmup):
         it, acc, conv = b.iterate(steps, tol=-1.0)
         wall = time.perf_counter() - t0
         ms_iter_ev, ms_sweep = b.timing()
+        # the same iteration with the sweep as one persistent launch
+        persistent = {"active": b.set_persistent(True)}
+        if persistent["active"]:
+            b.set_fields(w)
+            b.iterate(warmup, tol=-1.0)
+            t0 = time.perf_counter()
+            b.iterate(steps, tol=-1.0)
+            pwall = time.perf_counter() - t0
+            persistent.update(batch_iters_per_s=steps / pwall, ms_per_batch_iter=1000.0 * pwall / steps,
+                              ms_sweep=b.timing()[1])
+        b.set_persistent(False)
     ips = steps / wall
     bytes_iter = sum(algorithmic_bytes_iter(d["grid"][0] ** 3, sum(d["segments"]), M, 4) for d in inst)
     sweep_bytes = sum(2 * sum(d["segments"]) * 3 * d["grid"][0] ** 3 * 4 for d in inst)
@@ -179,7 +190,9 @@ def run_batched(adscf_amd, fields, steps, warmup):
                 contour_steps_per_s=ips * sum(2 * sum(d["segments"]) for d in inst),
                 iteration_bytes_frac=bytes_iter * ips / (HBM_PEAK_GBS * 1e9),
                 sweep_roofline_frac=sweep_bytes / (ms_sweep * 1e-3) / (HBM_PEAK_GBS * 1e9),
-                timing_valid=not bool(conv.any()),
+                timing_valid=not bool(conv.any()), ms_sweep=ms_sweep,
+                sweep_mode="one launch per pass and register shape (faster than the persistent launch below)",
+                persistent_sweep=persistent,
                 schedules=[l for l in report.splitlines() if "chosen" in l])
 
 

@@ -293,6 +293,13 @@ const char* adscf_batch_last_error(const adscf_batch* batch);
 /* fields of all instances concatenated, each (nx,ny,nz,2) column-major */
 int adscf_batch_set_fields(adscf_batch* batch, const double* fields);
 int adscf_batch_set_precision(adscf_batch* batch, int precision);
+/* Sweep of a batched iteration: enable 1 runs every propagator pass of every instance in one
+ * persistent launch (workgroups take tiles pass by pass and wait only for the previous pass of
+ * their own instance; bounded waits, a wait that expires makes adscf_batch_iterate fail);
+ * 0 (the default) runs one launch per pass and register shape.  The persistent sweep carries
+ * the shapes 4x8, 8x4, 8x6, 8x8 and cube4 (axis lengths 32, 48 and 64); active receives 1 when
+ * the batch's schedules allow it, else the per-pass launches stay.  Results are identical. */
+int adscf_batch_set_persistent(adscf_batch* batch, int enable, int* active);
 /* per-instance outputs (arrays of ninst); stops when every instance has converged */
 int adscf_batch_iterate(adscf_batch* batch, int niter, double tol, int* iters, double* accuracy, int* converged);
 int adscf_batch_energy(adscf_batch* batch, double* F_per_chain);

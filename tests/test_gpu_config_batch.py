@@ -12,7 +12,7 @@ General_Routines/mod_iterate.f90:92-150), all run by one batched graph replay pe
 * 23 FP64 iterations (three Anderson solves past m = 20) of one instance per grid size and
   contour length, each bit for bit equal to a single-instance context on the same line-pass
   schedules: the batched passes, Gram sums, wavefront LU and mix are the single-instance
-  ones.
+  ones; with the sweep as per-pass launches and as one persistent launch.
 """
 import numpy as np
 import pytest
@@ -77,7 +77,8 @@ def test_sweep_one_iteration_against_oracle(orc, sweep, prec):
     assert np.all(np.isfinite(F))
 
 
-def test_sweep_instances_match_single_contexts_past_m(sweep):
+@pytest.mark.parametrize("persistent", [False, True])
+def test_sweep_instances_match_single_contexts_past_m(sweep, persistent):
     import adscf_amd
 
     inst, w = sweep
@@ -91,8 +92,9 @@ def test_sweep_instances_match_single_contexts_past_m(sweep):
     sub = [inst[i] for i in pick]
     ws = [w[i] for i in pick]
     iters = M + 3
-    print(f"instances {pick} ({sorted(seen)}), {iters} FP64 iterations", flush=True)
+    print(f"instances {pick} ({sorted(seen)}), {iters} FP64 iterations, persistent {persistent}", flush=True)
     with adscf_amd.Batch(sub, mixing_dim=M, lam_mix=LAM, precision=adscf_amd.FP64) as b:
+        assert b.set_persistent(persistent) == persistent
         b.set_fields(ws)
         it, acc, conv = b.iterate(iters, tol=-1.0)
         got = [b.get(k) for k in range(len(sub))]
@@ -110,3 +112,21 @@ def test_sweep_instances_match_single_contexts_past_m(sweep):
         assert st.iters == it[k]
         assert st.accuracy == acc[k]
         assert np.array_equal(f1, got[k][0]) and np.array_equal(d1, got[k][1])
+
+
+def test_sweep_persistent_fp32_matches_per_pass(sweep):
+    """FP32-mixed, all 64 instances, 5 iterations: the persistent sweep and the per-pass
+    launches give the same fields, densities and residuals bit for bit."""
+    import adscf_amd
+
+    inst, w = sweep
+    out = {}
+    for persistent in (False, True):
+        with adscf_amd.Batch(inst, mixing_dim=M, lam_mix=LAM, precision=adscf_amd.FP32) as b:
+            assert b.set_persistent(persistent) == persistent
+            b.set_fields(w)
+            it, acc, conv = b.iterate(5, tol=-1.0)
+            out[persistent] = (acc, [b.get(i) for i in range(len(inst))])
+    assert np.array_equal(out[False][0], out[True][0])
+    for a, c in zip(out[False][1], out[True][1]):
+        assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
