@@ -462,14 +462,15 @@ def cpu_baseline(adscf_amd, fields, w0):
                 gpu_vs_oracle_same_iteration=parity)
 
 
-def profile_run(adscf_amd, fields, what, iters):
+def profile_run(adscf_amd, fields, what, iters, tune=False):
     """A workload without timing legs, for rocprofv3 kernel traces and counter runs:
     metric32 / metric64: `iters` SCF iterations of the 64^3 x 200 diblock workload above
     (the first m + 1 = 21 iterations mix without the Anderson solve; later ones run every
     kernel of the steady-state iteration); large32 / large64: `iters` iterations of the
     128^3 x 400 ABA triblock (lamellar seed); batch: `iters` batched iterations of the
     64-instance sweep (FP32-mixed).  With ADSCF_GRAPH=0 in the environment the iterations
-    are stream launches instead of graph replays."""
+    are stream launches instead of graph replays; with tune the metric workload first runs
+    the autotuner (many captured sweeps, each replayed several times)."""
     if what in ("metric32", "metric64"):
         prec = adscf_amd.FP32 if what == "metric32" else adscf_amd.FP64
         w0 = fields.cylinder_seed(N, L, CHIN, sigma=0.2, seed=2007)
@@ -478,6 +479,8 @@ def profile_run(adscf_amd, fields, what, iters):
             ctx.scf_setup([1, 2], fields.block_segments([FA, 1 - FA], NS), ensemble=1, rho=1.0, mixing_dim=M,
                           lam_mix=LAM)
             ctx.scf_set_field(w0)
+            if tune:
+                print(ctx.autotune(reps=3), flush=True)
             st = ctx.scf_iterate(iters, tol=-1.0)
     elif what in ("large32", "large64"):
         n, Lb, ns = 128, 6.0, 400
@@ -560,7 +563,7 @@ def main():
     if args.profile_run:
         if args.copy_probe:
             copy_probe()
-        profile_run(adscf_amd, fields, args.profile_run, args.iters)
+        profile_run(adscf_amd, fields, args.profile_run, args.iters, args.autotune)
         return
     w0 = fields.cylinder_seed(N, L, CHIN, sigma=0.2, seed=2007)
     ng = N ** 3
