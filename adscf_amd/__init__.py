@@ -99,6 +99,8 @@ SIGNATURES = {
     "adscf_autotune": (C.c_int, [_ctxp, C.c_int, C.c_char_p, C.c_int]),
     "adscf_get_schedule": (C.c_int, [_ctxp, C.c_int, _ip, _ip, _ip, _ip]),
     "adscf_set_schedule": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "adscf_set_pass_tiles": (C.c_int, [_ctxp, C.c_int, C.c_int]),
+    "adscf_get_pass_tiles": (C.c_int, [_ctxp, C.c_int, _ip]),
     "adscf_set_middle_axis": (C.c_int, [_ctxp, C.c_int]),
     "adscf_get_middle_axis": (C.c_int, [_ctxp, _ip]),
     "adscf_batch_create": (C.c_int, [C.POINTER(_ctxp), C.c_int, _ip, _dp, _dp, _ip, _ip, _ip, _dp, _dp, _dp,
@@ -552,6 +554,15 @@ class Context:
 
     def set_schedule(self, axis, n1, n2, waves=1):
         self._chk(self._lib.adscf_set_schedule(self._h, int(axis), int(n1), int(n2), int(waves)))
+
+    def set_pass_tiles(self, axis, tiles):
+        """Tiles per workgroup (1, 2, 4) of the sweep passes along an axis."""
+        self._chk(self._lib.adscf_set_pass_tiles(self._h, int(axis), int(tiles)))
+
+    def pass_tiles(self, axis):
+        t = np.zeros(1, dtype=np.int32)
+        self._chk(self._lib.adscf_get_pass_tiles(self._h, int(axis), _pi(t)))
+        return int(t[0])
 
     # -- adjoint ----------------------------------------------------------------
     def adjoint(self, block_types, block_segments, field, phi_target, lam=0.1, target_type=1, rho=1.0,

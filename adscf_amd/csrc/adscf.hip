@@ -82,6 +82,7 @@ struct AxisTables {
   int N1 = 0;     // register-resident shape (0: Stockham kernel)
   int N2 = 0;
   int waves = 1;  // wavefronts per workgroup of the register kernel
+  int tiles = 1;  // tiles per workgroup of the sweep's fixed-mode passes (k_line_pass_tiles)
 };
 
 // Captured graphs live until the context (or batch) is torn down: an executable graph that
@@ -367,20 +368,22 @@ struct RegShape {
   int n, N1, N2;
 };
 
-// N2 == 0: the R^3 cube engine with R = N1 (fft_cube.hpp)
+// N2 == 0: the R^3 cube engine with R = N1; N2 < 0: the A R^2 engine with A = -N2, R = N1
+// (fft_cube.hpp)
 const RegShape kRegShapes[] = {
     {2, 1, 2},    {3, 1, 3},   {4, 1, 4},   {5, 1, 5},    {6, 1, 6},    {8, 1, 8},   {16, 1, 16},  {10, 2, 5},
     {12, 4, 3},   {20, 4, 5},  {24, 8, 3},  {24, 4, 6},   {32, 4, 8},   {32, 8, 4},  {32, 2, 16},  {36, 6, 6},
     {40, 8, 5},   {48, 8, 6},  {48, 6, 8},  {48, 16, 3},  {64, 8, 8},   {64, 4, 0},  {64, 4, 16},  {64, 16, 4},
-    {80, 16, 5},  {96, 16, 6}, {128, 8, 16}, {128, 16, 8}, {256, 16, 16}};
+    {80, 16, 5},  {96, 16, 6}, {128, 8, 16}, {128, 16, 8}, {128, 4, -8}, {256, 16, 16}, {256, 4, -16}};
 
 std::string shape_name(int N1, int N2) {
   if (N1 == 0) return "stockham";
   if (N2 == 0) return "cube" + std::to_string(N1);
+  if (N2 < 0) return "tri" + std::to_string(-N2) + "x" + std::to_string(N1) + "x" + std::to_string(N1);
   return std::to_string(N1) + "x" + std::to_string(N2);
 }
 
-int reg_tpl_rt(int N1, int N2) { return N2 == 0 ? N1 * N1 : (N1 == 1 ? 1 : std::max(N1, N2)); }
+int reg_tpl_rt(int N1, int N2) { return N2 <= 0 ? N1 * N1 : (N1 == 1 ? 1 : std::max(N1, N2)); }
 
 // Tile geometry of a register-resident pass: `waves` wavefronts per block, L lines
 // per block; fills p.L and p.gx and returns the number of blocks.
@@ -396,9 +399,45 @@ int prep_reg(const adscf_ctx* c, PassArgs<T>& p, int N1, int N2, int waves, int*
   else if (p.axis == 1) items = c->nz * p.gx;
   else items = c->ny * p.gx;
   if (threads) *threads = TPL * L;
-  const size_t per_line = N2 == 0 ? (size_t)2 * N1 * N1 * (N1 + 1) : (N1 == 1 ? 0 : (size_t)N2 * (N1 + 1));
+  const size_t per_line = N2 == 0  ? (size_t)2 * N1 * N1 * (N1 + 1)
+                          : N2 < 0 ? (size_t)N1 * N1 * (1 - N2) + (size_t)(-N2) * N1 * (N1 + 1)
+                                   : (N1 == 1 ? 0 : (size_t)N2 * (N1 + 1));
   if (smem) *smem = (size_t)L * per_line * sizeof(C2<T>);
   return items;
+}
+
+// points of a line of the register shape (N1, N2)
+constexpr int reg_shape_n(int N1, int N2) { return N2 == 0 ? N1 * N1 * N1 : (N2 < 0 ? -N2 * N1 * N1 : N1 * N2); }
+
+// the K-tile variants (k_line_pass_tiles) of the fixed sweep modes; lengths of 32 and more
+template <typename T, int N1, int N2, int K>
+bool launch_tiles(adscf_ctx* c, const PassArgs<T>& p, int items, int threads, size_t smem) {
+  // FP64 with four tiles spills registers on the long-line shapes: not compiled
+  if constexpr (reg_shape_n(N1, N2) < 32 || (sizeof(T) == 8 && K == 4)) {
+    return false;
+  } else {
+    if (items < kXcds) return false;
+    const int per_xcd = (items + kXcds - 1) / kXcds;
+    const dim3 grid(kXcds * ((per_xcd + K - 1) / K)), block(threads);
+    switch (pass_mode(p)) {
+      case PM_FIRST:
+        hipLaunchKernelGGL((k_line_pass_tiles<T, N1, N2, PM_FIRST, K>), grid, block, smem, c->stream, p, items);
+        break;
+      case PM_PLAIN:
+        hipLaunchKernelGGL((k_line_pass_tiles<T, N1, N2, PM_PLAIN, K>), grid, block, smem, c->stream, p, items);
+        break;
+      case PM_MERGED:
+        hipLaunchKernelGGL((k_line_pass_tiles<T, N1, N2, PM_MERGED, K>), grid, block, smem, c->stream, p, items);
+        break;
+      case PM_LAST:
+        hipLaunchKernelGGL((k_line_pass_tiles<T, N1, N2, PM_LAST, K>), grid, block, smem, c->stream, p, items);
+        break;
+      default:
+        return false;
+    }
+    HIPCK(hipGetLastError());
+    return true;
+  }
 }
 
 template <typename T, int N1, int N2>
@@ -406,6 +445,9 @@ int launch_reg(adscf_ctx* c, PassArgs<T> p, int waves) {
   int threads;
   size_t smem;
   const int items = prep_reg<T>(c, p, N1, N2, waves, &threads, &smem);
+  const int tiles = c->ax[p.axis].tiles;
+  if (tiles == 2 && launch_tiles<T, N1, N2, 2>(c, p, items, threads, smem)) return items;
+  if (tiles == 4 && launch_tiles<T, N1, N2, 4>(c, p, items, threads, smem)) return items;
   const dim3 grid(items), block(threads);
   switch (pass_mode(p)) {
     case PM_FIRST: hipLaunchKernelGGL((k_line_pass_reg<T, N1, N2, PM_FIRST>), grid, block, smem, c->stream, p); break;
@@ -429,7 +471,8 @@ RegLauncher<T> reg_launcher(int n, int N1, int N2) {
   ADSCF_SHAPE(24, 8, 3) ADSCF_SHAPE(24, 4, 6) ADSCF_SHAPE(32, 4, 8) ADSCF_SHAPE(32, 8, 4) ADSCF_SHAPE(32, 2, 16)
   ADSCF_SHAPE(36, 6, 6) ADSCF_SHAPE(40, 8, 5) ADSCF_SHAPE(48, 8, 6) ADSCF_SHAPE(48, 6, 8) ADSCF_SHAPE(48, 16, 3)
   ADSCF_SHAPE(64, 8, 8) ADSCF_SHAPE(64, 4, 0) ADSCF_SHAPE(64, 4, 16) ADSCF_SHAPE(64, 16, 4) ADSCF_SHAPE(80, 16, 5)
-  ADSCF_SHAPE(96, 16, 6) ADSCF_SHAPE(128, 8, 16) ADSCF_SHAPE(128, 16, 8) ADSCF_SHAPE(256, 16, 16)
+  ADSCF_SHAPE(96, 16, 6) ADSCF_SHAPE(128, 8, 16) ADSCF_SHAPE(128, 16, 8) ADSCF_SHAPE(128, 4, -8)
+  ADSCF_SHAPE(256, 16, 16) ADSCF_SHAPE(256, 4, -16)
 #undef ADSCF_SHAPE
   return nullptr;
 }
@@ -441,6 +484,7 @@ RegLauncher<T> reg_launcher(int n, int N1, int N2) {
 void default_schedule(AxisTables& ax, int axis, int prec) {
   ax.N1 = ax.N2 = 0;
   ax.waves = 1;
+  ax.tiles = 1;
   for (const RegShape& s : kRegShapes)
     if (s.n == ax.n) {
       ax.N1 = s.N1;
@@ -451,9 +495,19 @@ void default_schedule(AxisTables& ax, int axis, int prec) {
     ax.N1 = 4;
     ax.N2 = 0;
   }
-  if (ax.n == 128) {
+  if (ax.n == 128) {  // the tuner's choice at 128^3 x 400 on the MI355X
     ax.N1 = 16;
     ax.N2 = 8;
+    if (prec == ADSCF_FP32) {
+      ax.waves = axis == 0 ? 1 : 2;
+      ax.tiles = axis == 2 ? 2 : 1;
+    } else if (axis == 0) {
+      ax.N1 = 4;  // the 8 x 4 x 4 engine
+      ax.N2 = -8;
+    } else if (axis == 1) {
+      ax.N1 = 8;
+      ax.N2 = 16;
+    }
   }
 }
 
@@ -1363,7 +1417,7 @@ std::string sweep_key(adscf_ctx* c) {
   for (int b = 0; b < c->cd.nblocks; ++b) k += "/" + std::to_string(c->cd.type[b]) + ":" + std::to_string(c->cd.seg[b]);
   for (const AxisTables& a : c->ax)
     k += "/" + std::to_string(a.N1) + "," + std::to_string(a.N2) + "," + std::to_string(a.waves) + "," +
-         std::to_string(a.L);
+         std::to_string(a.L) + "," + std::to_string(a.tiles);
   char buf[32];
   for (const void* p : graph_key<T>(c)) {
     snprintf(buf, sizeof buf, "/%p", p);
@@ -1425,15 +1479,21 @@ std::string autotune_impl(adscf_ctx* c, int reps) {
     std::vector<std::pair<int, int>> shapes = {{0, 0}};
     for (const RegShape& s : kRegShapes)
       if (s.n == ax.n) shapes.push_back({s.N1, s.N2});
+    // (waves per workgroup, tiles per workgroup) of a register shape; the multi-tile passes
+    // exist for lines of 32 points or more, in FP64 with two tiles only
+    const std::pair<int, int> geoms[] = {{1, 1}, {2, 1}, {4, 1}, {1, 2}, {1, 4}, {2, 2}};
     for (auto sh : shapes) {
-      for (int waves : {1, 2, 4}) {
-        if (sh.first == 0 && waves > 1) break;
+      for (auto wt : geoms) {
+        const int waves = wt.first, tiles = wt.second;
+        if (sh.first == 0 && (waves > 1 || tiles > 1)) break;
+        if (tiles > 1 && (ax.n < 32 || (sizeof(T) == 8 && tiles == 4))) continue;
         ax.N1 = sh.first;
         ax.N2 = sh.second;
         ax.waves = waves;
+        ax.tiles = tiles;
         const double ms = sweep_timing<T>(c, reps);
-        snprintf(line, sizeof line, "axis %d n=%d shape=%s waves=%d sweep=%.4f ms\n", a, ax.n,
-                 shape_name(sh.first, sh.second).c_str(), waves, ms);
+        snprintf(line, sizeof line, "axis %d n=%d shape=%s waves=%d tiles=%d sweep=%.4f ms\n", a, ax.n,
+                 shape_name(sh.first, sh.second).c_str(), waves, tiles, ms);
         rep += line;
         if (ms < best_ms) {
           best_ms = ms;
@@ -1442,8 +1502,8 @@ std::string autotune_impl(adscf_ctx* c, int reps) {
       }
     }
     ax = best;
-    snprintf(line, sizeof line, "axis %d chosen: %s waves=%d (%.4f ms)\n", a, shape_name(best.N1, best.N2).c_str(),
-             best.waves, best_ms);
+    snprintf(line, sizeof line, "axis %d chosen: %s waves=%d tiles=%d (%.4f ms)\n", a,
+             shape_name(best.N1, best.N2).c_str(), best.waves, best.tiles, best_ms);
     rep += line;
   }
   if (active_axes(c).size() == 3) {  // which axis takes the single-operator passes
@@ -1957,6 +2017,7 @@ struct adscf_batch {
   hipGraphExec_t graph = nullptr;
   GraphKeep keep;  // retired and timing graphs, destroyed at adscf_batch_destroy
   bool persist_req = false;  // adscf_batch_set_persistent
+  bool use_graph = true;     // ADSCF_GRAPH=0 in the environment: stream launches (profiling aid)
   DevBuf<int> perr;  // set by a persistent sweep whose bounded wait expired
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms_iter = 0;
@@ -1995,7 +2056,7 @@ bool persist_shapes_ok(adscf_batch* b) {
       PassArgs<T> p = base_pass<T>(c, a);
       int threads = 0;
       prep_reg<T>(c, p, ax.N1, ax.N2, 4, &threads, nullptr);
-      if (threads > kPersistThreads || (ax.N2 == 0 && threads != kPersistThreads)) return false;
+      if (threads > kPersistThreads || (ax.N2 <= 0 && threads != kPersistThreads)) return false;
     }
   return true;
 }
@@ -2027,7 +2088,7 @@ void batch_build_persist(adscf_batch* b) {
       size_t sm = 0;
       const int tiles = prep_reg<T>(c, p, ax.N1, ax.N2, 4, &threads, &sm);
       // the cube engine needs every thread of the workgroup on a line
-      if (ax.N2 == 0 && threads != kPersistThreads) return;
+      if (ax.N2 <= 0 && threads != kPersistThreads) return;
       if (threads > kPersistThreads) return;
       smem = std::max(smem, sm);
       args.push_back(p);
@@ -3457,6 +3518,22 @@ int adscf_get_schedule(const adscf_ctx* c, int axis, int* n, int* n1, int* n2, i
   return ADSCF_OK;
 }
 
+int adscf_set_pass_tiles(adscf_ctx* c, int axis, int tiles) {
+  return guarded(c, [&]() -> int {
+    if (axis < 0 || axis > 2 || (tiles != 1 && tiles != 2 && tiles != 4)) throw ArgError{"tiles must be 1, 2 or 4"};
+    c->ax[axis].tiles = tiles;
+    invalidate_tables(c);
+    ensure_current_tables(c);
+    return ADSCF_OK;
+  });
+}
+
+int adscf_get_pass_tiles(const adscf_ctx* c, int axis, int* tiles) {
+  if (!c || axis < 0 || axis > 2 || !tiles) return ADSCF_ERR_ARG;
+  *tiles = c->ax[axis].tiles;
+  return ADSCF_OK;
+}
+
 int adscf_set_schedule(adscf_ctx* c, int axis, int n1, int n2, int waves) {
   return guarded(c, [&]() -> int {
     if (axis < 0 || axis > 2 || waves < 1 || waves > 4) throw ArgError{"bad schedule"};
@@ -3466,6 +3543,7 @@ int adscf_set_schedule(adscf_ctx* c, int axis, int n1, int n2, int waves) {
     ax.N2 = n1 ? n2 : 0;
     ax.waves = waves;
     invalidate_tables(c);
+    ensure_current_tables(c);
     return ADSCF_OK;
   });
 }
@@ -3481,6 +3559,7 @@ int adscf_batch_create(adscf_batch** out, int ninst, const int* grid, const doub
   auto* b = new (std::nothrow) adscf_batch();
   if (!b) return ADSCF_ERR_HIP;
   b->prec = precision;
+  if (const char* g = getenv("ADSCF_GRAPH")) b->use_graph = strcmp(g, "0") != 0;
   int rc = bguard(b, [&]() -> int {
     HIPCK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     for (hipStream_t& s : b->side) HIPCK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -3578,8 +3657,14 @@ int adscf_batch_iterate(adscf_batch* b, int niter, double tol, int* iters, doubl
   return bguard(b, [&]() -> int {
     for (adscf_ctx* c : b->inst)
       if (!c->field_set) throw ArgError{"adscf_batch_set_fields has not been called"};
-    if (b->prec == ADSCF_FP64) batch_build_graph<double>(b);
-    else batch_build_graph<float>(b);
+    if (!b->use_graph) {
+      if (b->prec == ADSCF_FP64) batch_build_tables<double>(b);
+      else batch_build_tables<float>(b);
+    } else if (b->prec == ADSCF_FP64) {
+      batch_build_graph<double>(b);
+    } else {
+      batch_build_graph<float>(b);
+    }
     for (adscf_ctx* c : b->inst)
       hipLaunchKernelGGL(k_state_update, dim3(1), dim3(64), 0, b->stream, c->st.p, tol, 1, 0);
     HIPCK(hipGetLastError());
@@ -3612,7 +3697,15 @@ int adscf_batch_iterate(adscf_batch* b, int niter, double tol, int* iters, doubl
     while (done_iters < niter) {
       const int chunk = std::min(niter - done_iters, tol > 0 ? 25 : niter);
       HIPCK(hipEventRecord(b->ev0, b->stream));
-      for (int i = 0; i < chunk; ++i) HIPCK(hipGraphLaunch(b->graph, b->stream));
+      for (int i = 0; i < chunk; ++i) {
+        if (b->use_graph) {
+          HIPCK(hipGraphLaunch(b->graph, b->stream));
+        } else if (b->prec == ADSCF_FP64) {
+          batch_enqueue_lanes<double>(b, true);
+        } else {
+          batch_enqueue_lanes<float>(b, true);
+        }
+      }
       HIPCK(hipEventRecord(b->ev1, b->stream));
       for (int i = 0; i < n; ++i)
         HIPCK(hipMemcpyAsync(&st[i], b->inst[i]->st.p, sizeof(ScfState), hipMemcpyDeviceToHost, b->stream));

@@ -100,4 +100,119 @@ ADSCF_HD void cube_ai(C2<T>* v, const C2<T>* lds, int t, const CubeTables<T, R>&
   dft_reg<T, R, +1>(v);
 }
 
+// ---------------------------------------------------------------------------
+// n = A R^2 on R^2 threads, A points each (A a multiple of R): the cube engine with a
+// longer first stage.  128 = 8 x 4 x 4 on 16 threads keeps every thread busy in every
+// phase, where the 16 x 8 four-step of fft_reg.hpp leaves half of its 16 threads idle in
+// its 16-point phase.  Index j = t + R^2 m (t < R^2, m < A); bin k = k2 + A (c + R d).
+//   A   thread t:             DFT_A over m, twiddle W_n^(t k2)            -> S[t][k2]
+//   B   pair (k2, a), P each:  DFT_R over b of S[a + R b][k2], W_n^(A a c) -> S2[k2 R + c][a]
+//   C   row (k2, c), P each:   DFT_R over a -> bin k2 + A (c + R d), times K, inverse DFT_R
+//   then B and A inverted (conjugate twiddles, inverse DFTs).  P = A / R pairs and rows
+//   per thread, pair/row index p = t + R^2 i.
+// ---------------------------------------------------------------------------
+template <int A, int R>
+struct Tri {
+  static_assert(A % R == 0, "A must be a multiple of R");
+  static constexpr int n = A * R * R;
+  static constexpr int tpl = R * R;
+  static constexpr int np = A;
+  static constexpr int P = A / R;
+  static constexpr int sa = A + 1;  // padded row of S
+  static constexpr int sb = R + 1;  // padded row of S2
+  static constexpr int lds_elems = R * R * sa + A * R * sb;
+};
+
+template <typename T, int A, int R>
+struct TriTables {
+  static constexpr int P = Tri<A, R>::P;
+  C2<T> wa[A];     // W_n^(t k2)
+  C2<T> wb[P][R];  // W_n^(A a c) of pair i
+  T k[P][R];       // K[k2 + A (c + R d)] of row i
+  ADSCF_HD void load(int t, const C2<T>* tw, const T* kf) {
+    constexpr int TT = R * R;
+#pragma unroll
+    for (int q = 0; q < A; ++q) wa[q] = tw[t * q];
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+      const int p = t + TT * i;
+      const int a = p % R, k2 = p / R;  // pair (k2, a); row (k2, c) with c = p % R
+#pragma unroll
+      for (int c = 0; c < R; ++c) wb[i][c] = tw[A * a * c];
+#pragma unroll
+      for (int d = 0; d < R; ++d) k[i][d] = kf[k2 + A * (a + R * d)];
+    }
+  }
+};
+
+template <typename T, int A, int R>
+ADSCF_HD void tri_a(C2<T>* v, C2<T>* lds, int t, const TriTables<T, A, R>& W) {
+  constexpr int sa = Tri<A, R>::sa;
+  dft_reg<T, A, -1>(v);
+#pragma unroll
+  for (int q = 0; q < A; ++q) lds[t * sa + q] = (q > 0 && t > 0) ? cmul(v[q], W.wa[q]) : v[q];
+}
+template <typename T, int A, int R>
+ADSCF_HD void tri_b(C2<T>* lds, int t, const TriTables<T, A, R>& W) {
+  constexpr int sa = Tri<A, R>::sa, sb = Tri<A, R>::sb, TT = R * R;
+  C2<T>* S2 = lds + TT * sa;
+#pragma unroll
+  for (int i = 0; i < Tri<A, R>::P; ++i) {
+    const int p = t + TT * i;
+    const int a = p % R, k2 = p / R;
+    C2<T> u[R];
+#pragma unroll
+    for (int b = 0; b < R; ++b) u[b] = lds[(a + R * b) * sa + k2];
+    dft_reg<T, R, -1>(u);
+#pragma unroll
+    for (int c = 0; c < R; ++c) S2[(k2 * R + c) * sb + a] = (c > 0 && a > 0) ? cmul(u[c], W.wb[i][c]) : u[c];
+  }
+}
+template <typename T, int A, int R>
+ADSCF_HD void tri_c(C2<T>* lds, int t, const TriTables<T, A, R>& W) {
+  constexpr int sa = Tri<A, R>::sa, sb = Tri<A, R>::sb, TT = R * R;
+#pragma unroll
+  for (int i = 0; i < Tri<A, R>::P; ++i) {
+    C2<T>* row = lds + TT * sa + (t + TT * i) * sb;  // row (k2, c) = ((t + TT i) / R, (t + TT i) % R)
+    C2<T> u[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) u[q] = row[q];
+    dft_reg<T, R, -1>(u);
+#pragma unroll
+    for (int d = 0; d < R; ++d) u[d] = cscale(u[d], W.k[i][d]);
+    dft_reg<T, R, +1>(u);
+#pragma unroll
+    for (int q = 0; q < R; ++q) row[q] = u[q];
+  }
+}
+template <typename T, int A, int R>
+ADSCF_HD void tri_bi(C2<T>* lds, int t, const TriTables<T, A, R>& W) {
+  constexpr int sa = Tri<A, R>::sa, sb = Tri<A, R>::sb, TT = R * R;
+  const C2<T>* S2 = lds + TT * sa;
+#pragma unroll
+  for (int i = 0; i < Tri<A, R>::P; ++i) {
+    const int p = t + TT * i;
+    const int a = p % R, k2 = p / R;
+    C2<T> u[R];
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      const C2<T> z = S2[(k2 * R + c) * sb + a];
+      u[c] = (c > 0 && a > 0) ? cmulc(z, W.wb[i][c]) : z;
+    }
+    dft_reg<T, R, +1>(u);
+#pragma unroll
+    for (int b = 0; b < R; ++b) lds[(a + R * b) * sa + k2] = u[b];
+  }
+}
+template <typename T, int A, int R>
+ADSCF_HD void tri_ai(C2<T>* v, const C2<T>* lds, int t, const TriTables<T, A, R>& W) {
+  constexpr int sa = Tri<A, R>::sa;
+#pragma unroll
+  for (int q = 0; q < A; ++q) {
+    const C2<T> z = lds[t * sa + q];
+    v[q] = (q > 0 && t > 0) ? cmulc(z, W.wa[q]) : z;
+  }
+  dft_reg<T, A, +1>(v);
+}
+
 }  // namespace adscf

@@ -319,9 +319,30 @@ struct EngCube {  // n = R^3 on R^2 threads (fft_cube.hpp)
   }
 };
 
-// shape (N1, N2) of the schedule tables: N2 == 0 selects the cube engine with R = N1
+template <typename T, int A, int R>
+struct EngTri {  // n = A R^2 on R^2 threads (fft_cube.hpp, tri_*)
+  static constexpr int NP = A, TPL = R * R, NACT = R * R, STRIDE = R * R;
+  static constexpr int LDS = Tri<A, R>::lds_elems;
+  using Tables = TriTables<T, A, R>;
+  static __device__ __forceinline__ void op(C2<T> (&v)[NP], C2<T>* lds, int t, const Tables& W) {
+    tri_a<T, A, R>(v, lds, t, W);
+    __syncthreads();
+    tri_b<T, A, R>(lds, t, W);
+    __syncthreads();
+    tri_c<T, A, R>(lds, t, W);
+    __syncthreads();
+    tri_bi<T, A, R>(lds, t, W);
+    __syncthreads();
+    tri_ai<T, A, R>(v, lds, t, W);
+  }
+};
+
+// shape (N1, N2) of the schedule tables: N2 == 0 selects the cube engine with R = N1,
+// N2 < 0 the A R^2 engine with A = -N2 and R = N1, N2 > 0 the N1 x N2 four-step
 template <typename T, int N1, int N2>
-using PassEngine = typename std::conditional<N2 == 0, EngCube<T, N1>, EngFour<T, N1, N2 == 0 ? 1 : N2>>::type;
+using PassEngine = typename std::conditional<
+    N2 == 0, EngCube<T, N1>,
+    typename std::conditional<(N2 < 0), EngTri<T, (N2 < 0 ? -N2 : N1), N1>, EngFour<T, N1, N2 <= 0 ? 1 : N2>>::type>::type;
 
 // Pass variants fixed at compile time.  The four shapes of a propagator sweep
 // (first pass from q = 1, plain single-operator pass, merged pass with the mid
@@ -552,6 +573,134 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
   }
 }
 
+// Tile of the K-tile pass: block g (on XCD g % 8) takes tiles j = 0..K-1 of its XCD's
+// contiguous range, so that the tiles in flight together on one XCD stay neighbours.
+// Returns -1 past the range.  gridDim.x is a multiple of kXcds (launch_reg sizes it).
+__device__ __forceinline__ int xcd_tile_k(int g, int j, int nb, int G) {
+  const int x = g % 8, i = g / 8, gx = G / 8;
+  const int q = nb / 8, r = nb % 8;
+  const int start = x * q + (x < r ? x : r), len = q + (x < r ? 1 : 0);
+  const int k = i + j * gx;
+  return k < len ? start + k : -1;
+}
+
+// A pass whose workgroups each take K tiles in turn (fixed sweep modes only): the loads of
+// tile j+1 are issued before tile j is transformed and stored, so that a wave's memory
+// traffic overlaps its own arithmetic and the reads of some waves overlap the writes of
+// others (with one tile per wave every wave of a large grid reads, computes and writes in
+// step).  Every point takes the operations of reg_pass_body in the same order, so results
+// are bit-identical to the one-tile pass.
+template <typename T, class ENG, int MODE, int K>
+__device__ __forceinline__ void reg_pass_tiles(const PassArgs<T>& a, int nitems, unsigned char* smem_raw) {
+  static_assert(MODE != PM_GENERIC, "fixed modes only");
+  constexpr int NP = ENG::NP, STRIDE = ENG::STRIDE, TPL = ENG::TPL;
+  constexpr bool FIRST = MODE == PM_FIRST, MERGED = MODE == PM_MERGED, LAST = MODE == PM_LAST;
+  const int tid = threadIdx.x;
+  int l, t;
+  if (a.axis == 0) {
+    t = tid % TPL;
+    l = tid / TPL;
+  } else {
+    l = tid % a.L;
+    t = tid / a.L;
+  }
+  C2<T>* lds = reinterpret_cast<C2<T>*>(smem_raw) + l * ENG::LDS;
+  const int stop = *(a.done ? a.done : &g_no_stop);
+  const unsigned fstep = (unsigned)(STRIDE * a.stride) * (unsigned)sizeof(T);
+  struct Tile {
+    C2<T> v[NP];
+    T ea1[NP], ea2[NP], eb1[NP], eb2[NP];
+    unsigned f0;
+    bool act;
+  };
+  auto load = [&](int item, Tile& w) {
+    bool valid = false;
+    long base = 0;
+    if (item >= 0) base = pass_line_base(a, item, l, valid);
+    w.act = item >= 0 && t < ENG::NACT && valid;
+    w.f0 = (unsigned)(base + (long)t * a.stride) * (unsigned)sizeof(T);
+#pragma unroll
+    for (int j2 = 0; j2 < NP; ++j2) {
+      const unsigned fo = w.f0 + (unsigned)j2 * fstep;
+      C2<T> x = {T(0), T(0)};
+      w.ea1[j2] = w.ea2[j2] = w.eb1[j2] = w.eb2[j2] = T(1);
+      if (w.act) {
+        if (FIRST) x = {T(1), T(1)};
+        else x = ldo(a.tmp_in, 2u * fo);
+        if (FIRST || MERGED || LAST) {
+          w.ea1[j2] = ldo(a.Ea1, fo);
+          w.ea2[j2] = ldo(a.Ea2, fo);
+        }
+        if (MERGED) {
+          w.eb1[j2] = ldo(a.Eb1, fo);
+          w.eb2[j2] = ldo(a.Eb2, fo);
+        }
+      }
+      w.v[j2] = x;
+    }
+  };
+  typename ENG::Tables R;
+  R.load(t, a.tw, a.kf);
+  const T sc1 = (MERGED || LAST) ? exp2i<T>(a.dexp1[a.sidx]) : T(1);
+  const T sc2 = (MERGED || LAST) ? exp2i<T>(a.dexp2[a.sidx]) : T(1);
+  const int G = gridDim.x;
+  Tile cur, nxt;
+  int it_cur = xcd_tile_k(blockIdx.x, 0, nitems, G);
+  load(it_cur, cur);
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int it_nxt = j + 1 < K ? xcd_tile_k(blockIdx.x, j + 1, nitems, G) : -1;
+    if (j + 1 < K) load(it_nxt, nxt);
+    if (it_cur >= 0) {  // uniform over the block
+      if (FIRST) {
+#pragma unroll
+        for (int j2 = 0; j2 < NP; ++j2) {
+          cur.v[j2].x *= cur.ea1[j2];
+          cur.v[j2].y *= cur.ea2[j2];
+        }
+      }
+      ENG::op(cur.v, lds, t, R);
+      if (stop) return;  // uniform over the block
+      if (MERGED) {
+        if (cur.act) {
+#pragma unroll
+          for (int j2 = 0; j2 < NP; ++j2) {
+            const unsigned fo = cur.f0 + (unsigned)j2 * fstep;
+            C2<T> x = cur.v[j2];
+            x.x = x.x * cur.ea1[j2] * sc1;
+            x.y = x.y * cur.ea2[j2] * sc2;
+            sto(a.h1, fo, x.x);
+            sto(a.h2, fo, x.y);
+            x.x *= cur.eb1[j2];
+            x.y *= cur.eb2[j2];
+            cur.v[j2] = x;
+          }
+        }
+        ENG::op(cur.v, lds, t, R);
+      }
+      if (cur.act) {
+#pragma unroll
+        for (int j2 = 0; j2 < NP; ++j2) {
+          const unsigned fo = cur.f0 + (unsigned)j2 * fstep;
+          if (LAST) {
+            C2<T> x = cur.v[j2];
+            x.x = x.x * cur.ea1[j2] * sc1;
+            x.y = x.y * cur.ea2[j2] * sc2;
+            sto(a.h1, fo, x.x);
+            sto(a.h2, fo, x.y);
+          } else {
+            sto(a.tmp_out, 2u * fo, cur.v[j2]);
+          }
+        }
+      }
+    }
+    if (j + 1 < K) {
+      cur = nxt;
+      it_cur = it_nxt;
+    }
+  }
+}
+
 // Makes every kernel-argument field the pass reads live in scalar registers at
 // entry, so the compiler issues their constant-cache loads as one batch rather
 // than one batch per branch of the pass body.  (The empty asm emits nothing.)
@@ -581,6 +730,14 @@ __global__ void __launch_bounds__(kThreads) k_line_pass_reg(PassArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
   if constexpr (MODE != PM_GENERIC) pin_pass_args(a);
   reg_pass_body<T, PassEngine<T, N1, N2>, MODE>(a, xcd_tile(blockIdx.x, gridDim.x), smem_raw);
+}
+
+// K tiles per workgroup (reg_pass_tiles); nitems is the tile count of the pass
+template <typename T, int N1, int N2, int MODE, int K>
+__global__ void __launch_bounds__(kThreads) k_line_pass_tiles(PassArgs<T> a, int nitems) {
+  extern __shared__ unsigned char smem_raw[];
+  pin_pass_args(a);
+  reg_pass_tiles<T, PassEngine<T, N1, N2>, MODE, K>(a, nitems, smem_raw);
 }
 
 // One pass of many independent instances that share a register shape, in one

@@ -267,7 +267,8 @@ int adscf_epd_step(adscf_ctx* ctx, double timestep, double accuracy_scf, int ite
  * adscf_scf_setup + adscf_scf_set_field.  report (may be NULL) receives the timings. */
 int adscf_autotune(adscf_ctx* ctx, int reps, char* report, int report_cap);
 /* current schedule of an axis (0 x, 1 y, 2 z): n1 x n2 four-step register engine, n1 = 0 the
- * Stockham kernel, n2 = 0 the n1^3 cube engine (n = n1^3 points, n1^2 threads per line) */
+ * Stockham kernel, n2 = 0 the n1^3 cube engine (n = n1^3 points, n1^2 threads per line),
+ * n2 < 0 the (-n2) n1^2 engine (n1^2 threads per line, -n2 points each: 128 = 8 x 4 x 4) */
 int adscf_get_schedule(const adscf_ctx* ctx, int axis, int* n, int* n1, int* n2, int* waves);
 /* The axis whose operator gets one line pass per contour step in a 3-D sweep; the operators
  * of the other two meet across consecutive steps (the order alternates a,b,c / c,b,a) and
@@ -275,6 +276,12 @@ int adscf_get_schedule(const adscf_ctx* ctx, int axis, int* n, int* n1, int* n2,
 int adscf_set_middle_axis(adscf_ctx* ctx, int axis);
 int adscf_get_middle_axis(const adscf_ctx* ctx, int* axis);
 int adscf_set_schedule(adscf_ctx* ctx, int axis, int n1, int n2, int waves);
+/* tiles per workgroup (1, 2 or 4; FP64 1 or 2) of the sweep passes along an axis with a
+ * register schedule of 32 points or more: a workgroup issues the loads of its next tile
+ * before it transforms and stores the current one (same results bit for bit; other settings
+ * run one tile); adscf_autotune also chooses it */
+int adscf_set_pass_tiles(adscf_ctx* ctx, int axis, int tiles);
+int adscf_get_pass_tiles(const adscf_ctx* ctx, int axis, int* tiles);
 
 /* ---- batched independent instances (no counterpart in the reference) ---------- */
 /* Many independent two-type SCF problems (parameter sweeps) of uneven grid and contour

@@ -304,7 +304,7 @@ def test_scf_continues_after_convergence(amd, orc):
 @pytest.mark.parametrize("n,shape", [(32, (4, 8)), (32, (8, 4)), (32, (2, 16)), (48, (8, 6)), (48, (6, 8)),
                                      (48, (16, 3)), (64, (8, 8)), (64, (4, 16)), (64, (16, 4)), (24, (8, 3)),
                                      (24, (4, 6)), (128, (8, 16)), (128, (16, 8)), (16, (1, 16)), (64, (0, 0)),
-                                     (64, (4, 0))])
+                                     (64, (4, 0)), (128, (4, -8)), (256, (4, -16))])
 @pytest.mark.parametrize("waves", [1, 2, 4])
 def test_every_schedule_matches_oracle(amd, orc, n, shape, waves):
     if shape[0] == 0 and waves > 1:
@@ -320,6 +320,49 @@ def test_every_schedule_matches_oracle(amd, orc, n, shape, waves):
         got = ctx.propagate_chain([1, 2], [6, 9], E)
         assert _rel(got.density, ref.density) < FP64_RTOL
         assert abs(got.Qsum - ref.Qsum) < FP64_RTOL * ref.Qsum
+
+
+@pytest.mark.parametrize("shape,tiles", [((4, 0), 2), ((4, 0), 4), ((8, 8), 2), ((16, 4), 4), ((4, 16), 2)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_pass_tiles_bitwise(amd, shape, tiles, prec):
+    """Passes that take 2 or 4 tiles per workgroup (the loads of the next tile issued before
+    the current one is transformed) give the one-tile results bit for bit, on every axis of a
+    64 x 48 x 64 grid (x: contiguous lines, y/z: strided; 48 keeps its default shape)."""
+    P = amd.FP64 if prec == "fp64" else amd.FP32
+    dims = (64, 48, 64)
+    print(f"pass tiles shape={shape} tiles={tiles} {prec}", flush=True)
+    rng = np.random.default_rng(13)
+    E = np.exp(-0.005 * 2.0 * rng.standard_normal((2, dims[2], dims[1], dims[0])))
+    out = []
+    for k in (1, tiles):
+        with amd.Context(*dims, 3.0, 2.5, 3.5, 0.01, precision=P) as ctx:
+            for axis in (0, 2):
+                ctx.set_schedule(axis, shape[0], shape[1], 1)
+                ctx.set_pass_tiles(axis, k)
+            ctx.set_pass_tiles(1, k)
+            r = ctx.propagate_chain([1, 2], [13, 8], E)
+            out.append((r.density, r.Qsum))
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
+
+
+@pytest.mark.parametrize("axis", [1, 2])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_tri_engine_on_strided_axes(amd, orc, axis, prec):
+    """The 8 x 4 x 4 engine of 128-point lines on the y and z axes (lines strided in memory,
+    tiles of adjacent x), FP64 against the oracle at 1e-10 and FP32-mixed at its 2e-5 bound."""
+    dims = [6, 10, 8]
+    dims[axis] = 128
+    P = amd.FP64 if prec == "fp64" else amd.FP32
+    print(f"tri8x4x4 axis={axis} dims={dims} {prec}", flush=True)
+    with amd.Context(*dims, 3.0, 2.5, 3.5, 0.01, precision=P) as ctx:
+        ctx.set_schedule(axis, 4, -8, 1)
+        g = orc.Grid(*dims, 3.0, 2.5, 3.5, 0.01)
+        E = orc.expfield(g, _fields(g.shape, 9, amp=2.0))
+        ref = orc.propagate_chain(g, [1, 2], [6, 9], E)
+        got = ctx.propagate_chain([1, 2], [6, 9], E)
+        tol = FP64_RTOL if prec == "fp64" else 2e-5
+        assert _rel(got.density, ref.density) < tol
+        assert abs(got.Qsum - ref.Qsum) < tol * ref.Qsum
 
 
 def test_autotune_keeps_parity(amd, orc):
