@@ -84,9 +84,10 @@ def test_config_128cubed_fp32_mixed_errors_against_fp64():
 def test_config_128cubed_refinement_switch():
     """The FP64 refinement phase of the large configuration: the FP32-mixed loop (lam_mix 0.3)
     runs until max|dfield| < 3e-2, then adscf_set_precision(FP64) continues the same loop and
-    history.  After the switch the loop computes in FP64: an evaluation of its field equals an
-    independent FP64 evaluation of the same field bit for bit (before the switch the two differ
-    by FP32 rounding), and the iteration count carries on.  (The reference's Anderson form
+    history.  After the switch the loop computes in FP64: an evaluation of its field agrees
+    with an independent FP64 evaluation of the same field to FP64 rounding (1e-12; the two
+    contexts keep the line-FFT schedules of their starting precisions), where before the switch
+    the two differ by FP32 rounding, and the iteration count carries on.  (The reference's Anderson form
     approaches the homogeneous state F = chiN/4 - 1 per chain from this seed too slowly to
     reach 1e-5 in a test: 2.0e-3 after 3000 iterations at lam_mix 0.3.)"""
     import adscf_amd
@@ -120,7 +121,7 @@ def test_config_128cubed_refinement_switch():
         ref.scf_evaluate()
         _, d64 = ref.scf_get()
     e32 = float(np.max(np.abs(d32 - d64)) / np.max(d64))
-    print(f"densities at the switch field: FP32-mixed vs FP64 {e32:.2e}; after the switch "
-          f"{float(np.max(np.abs(d_after - d64))):.1e}")
+    e64 = float(np.max(np.abs(d_after - d64)) / np.max(d64))
+    print(f"densities at the switch field: FP32-mixed vs FP64 {e32:.2e}; after the switch {e64:.1e}")
     assert 0.0 < e32 < 1e-4
-    assert np.array_equal(d_after, d64)
+    assert e64 < 1e-12 and e64 < 1e-4 * e32
