@@ -526,6 +526,37 @@ def pmc_summary(db, pattern="."):
                                                                       name[:100]))
 
 
+def trace_summary(db, skip=0.0):
+    """Per-kernel durations of a rocprofv3 kernel-trace database (rocpd, SQLite), as a
+    markdown table: calls, mean and median duration, total, share of the summed kernel
+    time, grid, VGPRs and LDS; the first `skip` fraction of the dispatches (warm-up) is left
+    out."""
+    import collections
+    import re
+    import sqlite3
+    import statistics
+
+    con = sqlite3.connect("file:%s?mode=ro" % db, uri=True)
+    rows = con.execute("select name, start, end, grid_x, workgroup_x, vgpr_count, lds_size from kernels "
+                       "order by start").fetchall()
+    rows = rows[int(len(rows) * skip):]
+    agg, meta = collections.defaultdict(list), {}
+    for name, t0, t1, gx, wx, vg, lds in rows:
+        agg[name].append(t1 - t0)
+        meta[name] = (gx // max(wx, 1), wx, vg, lds)
+    tot = sum(sum(v) for v in agg.values())
+    print("| kernel | calls | mean us | median us | total ms | share | workgroups x threads | VGPR | LDS B |")
+    print("|:--|--:|--:|--:|--:|--:|--:|--:|--:|")
+    for name, v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+        wg, wx, vg, lds = meta[name]
+        print("| `%s` | %d | %.2f | %.2f | %.3f | %.1f%% | %d x %d | %d | %d |" % (
+            re.sub(r"\(.*", "", name)[:70], len(v), statistics.mean(v) / 1e3, statistics.median(v) / 1e3,
+            sum(v) / 1e6, 100.0 * sum(v) / tot, wg, wx, vg, lds))
+    if rows:
+        print("\nspan of the listed dispatches %.3f ms, kernel time %.3f ms" % ((rows[-1][2] - rows[0][1]) / 1e6,
+                                                                              tot / 1e6))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -549,9 +580,15 @@ def main():
     ap.add_argument("--iters", type=int, default=30, help="iterations of --profile-run")
     ap.add_argument("--pmc-summary", metavar="DB", help="summarise the counters of a rocprofv3 database and exit")
     ap.add_argument("--pmc-regex", default=".", help="kernel-name filter of --pmc-summary")
+    ap.add_argument("--trace-summary", metavar="DB", help="summarise a rocprofv3 kernel-trace database and exit")
+    ap.add_argument("--trace-skip", type=float, default=0.0,
+                    help="fraction of the first dispatches --trace-summary leaves out (warm-up)")
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(args.pmc_summary, args.pmc_regex)
+        return
+    if args.trace_summary:
+        trace_summary(args.trace_summary, args.trace_skip)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the SCFT path runs on a single MI355X; "
