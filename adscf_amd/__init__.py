@@ -85,6 +85,7 @@ SIGNATURES = {
     "adscf_scf_timing": (C.c_int, [_ctxp, _dp, _dp]),
     "adscf_scf_set_residual": (C.c_int, [_ctxp, C.c_int, C.c_int, _dp]),
     "adscf_scf_set_mixing": (C.c_int, [_ctxp, C.c_int]),
+    "adscf_set_contour_rule": (C.c_int, [_ctxp, C.c_int]),
     "adscf_scf_set_density": (C.c_int, [_ctxp, _dp]),
     "adscf_conjugate_fields": (C.c_int, [_ctxp, _dp, C.c_double, C.c_int, C.c_int, _ip, _dp]),
     "adscf_mobility_qq": (C.c_int, [_ctxp, C.c_int, C.c_double, _dp, C.c_double, _dp]),
@@ -445,6 +446,12 @@ class Context:
     def scf_set_mixing(self, mixing_type=3):
         """Mixing of scf_iterate: 1 simple, 2 lambda (lammax = lam_mix), 3 Anderson."""
         self._chk(self._lib.adscf_scf_set_mixing(self._h, int(mixing_type)))
+
+    def set_contour_rule(self, rule="trapezoid"):
+        """Contour rule of every density evaluation: "trapezoid" (the reference's, default) or
+        "simpson" (composite Simpson per block, a 3/8 panel closing odd blocks)."""
+        code = {"trapezoid": 0, "simpson": 1}.get(rule, rule)
+        self._chk(self._lib.adscf_set_contour_rule(self._h, int(code)))
 
     # -- conjugate fields and DDFT (main_multiblock.f90) -------------------------
     def conjugate_fields(self, density=None, thresh=5e-4, maxit=2000, measure=0):

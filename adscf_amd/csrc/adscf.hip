@@ -184,6 +184,7 @@ struct adscf_ctx {
   int mid_axis = -1;  // axis of the single-operator passes of a 3-D sweep (-1: y); the others merge
   int done_loop = -1;  // loop kind ((mixing*2 + resid_kind)*4 + measure) of the call that converged
   int last_loop = -1;  // loop kind of the last call since adscf_scf_set_field (-1: none yet)
+  int contour_rule = RULE_TRAPEZOID;  // adscf_set_contour_rule
   DevBuf<double> gsum_w;                 // lambda mixing: Gram sums of the field ring
   DevBuf<double> target;             // target densities (nt, ng): the DDFT density
   DevBuf<double> mob;                // mobility_qq (nk, nt, nt)
@@ -720,6 +721,7 @@ ContourArgs<T> contour_args(adscf_ctx* c, const ChainDesc& cd, const ScfState* s
   a.fh = c->fh;
   a.fh.nt = c->nt;
   a.ds = c->ds;
+  a.rule = c->contour_rule;
   a.h1 = En.h1.p;
   a.h2 = En.h2.p;
   a.e1 = c->e1.p;
@@ -1639,16 +1641,18 @@ __global__ void k_acc_partials(const double* __restrict__ part, int n, double* _
 
 struct PosWeights {
   double w1 = 0.0, w2 = 0.0;
-  int b1 = -1, b2 = -1;  // blocks whose trapezoid covers contour point s (two at a junction)
+  int b1 = -1, b2 = -1;  // blocks whose contour integral covers point s (two at a junction)
 };
 
-// trapezoid weights of contour point s (mod_multiblock.f90:157-168)
-PosWeights pos_weights(const ChainDesc& cd, double ds, int s) {
+// contour weights of point s in the blocks that cover it (mod_multiblock.f90:157-168 for
+// the trapezoid; contour_weight for both rules).  Chains reaching the adjoint have no empty
+// block, so at most two blocks meet at a point.
+PosWeights pos_weights(const ChainDesc& cd, int rule, double ds, int s) {
   PosWeights pw;
   for (int b = 0; b < cd.nblocks; ++b) {
     const int s0 = cd.start[b], s1 = cd.start[b] + cd.seg[b];
     if (s < s0 || s > s1) continue;
-    const double w = (s == s0 || s == s1) ? 0.5 * ds : ds;
+    const double w = contour_weight(rule, s - s0, cd.seg[b], ds);
     if (pw.b1 < 0) {
       pw.b1 = b;
       pw.w1 = w;
@@ -1810,7 +1814,7 @@ int adjoint_impl(adscf_ctx* c, const ChainDesc& cd, double rho, const double* fi
     R.recompute(a, b - a, En.sa1.p, pb, ns - a - pb, En.sa2.p);
     const int last = (b == ns) ? b : b - 1;
     for (int s = a; s <= last; ++s) {
-      const PosWeights pw = pos_weights(cd, c->ds, s);
+      const PosWeights pw = pos_weights(cd, c->contour_rule, c->ds, s);
       hipLaunchKernelGGL(k_adj_contour<T>, dim3(nb), dim3(kThreads), 0, c->stream,
                          (const T*)(En.sa1.p + (long)(s - a) * ng), (const T*)(En.sa2.p + (long)(ns - s - pb) * ng), ng,
                          pw.w1, pw.b1, pw.w2, pw.b2, dist.p);
@@ -1858,8 +1862,8 @@ int adjoint_impl(adscf_ctx* c, const ChainDesc& cd, double rho, const double* fi
   spart.ensure((size_t)ng + 64);
   sacc.ensure(3);
   HIPCK(hipMemsetAsync(sacc.p, 0, 3 * sizeof(double), c->stream));
-  const Seed sN = seed_of(cd, pos_weights(cd, c->ds, ns));
-  const Seed s0 = seed_of(cd, pos_weights(cd, c->ds, 0));
+  const Seed sN = seed_of(cd, pos_weights(cd, c->contour_rule, c->ds, ns));
+  const Seed s0 = seed_of(cd, pos_weights(cd, c->contour_rule, c->ds, 0));
   hipLaunchKernelGGL(k_adj_init<T>, dim3(nb), dim3(kThreads), 0, c->stream, (const double*)g.p, ng, A, sN, s0, dLdS,
                      En.mu.p, En.nu.p);
   int cur = 0;
@@ -1900,8 +1904,8 @@ int adjoint_impl(adscf_ctx* c, const ChainDesc& cd, double rho, const double* fi
       AdjStepArgs<T> st{};
       st.ng = ng;
       st.A = A;
-      st.seed_q = seed_of(cd, pos_weights(cd, c->ds, s - 1));
-      st.seed_p = seed_of(cd, pos_weights(cd, c->ds, ns - s + 1));
+      st.seed_q = seed_of(cd, pos_weights(cd, c->contour_rule, c->ds, s - 1));
+      st.seed_p = seed_of(cd, pos_weights(cd, c->contour_rule, c->ds, ns - s + 1));
       st.g = g.p;
       st.Eq = R.Eof(0, s);
       st.Ep = R.Eof(1, s);
@@ -3293,6 +3297,18 @@ int adscf_scf_set_density(adscf_ctx* c, const double* density) {
   });
 }
 
+int adscf_set_contour_rule(adscf_ctx* c, int rule) {
+  return guarded(c, [&]() -> int {
+    if (rule != RULE_TRAPEZOID && rule != RULE_SIMPSON) throw ArgError{"rule must be 0 (trapezoid) or 1 (Simpson)"};
+    if (rule != c->contour_rule) {
+      c->contour_rule = rule;
+      c->keep.retire(c->graph_src, c->graph);  // the captured iteration bakes the rule in
+      c->phi_fresh = false;
+    }
+    return ADSCF_OK;
+  });
+}
+
 int adscf_scf_set_mixing(adscf_ctx* c, int mixing_type) {
   return guarded(c, [&]() -> int {
     if (mixing_type < 1 || mixing_type > 3) throw ArgError{"mixing_type must be 1 (simple), 2 (lambda) or 3 (Anderson)"};
@@ -3469,6 +3485,8 @@ int adscf_adjoint(adscf_ctx* c, int nblocks, const int* monomer_type, const int*
     if (!c->fh_set) throw ArgError{"adscf_set_interactions has not been called"};
     if (target_type < 1 || target_type > c->nt) throw ArgError{"target_type out of range 1..ntypes"};
     ChainDesc cd = make_chain(c, nblocks, monomer_type, block_segments, 1);
+    for (int b = 0; b < cd.nblocks; ++b)
+      if (cd.seg[b] < 1) throw ArgError{"adscf_adjoint: every block needs at least one contour segment"};
     return c->prec == ADSCF_FP64
                ? adjoint_impl<double>(c, cd, rho, field, target_type - 1, phi_target, lam, checkpoint_every, loss,
                                       free_energy, grad_field, grad_box, phi)

@@ -224,6 +224,29 @@ __global__ void k_qfinal(const double* __restrict__ part, int nblk, const int* _
 }
 
 enum ContourMode : int { CONTOUR_BLOCKS = 0, CONTOUR_SCF = 1 };
+enum ContourRule : int { RULE_TRAPEZOID = 0, RULE_SIMPSON = 1 };
+
+// Weight of contour point s0 + j of a block of nb segments (0 <= j <= nb).
+// RULE_TRAPEZOID is the reference's rule (mod_multiblock.f90:157-168): ds/2 at both ends
+// and ds inside; an empty block (nb = 0) takes both halves at its one point.
+// RULE_SIMPSON is composite Simpson: 1/3-rule panels of two intervals (ds/3 [1 4 2 ... 4 1]);
+// an odd nb >= 3 ends with one 3/8-rule panel over the last three intervals
+// (3 ds/8 [1 3 3 1]), so the rule stays exact for cubics in s; nb < 2 keeps the trapezoid.
+// Both rules' weights sum to nb ds (nb = 0: ds), so the canonical mass is unchanged.
+__host__ __device__ inline double contour_weight(int rule, int j, int nb, double ds) {
+  if (rule != RULE_SIMPSON || nb < 2) {
+    if (nb == 0) return ds;
+    return (j == 0 || j == nb) ? 0.5 * ds : ds;
+  }
+  const int n13 = (nb & 1) ? nb - 3 : nb;  // intervals under 1/3-rule panels
+  double w = 0.0;
+  if (n13 > 0 && j <= n13) w += (j == 0 || j == n13) ? ds / 3.0 : ((j & 1) ? 4.0 * ds / 3.0 : 2.0 * ds / 3.0);
+  if (nb & 1) {
+    const int k = j - n13;
+    if (k >= 0) w += (k == 0 || k == 3) ? 3.0 * ds / 8.0 : 9.0 * ds / 8.0;
+  }
+  return w;
+}
 
 template <typename T>
 struct ContourArgs {
@@ -237,6 +260,7 @@ struct ContourArgs {
   const int* e2;
   const ScfState* st; // norm
   int mode;
+  int rule;           // ContourRule
   double* dist;       // CONTOUR_BLOCKS: (nb, ng) or null
   double* dens;       // CONTOUR_BLOCKS: (nb, ng) or null
   const double* w0;   // CONTOUR_SCF: field ring base; slot `head` is the current field (nt, ng)
@@ -259,7 +283,8 @@ struct alignas(sizeof(T) * VP) PackT {
 };
 
 // Trapezoid contour integral (mod_multiblock.f90:157-168) in the reference's
-// summation order, then the ensemble normalisation (173-180).  In SCF mode the
+// summation order (or, with RULE_SIMPSON, the weights of contour_weight in contour
+// order), then the ensemble normalisation (173-180).  In SCF mode the
 // species densities (main_multiblock.f90:530-534) and the residual follow in the
 // same pass: either the mean fields (mod_FloryHuggins.f90:104-129) and
 // dfield = new - field (main_multiblock.f90:674-676), or, for the conjugate-field
@@ -293,39 +318,65 @@ __global__ void __launch_bounds__(kThreads) k_contour(ContourArgs<T> a) {
     for (int b = 0; b < a.cd.nblocks; ++b) {
       const int s0 = a.cd.start[b], nb = a.cd.seg[b];
       double acc[VP];
-      {
-        const P x1 = ld(a.h1, s0, i0), x2 = ld(a.h2, ns - s0, i0);
-        const double sc = scl(s0);
+      if (a.rule == RULE_SIMPSON && nb >= 2) {
 #pragma unroll
-        for (int v = 0; v < VP; ++v) acc[v] = ((hds * (double)x1.v[v]) * (double)x2.v[v]) * sc;
-      }
-      int s = s0 + 1;
-      for (; s + U <= s0 + nb; s += U) {
-        P x1[U], x2[U];
-        double sc[U];
+        for (int v = 0; v < VP; ++v) acc[v] = 0.0;
+        int j = 0;
+        for (; j + U <= nb + 1; j += U) {
+          P x1[U], x2[U];
+          double sc[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          x1[u] = ld(a.h1, s + u, i0);
-          x2[u] = ld(a.h2, ns - s - u, i0);
-          sc[u] = scl(s + u);
+          for (int u = 0; u < U; ++u) {
+            x1[u] = ld(a.h1, s0 + j + u, i0);
+            x2[u] = ld(a.h2, ns - s0 - j - u, i0);
+            sc[u] = scl(s0 + j + u) * contour_weight(RULE_SIMPSON, j + u, nb, ds);
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int v = 0; v < VP; ++v) acc[v] = acc[v] + ((double)x1[u].v[v] * (double)x2[u].v[v]) * sc[u];
         }
+        for (; j <= nb; ++j) {
+          const P x1 = ld(a.h1, s0 + j, i0), x2 = ld(a.h2, ns - s0 - j, i0);
+          const double sc = scl(s0 + j) * contour_weight(RULE_SIMPSON, j, nb, ds);
 #pragma unroll
-        for (int u = 0; u < U; ++u)
+          for (int v = 0; v < VP; ++v) acc[v] = acc[v] + ((double)x1.v[v] * (double)x2.v[v]) * sc;
+        }
+      } else {
+        {
+          const P x1 = ld(a.h1, s0, i0), x2 = ld(a.h2, ns - s0, i0);
+          const double sc = scl(s0);
 #pragma unroll
-          for (int v = 0; v < VP; ++v) acc[v] = acc[v] + ((ds * (double)x1[u].v[v]) * (double)x2[u].v[v]) * sc[u];
-      }
-      for (; s <= s0 + nb - 1; ++s) {
-        const P x1 = ld(a.h1, s, i0), x2 = ld(a.h2, ns - s, i0);
-        const double sc = scl(s);
+          for (int v = 0; v < VP; ++v) acc[v] = ((hds * (double)x1.v[v]) * (double)x2.v[v]) * sc;
+        }
+        int s = s0 + 1;
+        for (; s + U <= s0 + nb; s += U) {
+          P x1[U], x2[U];
+          double sc[U];
 #pragma unroll
-        for (int v = 0; v < VP; ++v) acc[v] = acc[v] + ((ds * (double)x1.v[v]) * (double)x2.v[v]) * sc;
-      }
-      {
-        const int se = s0 + nb;
-        const P x1 = ld(a.h1, se, i0), x2 = ld(a.h2, ns - se, i0);
-        const double sc = scl(se);
+          for (int u = 0; u < U; ++u) {
+            x1[u] = ld(a.h1, s + u, i0);
+            x2[u] = ld(a.h2, ns - s - u, i0);
+            sc[u] = scl(s + u);
+          }
 #pragma unroll
-        for (int v = 0; v < VP; ++v) acc[v] = acc[v] + ((hds * (double)x1.v[v]) * (double)x2.v[v]) * sc;
+          for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int v = 0; v < VP; ++v) acc[v] = acc[v] + ((ds * (double)x1[u].v[v]) * (double)x2[u].v[v]) * sc[u];
+        }
+        for (; s <= s0 + nb - 1; ++s) {
+          const P x1 = ld(a.h1, s, i0), x2 = ld(a.h2, ns - s, i0);
+          const double sc = scl(s);
+#pragma unroll
+          for (int v = 0; v < VP; ++v) acc[v] = acc[v] + ((ds * (double)x1.v[v]) * (double)x2.v[v]) * sc;
+        }
+        {
+          const int se = s0 + nb;
+          const P x1 = ld(a.h1, se, i0), x2 = ld(a.h2, ns - se, i0);
+          const double sc = scl(se);
+#pragma unroll
+          for (int v = 0; v < VP; ++v) acc[v] = acc[v] + ((hds * (double)x1.v[v]) * (double)x2.v[v]) * sc;
+        }
       }
 #pragma unroll
       for (int v = 0; v < VP; ++v) {

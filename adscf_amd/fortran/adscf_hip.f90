@@ -189,6 +189,12 @@ module adscf_hip
       type(c_ptr), value :: ctx
       real(c_double), intent(in) :: field(*)
     end function
+    ! 0 trapezoid (the reference's contour rule, the default), 1 composite Simpson per block
+    integer(c_int) function adscf_set_contour_rule(ctx, rule) bind(C, name='adscf_set_contour_rule')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: rule
+    end function
     integer(c_int) function adscf_scf_iterate(ctx, niter, tol, iters, accuracy, converged) &
         bind(C, name='adscf_scf_iterate')
       import :: c_ptr, c_int, c_double

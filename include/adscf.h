@@ -206,6 +206,15 @@ int adscf_scf_set_density(adscf_ctx* ctx, const double* density);
  * both drivers do. */
 int adscf_scf_set_mixing(adscf_ctx* ctx, int mixing_type);
 
+/* Contour rule of every density evaluation of the context (chains, homopolymers, the SCF
+ * loop, the adjoint).  0: the reference's trapezoid (Polymer_Modules/mod_multiblock.f90:157-168,
+ * mod_diblock.f90:143-154), the default and the rule parity is stated for.  1: composite
+ * Simpson per block (1/3-rule panels; an odd number of segments ends with one 3/8-rule panel
+ * over its last three; blocks of fewer than two segments keep the trapezoid); no counterpart
+ * in the reference.  The weights of both rules sum to the block length, so the canonical
+ * mass is the same. */
+int adscf_set_contour_rule(adscf_ctx* ctx, int rule);
+
 /* ---- conjugate fields and DDFT (Main_Multiblock/main_multiblock.f90) ------------ */
 /* Get_Conjugate_Fields (main_multiblock.f90:566-640; main_blend.f90:376-428): iterate the
  * current field (adscf_scf_set_field, or the field a previous call left) with the Anderson

@@ -88,6 +88,8 @@ def _load():
         "orc_propagate_step": (None, [G, _dp, _dp]),
         "orc_propagate_chain": (C.c_int, [G, C.POINTER(_Chain), _dp, _dp, _dp, _dp, _dp]),
         "orc_block_segments": (C.c_int, [C.c_double, C.c_double]),
+        "orc_set_contour_rule": (None, [C.c_int]),
+        "orc_contour_weight": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_double]),
         "orc_propagate_homopolymer": (C.c_int, [G, C.POINTER(_Chain), _dp, _dp, _dp]),
         "orc_update_solvent": (C.c_int, [G, C.POINTER(_Chain), _dp, _dp, _dp]),
         "orc_mean_fields": (None, [G, C.POINTER(_FH), _dp, _dp]),
@@ -240,6 +242,19 @@ def propagate_step(g: Grid, prop, expfield_type):
 
 def block_segments(block_length, ds):
     return lib().orc_block_segments(float(block_length), float(ds))
+
+
+RULES = {"trapezoid": 0, "simpson": 1}
+
+
+def set_contour_rule(rule):
+    """Contour rule of propagate_chain / propagate_homopolymer and everything built on them
+    (process-wide): "trapezoid" (the reference's, default) or "simpson"."""
+    lib().orc_set_contour_rule(RULES.get(rule, rule))
+
+
+def contour_weight(rule, j, nb, ds):
+    return lib().orc_contour_weight(RULES.get(rule, rule), int(j), int(nb), float(ds))
 
 
 class ChainResult:

@@ -273,6 +273,36 @@ void orc_propagate_step(const orc_grid* g, double* prop, const double* expfield_
   free(pf);
 }
 
+/* ===================================================================== */
+/* Contour rules                                                          */
+/* ===================================================================== */
+
+static int g_contour_rule = 0;
+void orc_set_contour_rule(int rule) { g_contour_rule = rule; }
+
+/* Rule 0 is the reference's trapezoid (mod_multiblock.f90:157-168): half weights at the two
+ * ends of a block; a block of zero segments adds both halves at its one point.  Rule 1
+ * restates the textbook composite Simpson rule panel by panel: panels [2p, 2p+2] with
+ * weights ds/3 (1, 4, 1), and for an odd nb the last three intervals as one Simpson 3/8
+ * panel, 3ds/8 (1, 3, 3, 1); a block of one segment keeps the trapezoid. */
+double orc_contour_weight(int rule, int j, int nb, double ds) {
+  if (nb == 0) return ds;
+  if (rule == 0 || nb == 1) return (j == 0 || j == nb) ? 0.5 * ds : ds;
+  const int npanel = (nb % 2 == 0) ? nb / 2 : (nb - 3) / 2;
+  double w = 0.0;
+  for (int p = 0; p < npanel; ++p) {
+    const int a = 2 * p;
+    if (j == a || j == a + 2) w += ds / 3.0;
+    if (j == a + 1) w += 4.0 * ds / 3.0;
+  }
+  if (nb % 2 == 1) {
+    const int a = 2 * npanel;
+    if (j == a || j == a + 3) w += 3.0 * ds / 8.0;
+    if (j == a + 1 || j == a + 2) w += 9.0 * ds / 8.0;
+  }
+  return w;
+}
+
 /* nint(block_length/ds): Fortran NINT rounds half away from zero (mod_multiblock.f90:85) */
 /* mod_homopolymer.f90:90-141 */
 int orc_propagate_homopolymer(const orc_grid* g, orc_chain* c, const double* expfield, double* distribution,
@@ -302,11 +332,20 @@ int orc_propagate_homopolymer(const orc_grid* g, orc_chain* c, const double* exp
   for (long i = 0; i < ng; ++i) sum += qq[(long)mono * ng + i];
   c->Qsum = dvol * sum;
   /* :119-125 */
-  for (long i = 0; i < ng; ++i) distribution[i] = 0.5 * ds * qq[i] * qq[(long)mono * ng + i];
-  for (int s = 1; s <= mono - 1; ++s)
-    for (long i = 0; i < ng; ++i)
-      distribution[i] = distribution[i] + ds * qq[(long)s * ng + i] * qq[(long)(mono - s) * ng + i];
-  for (long i = 0; i < ng; ++i) distribution[i] = distribution[i] + 0.5 * ds * qq[(long)mono * ng + i] * qq[i];
+  if (g_contour_rule == 0) {
+    for (long i = 0; i < ng; ++i) distribution[i] = 0.5 * ds * qq[i] * qq[(long)mono * ng + i];
+    for (int s = 1; s <= mono - 1; ++s)
+      for (long i = 0; i < ng; ++i)
+        distribution[i] = distribution[i] + ds * qq[(long)s * ng + i] * qq[(long)(mono - s) * ng + i];
+    for (long i = 0; i < ng; ++i) distribution[i] = distribution[i] + 0.5 * ds * qq[(long)mono * ng + i] * qq[i];
+  } else {
+    for (long i = 0; i < ng; ++i) distribution[i] = 0.0;
+    for (int s = 0; s <= mono; ++s) {
+      const double w = orc_contour_weight(g_contour_rule, s, mono, ds);
+      for (long i = 0; i < ng; ++i)
+        distribution[i] = distribution[i] + w * qq[(long)s * ng + i] * qq[(long)(mono - s) * ng + i];
+    }
+  }
   free(prop);
   free(qq);
   /* :127-138 */
@@ -409,7 +448,20 @@ int orc_propagate_chain(const orc_grid* g, orc_chain* c, const double* expfield,
   c->Qsum = dvol * sum;
   /* trapezoid contour integral per block (:157-168) */
   s0 = 0;
-  for (int b = 0; b < nb; ++b) {
+  for (int b = 0; b < nb && g_contour_rule != 0; ++b) { /* or another rule's weights */
+    double* dist = distribution + (long)b * ng;
+    const int n = c->block_segments[b];
+    for (long i = 0; i < ng; ++i) dist[i] = 0.0;
+    for (int j = 0; j <= n; ++j) {
+      const double w = orc_contour_weight(g_contour_rule, j, n, ds);
+      const double* a = qq + (long)(s0 + j) * ng;
+      const double* d = qd + (long)(mono - s0 - j) * ng;
+      for (long i = 0; i < ng; ++i) dist[i] = dist[i] + w * a[i] * d[i];
+    }
+    s0 += n;
+  }
+  s0 = 0;
+  for (int b = 0; b < nb && g_contour_rule == 0; ++b) {
     double* dist = distribution + (long)b * ng;
     const double* a0 = qq + (long)s0 * ng;
     const double* d0 = qd + (long)(mono - s0) * ng;

@@ -69,6 +69,12 @@ int orc_propagate_chain(const orc_grid* g, orc_chain* c, const double* expfield,
                         double* distribution, double* density, double* q_out, double* qdag_out);
 /* block_segments = nint(block_length/ds) (mod_multiblock.f90:85, mod_diblock.f90:74) */
 int orc_block_segments(double block_length, double ds);
+/* Contour rule of orc_propagate_chain and orc_propagate_homopolymer (process-wide; tests
+ * set it back to 0).  0: the reference's trapezoid (mod_multiblock.f90:157-168), the
+ * default.  1: composite Simpson, an extra mode with no reference counterpart. */
+void orc_set_contour_rule(int rule);
+/* weight of point j (0..nb) of a block of nb segments under `rule` */
+double orc_contour_weight(int rule, int j, int nb, double ds);
 
 /* ---- Homopolymer and solvent (Polymer_Modules/mod_homopolymer.f90, mod_solvent.f90) */
 /* Propagate_Homopolymer (mod_homopolymer.f90:90-141): one propagator qq(s), s = 0..segments,
