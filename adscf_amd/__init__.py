@@ -97,6 +97,8 @@ SIGNATURES = {
     "adscf_blend_setup": (C.c_int, [_ctxp, C.c_int, _ip, _ip, _dp, C.c_int, C.c_double]),
     "adscf_blend_scalars": (C.c_int, [_ctxp, _dp, _dp, _dp]),
     "adscf_epd_step": (C.c_int, [_ctxp, C.c_double, C.c_double, C.c_int, _ip]),
+    "adscf_field_derivative": (C.c_int, [_ctxp, _dp]),
+    "adscf_epd_field_step": (C.c_int, [_ctxp, C.c_double, C.c_int]),
     "adscf_autotune": (C.c_int, [_ctxp, C.c_int, C.c_char_p, C.c_int]),
     "adscf_get_schedule": (C.c_int, [_ctxp, C.c_int, _ip, _ip, _ip, _ip]),
     "adscf_set_schedule": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -520,6 +522,19 @@ class Context:
         self._chk(self._lib.adscf_epd_step(self._h, float(timestep), float(accuracy_scf), int(itermax),
                                            C.byref(it)))
         return it.value
+
+    def field_derivative(self):
+        """Calculate_Field_Derivative (main_blend.f90:709-740) at the device-resident field:
+        -laplace * (mean fields of the densities - field), (ntypes, nz, ny, nx)."""
+        out = np.empty((self.ntypes,) + self.shape)
+        self._chk(self._lib.adscf_field_derivative(self._h, _p(out)))
+        return out
+
+    def epd_field_step(self, timestep, method="euler"):
+        """EPD_Timestep_Euler (main_blend.f90:580-596) or, method "rk4",
+        EPD_Timestep_RungeKutta (:678-701) on the device-resident field."""
+        code = {"euler": 0, "rk4": 1}.get(method, method)
+        self._chk(self._lib.adscf_epd_field_step(self._h, float(timestep), int(code)))
 
     def ddft_energy(self):
         F, mu, rho = C.c_double(), C.c_double(), C.c_double()

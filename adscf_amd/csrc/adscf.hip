@@ -196,6 +196,7 @@ struct adscf_ctx {
   std::vector<double> bl_rho;
   DevBuf<ScfState> pst;              // chain scalars (Q, norm, mu, energy) of each polymer
   DevBuf<double> nab, dold, dfa;     // EPD: nabla (imaginary parts), old densities, gradients
+  DevBuf<double> lapr, epdw;         // EPD field steps: laplace's real parts; field, stage, 4 derivatives
 };
 
 namespace {
@@ -1114,7 +1115,7 @@ void enqueue_evaluate(adscf_ctx* c) {
 }
 
 template <typename T>
-void scf_set_field_impl(adscf_ctx* c, const double* field) {
+void scf_set_field_impl(adscf_ctx* c, const double* field, hipMemcpyKind kind = hipMemcpyHostToDevice) {
   ensure_tables<T>(c);
   ensure_hist<T>(c, hist_ns(c));
   ensure_partials(c);
@@ -1130,7 +1131,7 @@ void scf_set_field_impl(adscf_ctx* c, const double* field) {
   c->resid.ensure(200000);
   HIPCK(hipMemsetAsync(c->wring.p, 0, sizeof(double) * (m + 1) * nvar, c->stream));
   HIPCK(hipMemsetAsync(c->dring.p, 0, sizeof(double) * (m + 1) * nvar, c->stream));
-  HIPCK(hipMemcpyAsync(c->wring.p, field, nvar * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCK(hipMemcpyAsync(c->wring.p, field, nvar * sizeof(double), kind, c->stream));
   ScfState init{};
   init.head = 0;
   HIPCK(hipMemcpyAsync(c->st.p, &init, sizeof init, hipMemcpyHostToDevice, c->stream));
@@ -1410,6 +1411,110 @@ int epd_step_impl(adscf_ctx* c, double dt, double thresh, int itermax, int* iter
   c->phi_fresh = false;
   double acc = 0.0;
   return conjugate_impl<T>(c, thresh, itermax, 1, iters, &acc);  // :663
+}
+
+// laplace's real parts (-q^2 per half-spectrum point) on the device, from the host table
+// of Calculate_Laplace
+void ensure_lap(adscf_ctx* c) {
+  const long nk = (long)(c->nx / 2 + 1) * c->ny * c->nz;
+  if (c->lapr.n >= (size_t)nk) return;
+  std::vector<double> lap(2 * nk), re(nk);
+  const int rc = adscf_laplace(c, lap.data(), nullptr, nullptr);
+  if (rc != ADSCF_OK) throw ArgError{"laplace table"};
+  for (long i = 0; i < nk; ++i) re[i] = lap[2 * i];
+  c->lapr.ensure(nk);
+  HIPCK(hipMemcpy(c->lapr.p, re.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+}
+
+// Calculate_Field_Derivative (Main_Blend/main_blend.f90:709-740) of the device field
+// `field` (nt, ng) into `dfield`: E and the rescale schedule of the field, the densities
+// (Calculate_Densities, :727), mean fields minus field (:728-729), then per type the
+// product with -laplace in Fourier space (:731-737).
+template <typename T>
+void enqueue_field_derivative(adscf_ctx* c, const double* field, double* dfield) {
+  const long ng = c->ng;
+  const long nk = (long)(c->nx / 2 + 1) * c->ny * c->nz;
+  const int nb = grid_for(ng);
+  Engine<T>& En = eng<T>(c);
+  hipLaunchKernelGGL(k_expfield<T>, dim3(nb), dim3(kThreads), 0, c->stream, field, En.E.p, ng, c->nt, c->ds,
+                     c->e2part.p, (const int*)nullptr);
+  if (c->bl_cd.empty())  // a blend's densities compute each polymer's schedule themselves
+    hipLaunchKernelGGL(k_schedule, dim3(1), dim3(kThreads), 0, c->stream, c->cd, (const double*)c->e2part.p, nb, ng,
+                       c->nt, c->e1.p, c->e2.p, c->d1.p, c->d2.p, (ScfState*)nullptr, 0, c->m, (const int*)nullptr);
+  enqueue_evaluate<T>(c);
+  FHDesc fh = c->fh;
+  fh.nt = c->nt;
+  hipLaunchKernelGGL(k_ddft_mulocal, dim3(nb), dim3(kThreads), 0, c->stream, (const double*)c->phi.p, field,
+                     (const ScfState*)nullptr, fh, ng, dfield);
+  for (int t = 0; t < c->nt; ++t) {
+    enqueue_r2c<T>(c, dfield + t * ng, c->dmuq.p);
+    hipLaunchKernelGGL(k_neg_laplace_mul, dim3(grid_for(nk)), dim3(kThreads), 0, c->stream, (const double*)c->dmuq.p,
+                       (const double*)c->lapr.p, nk, c->ddq.p);
+    enqueue_c2r<T>(c, c->ddq.p, dfield + t * ng);
+  }
+  HIPCK(hipGetLastError());
+}
+
+// the device-resident field (ring slot head) copied into `out`
+void copy_ring_field(adscf_ctx* c, double* out) {
+  ScfState st{};
+  HIPCK(hipMemcpyAsync(&st, c->st.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  const long nvar = c->ng * c->nt;
+  HIPCK(hipMemcpyAsync(out, c->wring.p + (size_t)st.head * nvar, nvar * sizeof(double), hipMemcpyDeviceToDevice,
+                       c->stream));
+}
+
+void ensure_epd_work(adscf_ctx* c) {
+  const long nvar = c->ng * c->nt;
+  const long nk = (long)(c->nx / 2 + 1) * c->ny * c->nz;
+  ensure_lap(c);
+  c->dmuq.ensure(2 * nk);
+  c->ddq.ensure(2 * nk);
+  c->epdw.ensure(6 * nvar);
+}
+
+// Calculate_Field_Derivative at the device-resident field, copied to the host
+template <typename T>
+void field_derivative_impl(adscf_ctx* c, double* dfield) {
+  ensure_epd_work(c);
+  const long nvar = c->ng * c->nt;
+  double* f0 = c->epdw.p;
+  double* d = c->epdw.p + 2 * nvar;
+  copy_ring_field(c, f0);
+  enqueue_field_derivative<T>(c, f0, d);
+  HIPCK(hipMemcpyAsync(dfield, d, nvar * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  refresh_E<T>(c);  // E and the rescale schedule of the device-resident field again
+  c->phi_fresh = false;
+}
+
+// EPD_Timestep_Euler (main_blend.f90:580-596, method 0) or EPD_Timestep_RungeKutta
+// (:678-701, method 1) on the device-resident field; the new field then starts the SCF
+// state afresh, as adscf_scf_set_field does.
+template <typename T>
+void epd_field_step_impl(adscf_ctx* c, double dt, int method) {
+  ensure_epd_work(c);
+  const long nvar = c->ng * c->nt;
+  const int nbv = grid_for(nvar);
+  double* f0 = c->epdw.p;
+  double* stage = c->epdw.p + nvar;
+  double* df = c->epdw.p + 2 * nvar;  // df1 .. df4
+  copy_ring_field(c, f0);
+  enqueue_field_derivative<T>(c, f0, df);  // :590 / :690
+  if (method == 0) {
+    hipLaunchKernelGGL(k_axpy, dim3(nbv), dim3(kThreads), 0, c->stream, (const double*)f0, dt, (const double*)df, nvar,
+                       f0);  // :592
+  } else {
+    const double h[3] = {0.5 * dt, 0.5 * dt, dt};  // :691, :693, :695
+    for (int k = 0; k < 3; ++k) {
+      hipLaunchKernelGGL(k_axpy, dim3(nbv), dim3(kThreads), 0, c->stream, (const double*)f0, h[k],
+                         (const double*)(df + k * nvar), nvar, stage);
+      enqueue_field_derivative<T>(c, stage, df + (k + 1) * nvar);
+    }
+    hipLaunchKernelGGL(k_rk4_update, dim3(nbv), dim3(kThreads), 0, c->stream, f0, (const double*)df, nvar, dt);  // :698
+  }
+  HIPCK(hipGetLastError());
+  scf_set_field_impl<T>(c, f0, hipMemcpyDeviceToDevice);
 }
 
 // the schedule and buffers a captured sweep bakes in
@@ -3263,6 +3368,34 @@ int adscf_epd_step(adscf_ctx* c, double timestep, double accuracy_scf, int iterm
     if (itermax < 1) throw ArgError{"itermax must be >= 1"};
     return c->prec == ADSCF_FP64 ? epd_step_impl<double>(c, timestep, accuracy_scf, itermax, iterations)
                                  : epd_step_impl<float>(c, timestep, accuracy_scf, itermax, iterations);
+  });
+}
+
+int adscf_field_derivative(adscf_ctx* c, double* dfield) {
+  return guarded(c, [&]() -> int {
+    if (!dfield) throw ArgError{"null array"};
+    if (!c->field_set || !c->fh_set) {
+      c->err = "adscf_field_derivative needs adscf_set_interactions and adscf_scf_set_field";
+      return ADSCF_ERR_STATE;
+    }
+    if (c->prec == ADSCF_FP64) field_derivative_impl<double>(c, dfield);
+    else field_derivative_impl<float>(c, dfield);
+    return ADSCF_OK;
+  });
+}
+
+int adscf_epd_field_step(adscf_ctx* c, double timestep, int method) {
+  return guarded(c, [&]() -> int {
+    if (method != 0 && method != 1) throw ArgError{"method must be 0 (Euler) or 1 (Runge-Kutta)"};
+    if (!c->field_set || !c->fh_set) {
+      c->err = "adscf_epd_field_step needs adscf_set_interactions and adscf_scf_set_field";
+      return ADSCF_ERR_STATE;
+    }
+    if (c->prec == ADSCF_FP64) epd_field_step_impl<double>(c, timestep, method);
+    else epd_field_step_impl<float>(c, timestep, method);
+    c->phi_fresh = false;
+    c->last_loop = -1;
+    return ADSCF_OK;
   });
 }
 

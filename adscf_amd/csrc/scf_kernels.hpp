@@ -1060,11 +1060,12 @@ __global__ void k_scf_restart(ScfState* st) {
 // ---------------------------------------------------------------------------
 
 // mulocal = mean fields of the density - field (:824-825); the field is ring slot `head`
+// of wring, or wring itself when st is null
 __global__ void __launch_bounds__(kThreads) k_ddft_mulocal(const double* __restrict__ den,
                                                            const double* __restrict__ wring, const ScfState* st,
                                                            FHDesc fh, long ng, double* __restrict__ mu) {
   const int nt = fh.nt;
-  const double* w = wring + (long)st->head * nt * ng;
+  const double* w = st ? wring + (long)st->head * nt * ng : wring;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ng; i += (long)gridDim.x * blockDim.x) {
     double vf[kMaxTypes], tot = 0.0;
     for (int t = 0; t < nt; ++t) vf[t] = den[(long)t * ng + i] * fh.vol[t];
@@ -1149,6 +1150,41 @@ __global__ void __launch_bounds__(kThreads) k_epd_update(double* __restrict__ de
     sum = sum + dd[n + i];
     sum = sum + dd[2 * n + i];
     den[i] = den[i] + sum * dt / eps;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// EPD_Timestep_Euler / EPD_Timestep_RungeKutta (Main_Blend/main_blend.f90:580-740)
+// ---------------------------------------------------------------------------
+
+// dummyfourier = - laplace*dummyfourier (:735): Fortran's complex product with
+// laplace = (lr, 0), then the negation of both parts
+__global__ void __launch_bounds__(kThreads) k_neg_laplace_mul(const double* __restrict__ in,
+                                                              const double* __restrict__ lr, long nk,
+                                                              double* __restrict__ out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nk; i += (long)gridDim.x * blockDim.x) {
+    const double a = lr[i], x = in[2 * i], y = in[2 * i + 1];
+    out[2 * i] = -(a * x - 0.0 * y);
+    out[2 * i + 1] = -(a * y + 0.0 * x);
+  }
+}
+
+// out = x + a*d (the stage fields of :691, :693, :695 and the Euler update of :592)
+__global__ void __launch_bounds__(kThreads) k_axpy(const double* __restrict__ x, double a,
+                                                   const double* __restrict__ d, long n, double* __restrict__ out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = x[i] + a * d[i];
+}
+
+// field = field + timestep*(df1/6 + df2/3 + df3/3 + df4/6) (:698), summed left to right
+__global__ void __launch_bounds__(kThreads) k_rk4_update(double* __restrict__ f, const double* __restrict__ df,
+                                                         long n, double dt) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double s = df[i] / 6.0;
+    s = s + df[n + i] / 3.0;
+    s = s + df[2 * n + i] / 3.0;
+    s = s + df[3 * n + i] / 6.0;
+    f[i] = f[i] + dt * s;
   }
 }
 

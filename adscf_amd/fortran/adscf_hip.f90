@@ -209,6 +209,18 @@ module adscf_hip
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: field(*), density(*)
     end function
+    ! EPD_Timestep_Euler (method 0) / EPD_Timestep_RungeKutta (method 1) on the device field
+    integer(c_int) function adscf_epd_field_step(ctx, timestep, method) bind(C, name='adscf_epd_field_step')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: timestep
+      integer(c_int), value :: method
+    end function
+    integer(c_int) function adscf_field_derivative(ctx, dfield) bind(C, name='adscf_field_derivative')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: dfield(*)
+    end function
     integer(c_int) function adscf_conjugate_fields(ctx, density, thresh, maxit, measure, iterations, accuracy) &
         bind(C, name='adscf_conjugate_fields')
       import :: c_ptr, c_int, c_double

@@ -268,6 +268,15 @@ int adscf_blend_scalars(adscf_ctx* ctx, double* qsum, double* mu, double* energy
  * accuracy sum|dfield|*dvol and exit threshold accuracy_scf (the driver: 0.01 single
  * precision, itermax 100000).  iterations: the solve's loop index. */
 int adscf_epd_step(adscf_ctx* ctx, double timestep, double accuracy_scf, int itermax, int* iterations);
+/* Calculate_Field_Derivative (main_blend.f90:709-740) at the device-resident field (after
+ * adscf_scf_set_field, on a blend or a single chain): dfield (nx,ny,nz,ntypes) =
+ * -laplace * (mean fields of the densities - field) in Fourier space. */
+int adscf_field_derivative(adscf_ctx* ctx, double* dfield);
+/* EPD_Timestep_Euler (main_blend.f90:580-596; method 0: field += timestep*dfield) or
+ * EPD_Timestep_RungeKutta (:678-701; method 1: the classical four-stage rule) on the
+ * device-resident field, which stays in HBM; the new field then starts the SCF state
+ * afresh, as adscf_scf_set_field does (read it with adscf_scf_get). */
+int adscf_epd_field_step(adscf_ctx* ctx, double timestep, int method);
 
 /* ---- schedule autotuner (no counterpart in the reference) --------------------- */
 /* Times the captured propagator sweep of the configured SCF problem for every line-FFT

@@ -115,6 +115,9 @@ def _load():
         "orc_blend_densities": (C.c_int, [G, C.POINTER(_Blend), C.c_int, _dp, _dp]),
         "orc_blend_conjugate_fields": (C.c_int, [G, C.POINTER(_Blend), C.c_int, _dp, _dp, C.c_int, C.c_double,
                                                  C.c_double, C.c_int, _dp]),
+        "orc_field_derivative": (C.c_int, [G, C.POINTER(_Blend), C.POINTER(_FH), C.c_int, _dp, _dp]),
+        "orc_epd_timestep_euler": (C.c_int, [G, C.POINTER(_Blend), C.POINTER(_FH), C.c_int, _dp, C.c_double]),
+        "orc_epd_timestep_rk4": (C.c_int, [G, C.POINTER(_Blend), C.POINTER(_FH), C.c_int, _dp, C.c_double]),
         "orc_epd_step": (C.c_int, [G, C.POINTER(_Blend), C.POINTER(_FH), C.c_int, _dp, _dp, C.c_double, C.c_int,
                                    C.c_double, C.c_int, _ip]),
         "orc_blend_scf": (C.c_int, [G, C.POINTER(_Blend), C.POINTER(_FH), C.c_int, _dp, _dp, C.c_int, C.c_int,
@@ -534,6 +537,25 @@ def epd_step(g: Grid, blend: Blend, fh: FH, density, field, timestep, m=2, accur
     if rc != 0:
         raise ValueError("Invalid ensemble")
     return it.value
+
+
+def field_derivative(g: Grid, blend: Blend, fh: FH, field):
+    """Calculate_Field_Derivative (main_blend.f90:709-740): -laplace * (newfield - field)."""
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    out = np.empty_like(f)
+    if lib().orc_field_derivative(C.byref(g._g), C.byref(blend._b), C.byref(fh._fh), f.shape[0], _p(f), _p(out)):
+        raise ValueError("Invalid ensemble")
+    return out
+
+
+def epd_timestep(g: Grid, blend: Blend, fh: FH, field, timestep, method="euler"):
+    """EPD_Timestep_Euler (main_blend.f90:580-596) or EPD_Timestep_RungeKutta (:678-701);
+    returns the new field."""
+    f = np.array(field, dtype=np.float64, order="C")
+    fn = lib().orc_epd_timestep_euler if method == "euler" else lib().orc_epd_timestep_rk4
+    if fn(C.byref(g._g), C.byref(blend._b), C.byref(fh._fh), f.shape[0], _p(f), float(timestep)):
+        raise ValueError("Invalid ensemble")
+    return f
 
 
 BLEND_SCF_ACCURACY = float(np.float32(0.0000001))  # main_blend.f90:451

@@ -1149,6 +1149,86 @@ int orc_epd_step(const orc_grid* g, orc_blend* b, const orc_fh* fh, int ntypes, 
   return rc;
 }
 
+/* Calculate_Field_Derivative (main_blend.f90:709-740): densities and mean fields of the
+ * field, dfield = newfield - field, then per type the spectral product
+ * dummyfourier = - laplace*dummyfourier (:735; Fortran's unary minus applies to the
+ * product), back to real space. */
+int orc_field_derivative(const orc_grid* g, orc_blend* b, const orc_fh* fh, int ntypes, const double* field,
+                         double* dfield) {
+  long ng = (long)g->nx * g->ny * g->nz;
+  long nvar = ng * ntypes;
+  long nk = (long)(g->nx / 2 + 1) * g->ny * g->nz;
+  double* den = (double*)malloc(sizeof(double) * (size_t)nvar);
+  double* newfield = (double*)malloc(sizeof(double) * (size_t)nvar);
+  double* lap = (double*)malloc(sizeof(double) * 2 * (size_t)nk);
+  double* spec = (double*)malloc(sizeof(double) * 2 * (size_t)nk);
+  int rc = orc_blend_densities(g, b, ntypes, field, den); /* :727 */
+  if (rc == 0) {
+    orc_mean_fields(g, fh, den, newfield);                                  /* :728 */
+    for (long i = 0; i < nvar; ++i) dfield[i] = newfield[i] - field[i];    /* :729 */
+    orc_calculate_laplace(g, lap, NULL, NULL);
+    for (int t = 0; t < ntypes; ++t) {                                      /* :731-737 */
+      orc_real2fourier(g, dfield + (long)t * ng, spec);
+      for (long k = 0; k < nk; ++k) {
+        const cplx p = (lap[2 * k] + lap[2 * k + 1] * I) * (spec[2 * k] + spec[2 * k + 1] * I);
+        spec[2 * k] = -creal(p);
+        spec[2 * k + 1] = -cimag(p);
+      }
+      orc_fourier2real(g, spec, dfield + (long)t * ng);
+    }
+  }
+  free(spec);
+  free(lap);
+  free(newfield);
+  free(den);
+  return rc == 0 ? 0 : -1;
+}
+
+/* EPD_Timestep_Euler (main_blend.f90:580-596): field = field + timestep*dfield */
+int orc_epd_timestep_euler(const orc_grid* g, orc_blend* b, const orc_fh* fh, int ntypes, double* field,
+                           double timestep) {
+  long nvar = (long)g->nx * g->ny * g->nz * ntypes;
+  double* df = (double*)malloc(sizeof(double) * (size_t)nvar);
+  int rc = orc_field_derivative(g, b, fh, ntypes, field, df);
+  if (rc == 0)
+    for (long i = 0; i < nvar; ++i) field[i] = field[i] + timestep * df[i];
+  free(df);
+  return rc;
+}
+
+/* EPD_Timestep_RungeKutta (main_blend.f90:678-701): the classical four-stage rule,
+ * field += timestep*(df1/6 + df2/3 + df3/3 + df4/6) with the stages at field,
+ * field + timestep/2 df1, field + timestep/2 df2 and field + timestep df3 */
+int orc_epd_timestep_rk4(const orc_grid* g, orc_blend* b, const orc_fh* fh, int ntypes, double* field,
+                         double timestep) {
+  long nvar = (long)g->nx * g->ny * g->nz * ntypes;
+  double* df = (double*)malloc(sizeof(double) * (size_t)nvar * 4);
+  double* stage = (double*)malloc(sizeof(double) * (size_t)nvar);
+  double* d1 = df;
+  double* d2 = df + nvar;
+  double* d3 = df + 2 * nvar;
+  double* d4 = df + 3 * nvar;
+  int rc = orc_field_derivative(g, b, fh, ntypes, field, d1);                        /* :690 */
+  if (rc == 0) {
+    for (long i = 0; i < nvar; ++i) stage[i] = field[i] + 0.5 * timestep * d1[i];   /* :691 */
+    rc = orc_field_derivative(g, b, fh, ntypes, stage, d2);                         /* :692 */
+  }
+  if (rc == 0) {
+    for (long i = 0; i < nvar; ++i) stage[i] = field[i] + 0.5 * timestep * d2[i];   /* :693 */
+    rc = orc_field_derivative(g, b, fh, ntypes, stage, d3);                         /* :694 */
+  }
+  if (rc == 0) {
+    for (long i = 0; i < nvar; ++i) stage[i] = field[i] + timestep * d3[i];         /* :695 */
+    rc = orc_field_derivative(g, b, fh, ntypes, stage, d4);                         /* :696 */
+  }
+  if (rc == 0)
+    for (long i = 0; i < nvar; ++i)                                                 /* :698 */
+      field[i] = field[i] + timestep * (d1[i] / 6.0 + d2[i] / 3.0 + d3[i] / 3.0 + d4[i] / 6.0);
+  free(stage);
+  free(df);
+  return rc;
+}
+
 /* ======================================================================
  * Main_Diblock_Melt/main_diblock.f90:398-451
  * ====================================================================== */

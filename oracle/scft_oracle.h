@@ -203,6 +203,17 @@ int orc_blend_scf(const orc_grid* g, orc_blend* b, const orc_fh* fh, int ntypes,
  * solve's loop index (may be NULL).  Returns 0 or -1. */
 int orc_epd_step(const orc_grid* g, orc_blend* b, const orc_fh* fh, int ntypes, double* density, double* field,
                  double timestep, int m, double accuracy_scf, int itermax, int* iterations);
+/* Calculate_Field_Derivative (main_blend.f90:709-740): dfield = the Fourier product
+ * -laplace * (newfield - field) with newfield the mean fields of the blend's densities at
+ * field.  Returns 0 or -1. */
+int orc_field_derivative(const orc_grid* g, orc_blend* b, const orc_fh* fh, int ntypes, const double* field,
+                         double* dfield);
+/* EPD_Timestep_Euler (main_blend.f90:580-596) and EPD_Timestep_RungeKutta (:678-701) on
+ * field (inout).  Return 0 or -1. */
+int orc_epd_timestep_euler(const orc_grid* g, orc_blend* b, const orc_fh* fh, int ntypes, double* field,
+                           double timestep);
+int orc_epd_timestep_rk4(const orc_grid* g, orc_blend* b, const orc_fh* fh, int ntypes, double* field,
+                         double timestep);
 
 #ifdef __cplusplus
 }
