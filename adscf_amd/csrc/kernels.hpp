@@ -271,15 +271,40 @@ __global__ void __launch_bounds__(kThreads) k_line_pass(PassArgs<T> a) {
 // ---------------------------------------------------------------------------
 // register-resident line pass for n = N1 * N2 (fft_reg.hpp)
 // ---------------------------------------------------------------------------
+
+// The exchanges inside one line's operator.  A wavefront's LDS operations are performed
+// in issue order, so when every line of the workgroup lies inside one wavefront the
+// phases need only a compiler fence at wavefront scope: no workgroup barrier and no wait
+// for the LDS stores to complete before the loads that read them are issued.  Otherwise
+// (lines spread over the waves of a multi-wave workgroup) a workgroup barrier.
+__device__ __forceinline__ void line_sync(bool wave_local) {
+  if (wave_local) {  // uniform over the block
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// Whether every line of the block lies inside one wavefront: a one-wave block, or an x
+// pass (thread t of line l at lane l TPL + t) whose threads per line divide 64.  The y and
+// z passes put thread t of line l at l + t L, which spreads a line over the block.
+template <int TPL>
+__device__ __forceinline__ bool lines_in_wave(int axis) {
+  return blockDim.x <= 64 || (axis == 0 && 64 % TPL == 0);
+}
+
 template <typename T, int N1, int N2>
-__device__ __forceinline__ void reg_line_op(C2<T> (&v)[N2], C2<T>* lds, int t, const RegTables<T, N1, N2>& R) {
+__device__ __forceinline__ void reg_line_op(C2<T> (&v)[N2], C2<T>* lds, int t, const RegTables<T, N1, N2>& R,
+                                            bool wl) {
   if constexpr (N1 == 1) {
     line_op_single<T, N2>(v, R);
   } else {
     if (t < N1) fs_phase_a<T, N1, N2>(v, lds, t, R);
-    __syncthreads();
+    line_sync(wl);
     if (t < N2) fs_phase_b<T, N1, N2>(lds, t, R);
-    __syncthreads();
+    line_sync(wl);
     if (t < N1) fs_phase_c<T, N1, N2>(v, lds, t);
   }
 }
@@ -296,8 +321,8 @@ struct EngFour {  // n = N1 * N2 four-step (fft_reg.hpp)
   static constexpr int NP = N2, TPL = reg_tpl<N1, N2>(), NACT = N1, STRIDE = N1;
   static constexpr int LDS = N1 == 1 ? 0 : FourStep<N1, N2>::lds_elems;
   using Tables = RegTables<T, N1, N2>;
-  static __device__ __forceinline__ void op(C2<T> (&v)[NP], C2<T>* lds, int t, const Tables& R) {
-    reg_line_op<T, N1, N2>(v, lds, t, R);
+  static __device__ __forceinline__ void op(C2<T> (&v)[NP], C2<T>* lds, int t, const Tables& R, bool wl) {
+    reg_line_op<T, N1, N2>(v, lds, t, R, wl);
   }
 };
 
@@ -306,15 +331,15 @@ struct EngCube {  // n = R^3 on R^2 threads (fft_cube.hpp)
   static constexpr int NP = R, TPL = R * R, NACT = R * R, STRIDE = R * R;
   static constexpr int LDS = Cube<R>::lds_elems;
   using Tables = CubeTables<T, R>;
-  static __device__ __forceinline__ void op(C2<T> (&v)[NP], C2<T>* lds, int t, const Tables& W) {
+  static __device__ __forceinline__ void op(C2<T> (&v)[NP], C2<T>* lds, int t, const Tables& W, bool wl) {
     cube_a<T, R>(v, lds, t, W);
-    __syncthreads();
+    line_sync(wl);
     cube_b<T, R>(lds, t, W);
-    __syncthreads();
+    line_sync(wl);
     cube_c<T, R>(lds, t, W);
-    __syncthreads();
+    line_sync(wl);
     cube_bi<T, R>(lds, t, W);
-    __syncthreads();
+    line_sync(wl);
     cube_ai<T, R>(v, lds, t, W);
   }
 };
@@ -324,15 +349,15 @@ struct EngTri {  // n = A R^2 on R^2 threads (fft_cube.hpp, tri_*)
   static constexpr int NP = A, TPL = R * R, NACT = R * R, STRIDE = R * R;
   static constexpr int LDS = Tri<A, R>::lds_elems;
   using Tables = TriTables<T, A, R>;
-  static __device__ __forceinline__ void op(C2<T> (&v)[NP], C2<T>* lds, int t, const Tables& W) {
+  static __device__ __forceinline__ void op(C2<T> (&v)[NP], C2<T>* lds, int t, const Tables& W, bool wl) {
     tri_a<T, A, R>(v, lds, t, W);
-    __syncthreads();
+    line_sync(wl);
     tri_b<T, A, R>(lds, t, W);
-    __syncthreads();
+    line_sync(wl);
     tri_c<T, A, R>(lds, t, W);
-    __syncthreads();
+    line_sync(wl);
     tri_bi<T, A, R>(lds, t, W);
-    __syncthreads();
+    line_sync(wl);
     tri_ai<T, A, R>(v, lds, t, W);
   }
 };
@@ -453,6 +478,7 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
   auto fo = [&](int j2) { return f0 + (unsigned)j2 * fstep; };
   auto fc = [&](int j2) { return 2u * (f0 + (unsigned)j2 * fstep); };
   C2<T>* lds = reinterpret_cast<C2<T>*>(smem_raw) + l * ENG::LDS;
+  const bool wl = lines_in_wave<TPL>(a.axis);
   const bool act = t < ENG::NACT && valid;
   const int stop = *(a.done ? a.done : &g_no_stop);
   const bool useEa = pre_E != 0 || nops == 2 || out_kind == OUT_DOT || (out_kind == OUT_PLANAR && post_E);
@@ -510,9 +536,9 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
       v[j2].y /= ea2[j2];
     }
   }
-  ENG::op(v, lds, t, R);
-  // The stop flag is tested only here, after the first operator: its barriers keep
-  // every load above in flight together, and nothing has been stored yet.
+  ENG::op(v, lds, t, R, wl);
+  // The stop flag is tested only here, after the first operator, so its load flies with
+  // every load above, and nothing has been stored yet.
   if (stop) return;  // uniform over the block
   if (LEAN && act) {
 #pragma unroll
@@ -541,7 +567,7 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
         v[j2] = x;
       }
     }
-    ENG::op(v, lds, t, R);
+    ENG::op(v, lds, t, R, wl);
   }
   if (out_kind == OUT_DOT) {
     double acc = 0.0;
@@ -605,6 +631,7 @@ __device__ __forceinline__ void reg_pass_tiles(const PassArgs<T>& a, int nitems,
     t = tid / a.L;
   }
   C2<T>* lds = reinterpret_cast<C2<T>*>(smem_raw) + l * ENG::LDS;
+  const bool wl = lines_in_wave<TPL>(a.axis);
   const int stop = *(a.done ? a.done : &g_no_stop);
   const unsigned fstep = (unsigned)(STRIDE * a.stride) * (unsigned)sizeof(T);
   struct Tile {
@@ -659,7 +686,7 @@ __device__ __forceinline__ void reg_pass_tiles(const PassArgs<T>& a, int nitems,
           cur.v[j2].y *= cur.ea2[j2];
         }
       }
-      ENG::op(cur.v, lds, t, R);
+      ENG::op(cur.v, lds, t, R, wl);
       if (stop) return;  // uniform over the block
       if (MERGED) {
         if (cur.act) {
@@ -676,7 +703,7 @@ __device__ __forceinline__ void reg_pass_tiles(const PassArgs<T>& a, int nitems,
             cur.v[j2] = x;
           }
         }
-        ENG::op(cur.v, lds, t, R);
+        ENG::op(cur.v, lds, t, R, wl);
       }
       if (cur.act) {
 #pragma unroll
