@@ -28,7 +28,7 @@ def loss_and_grads(scft, chain, w, box, phi_target, lam=0.1, ckpt=0):
     bt = torch.tensor(box, dtype=torch.float64, requires_grad=True)
     L = ScftLoss.apply(wt, bt, scft, chain, phi_target, lam, ckpt)
     L.backward()
-    return float(L), wt.grad.numpy().copy(), bt.grad.numpy().copy()
+    return float(L.detach()), wt.grad.numpy().copy(), bt.grad.numpy().copy()
 
 
 def loss_only(scft, chain, w, box, phi_target, lam=0.1, ckpt=0):
