@@ -104,6 +104,8 @@ SIGNATURES = {
     "adscf_set_schedule": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "adscf_set_pass_tiles": (C.c_int, [_ctxp, C.c_int, C.c_int]),
     "adscf_get_pass_tiles": (C.c_int, [_ctxp, C.c_int, _ip]),
+    "adscf_set_plane_passes": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int]),
+    "adscf_get_plane_passes": (C.c_int, [_ctxp, _ip, _ip, _ip]),
     "adscf_set_middle_axis": (C.c_int, [_ctxp, C.c_int]),
     "adscf_get_middle_axis": (C.c_int, [_ctxp, _ip]),
     "adscf_batch_create": (C.c_int, [C.POINTER(_ctxp), C.c_int, _ip, _dp, _dp, _ip, _ip, _ip, _dp, _dp, _dp,
@@ -585,6 +587,17 @@ class Context:
         t = np.zeros(1, dtype=np.int32)
         self._chk(self._lib.adscf_get_pass_tiles(self._h, int(axis), _pi(t)))
         return int(t[0])
+
+    def set_plane_passes(self, mode=-1, n1=0, n2=0):
+        """xy-plane passes of the 3-D sweep: -1 where the grid allows, 0 never, 1 required;
+        (n1, n2) the engine shape of the plane operators (n1 = 0 keeps the current one)."""
+        self._chk(self._lib.adscf_set_plane_passes(self._h, int(mode), int(n1), int(n2)))
+
+    def plane_passes(self):
+        """dict(active, N1, N2): whether the sweep of the context's precision uses plane passes."""
+        t = np.zeros(3, dtype=np.int32)
+        self._chk(self._lib.adscf_get_plane_passes(self._h, _pi(t[0:]), _pi(t[1:]), _pi(t[2:])))
+        return dict(active=bool(t[0]), N1=int(t[1]), N2=int(t[2]))
 
     # -- adjoint ----------------------------------------------------------------
     def adjoint(self, block_types, block_segments, field, phi_target, lam=0.1, target_type=1, rho=1.0,

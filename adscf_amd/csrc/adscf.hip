@@ -22,6 +22,7 @@
 
 #include "../../include/adscf.h"
 #include "kernels.hpp"
+#include "plane.hpp"
 #include "lu.hpp"
 #include "scf_kernels.hpp"
 #include "adjoint.hpp"
@@ -182,6 +183,11 @@ struct adscf_ctx {
   int user_kind = 0, user_measure = 0;  // the loop adscf_scf_iterate runs (adscf_scf_set_residual)
   int user_mixing = 3, mixing = 3;       // 1 simple, 2 lambda, 3 Anderson (adscf_scf_set_mixing)
   int mid_axis = -1;  // axis of the single-operator passes of a 3-D sweep (-1: y); the others merge
+  // xy-plane passes (plane.hpp): -1 where the grid allows them, 0 never (line passes only), 1 asked
+  // for (an error where the grid does not allow them); the engine shape of the plane operators
+  int plane = -1;
+  bool plane_shape_set = false;  // plane_N1/N2 chosen by the caller or the tuner (else per precision)
+  int plane_N1 = 0, plane_N2 = 0;
   int done_loop = -1;  // loop kind ((mixing*2 + resid_kind)*4 + measure) of the call that converged
   int last_loop = -1;  // loop kind of the last call since adscf_scf_set_field (-1: none yet)
   int contour_rule = RULE_TRAPEZOID;  // adscf_set_contour_rule
@@ -348,6 +354,20 @@ PassArgs<T> base_pass(adscf_ctx* c, int axis) {
   p.ny = c->ny;
   p.nz = c->nz;
   p.axis = axis;
+  if (axis == kAxisPlane) {  // the x and y operators of a step over one z-plane (plane.hpp)
+    p.plan = c->ax[0].plan;
+    p.L = 1;
+    p.ldn = c->nx + 1;
+    p.gx = 1;
+    p.stride = 1;
+    p.tw = E.tw[0].p;
+    p.kf = E.kf[0].p;
+    p.tw2 = E.tw[1].p;
+    p.kf2 = E.kf[1].p;
+    p.nops = 1;
+    p.post_E = 1;
+    return p;
+  }
   const int n = c->ax[axis].n;
   p.plan = c->ax[axis].plan;
   p.L = c->ax[axis].L;
@@ -404,7 +424,7 @@ int prep_reg(const adscf_ctx* c, PassArgs<T>& p, int N1, int N2, int waves, int*
   const size_t per_line = N2 == 0  ? (size_t)2 * N1 * N1 * (N1 + 1)
                           : N2 < 0 ? (size_t)N1 * N1 * (1 - N2) + (size_t)(-N2) * N1 * (N1 + 1)
                                    : (N1 == 1 ? 0 : (size_t)N2 * (N1 + 1));
-  if (smem) *smem = (size_t)L * per_line * sizeof(C2<T>);
+  if (smem) *smem = (size_t)L * line_lds_stride((int)per_line) * sizeof(C2<T>);
   return items;
 }
 
@@ -479,6 +499,105 @@ RegLauncher<T> reg_launcher(int n, int N1, int N2) {
   return nullptr;
 }
 
+// xy-plane passes (plane.hpp): the engine shapes compiled for the plane operators, n = nx = ny
+struct PlaneShape {
+  int n, N1, N2;
+};
+const PlaneShape kPlaneShapes[] = {{32, 4, 8}, {32, 8, 4}, {48, 8, 6}, {48, 6, 8}, {64, 4, 0}, {64, 8, 8}};
+
+template <typename T, int N1, int N2>
+int launch_plane_shape(adscf_ctx* c, const PassArgs<T>& p) {
+  using G = PlaneGeom<T, PassEngine<T, N1, N2>>;
+  const dim3 grid(16 * ((c->nz + 7) / 8)), block(G::threads);
+  switch (pass_mode(p)) {
+    case PM_FIRST:
+      hipLaunchKernelGGL((k_plane_pass<T, N1, N2, PM_FIRST>), grid, block, G::lds_bytes, c->stream, p);
+      break;
+    case PM_MERGED:
+      hipLaunchKernelGGL((k_plane_pass<T, N1, N2, PM_MERGED>), grid, block, G::lds_bytes, c->stream, p);
+      break;
+    case PM_LAST:
+      hipLaunchKernelGGL((k_plane_pass<T, N1, N2, PM_LAST>), grid, block, G::lds_bytes, c->stream, p);
+      break;
+    default:
+      throw ArgError{"plane pass of a kind the sweep does not build"};
+  }
+  HIPCK(hipGetLastError());
+  return (int)grid.x;
+}
+
+template <typename T>
+using PlaneLauncher = int (*)(adscf_ctx*, const PassArgs<T>&);
+
+// launcher and LDS bytes of a plane shape, or null / 0 if it is not compiled
+template <typename T>
+PlaneLauncher<T> plane_launcher(int n, int N1, int N2, size_t* lds) {
+#define ADSCF_PLANE(a, b, cc)                                               \
+  if (n == a && N1 == b && N2 == cc) {                                      \
+    if (lds) *lds = PlaneGeom<T, PassEngine<T, b, cc>>::lds_bytes;          \
+    return &launch_plane_shape<T, b, cc>;                                   \
+  }
+  ADSCF_PLANE(32, 4, 8) ADSCF_PLANE(32, 8, 4) ADSCF_PLANE(48, 8, 6) ADSCF_PLANE(48, 6, 8) ADSCF_PLANE(64, 4, 0)
+  ADSCF_PLANE(64, 8, 8)
+#undef ADSCF_PLANE
+  if (lds) *lds = 0;
+  return nullptr;
+}
+
+size_t device_lds_max() {
+  static size_t v = 0;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    int dev = 0, b = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&b, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess)
+      v = (size_t)b;
+  });
+  return v;
+}
+
+// Whether the grid and shape allow plane passes in the precision of T: square xy planes, a
+// third axis to alternate with, a compiled plane shape and its LDS within the device's limit.
+template <typename T>
+bool plane_allowed(const adscf_ctx* c, int N1, int N2) {
+  if (c->nx != c->ny || c->nz < 2 || c->nx < 2) return false;
+  size_t lds = 0;
+  if (!plane_launcher<T>(c->nx, N1, N2, &lds)) return false;
+  return lds <= device_lds_max();
+}
+
+// Engine shape of the plane operators in the precision of T: the one the caller or the tuner
+// set, else the default, which is the tuner's choice at 64^3 x 200 on the MI355X (FP32 the cube
+// engine, FP64 8 x 8) and the first compiled shape at other lengths.
+template <typename T>
+void plane_shape(const adscf_ctx* c, int& N1, int& N2) {
+  N1 = N2 = 0;
+  if (c->plane_shape_set) {
+    N1 = c->plane_N1;
+    N2 = c->plane_N2;
+    return;
+  }
+  for (const PlaneShape& s : kPlaneShapes)
+    if (s.n == c->nx) {
+      N1 = s.N1;
+      N2 = s.N2;
+      break;
+    }
+  if (c->nx == 64) {
+    N1 = sizeof(T) == 8 ? 8 : 4;
+    N2 = sizeof(T) == 8 ? 8 : 0;
+  }
+}
+
+// whether build_sweep alternates plane passes with z passes
+template <typename T>
+bool plane_sweep(const adscf_ctx* c) {
+  if (c->plane == 0) return false;
+  int N1, N2;
+  plane_shape<T>(c, N1, N2);
+  return plane_allowed<T>(c, N1, N2);
+}
+
 // Default schedule for an axis of n points: the first registered shape, one wave
 // per block; at n = 64 and 128 the choice the autotuner makes on the MI355X for the
 // SCF sweep (n = 64: the cube engine for FP64 on every axis and for FP32 on x, 8 x 8
@@ -524,6 +643,13 @@ int launch_pass_reg(adscf_ctx* c, const PassArgs<T>& p) {
 // launches one line pass; returns the number of workgroups (OUT_DOT writes one partial each)
 template <typename T>
 int launch_pass(adscf_ctx* c, const PassArgs<T>& p) {
+  if (p.axis == kAxisPlane) {
+    int N1, N2;
+    plane_shape<T>(c, N1, N2);
+    PlaneLauncher<T> f = plane_launcher<T>(c->nx, N1, N2, nullptr);
+    if (!f) throw ArgError{"no plane kernel for this shape"};
+    return f(c, p);
+  }
   {
     const int items = launch_pass_reg<T>(c, p);
     if (items > 0) return items;
@@ -569,7 +695,8 @@ std::vector<PassArgs<T>> build_sweep(adscf_ctx* c, const ChainDesc& cd, const T*
   // of consecutive steps meet and share a pass; the middle axis gets one pass per step.  The
   // line operators of different axes commute, so any order gives the same propagator.
   std::vector<int> A = active_axes(c);
-  if (A.size() == 3 && c->mid_axis >= 0 && c->mid_axis != A[1]) {
+  if (A.size() == 3 && plane_sweep<T>(c)) A = {kAxisPlane, 2};  // plane passes alternate with z passes
+  else if (A.size() == 3 && c->mid_axis >= 0 && c->mid_axis != A[1]) {
     std::vector<int> o;
     for (int a : A)
       if (a != c->mid_axis) o.push_back(a);
@@ -1520,7 +1647,11 @@ void epd_field_step_impl(adscf_ctx* c, double dt, int method) {
 // the schedule and buffers a captured sweep bakes in
 template <typename T>
 std::string sweep_key(adscf_ctx* c) {
-  std::string k = std::to_string(sizeof(T)) + "/" + std::to_string(c->mid_axis) + "/" + std::to_string(c->cd.ns);
+  int pN1, pN2;
+  plane_shape<T>(c, pN1, pN2);
+  std::string k = std::to_string(sizeof(T)) + "/" + std::to_string(c->mid_axis) + "/" + std::to_string(c->cd.ns) +
+                  "/p" + std::to_string(plane_sweep<T>(c) ? 1 : 0) + "," + std::to_string(pN1) + "," +
+                  std::to_string(pN2);
   for (int b = 0; b < c->cd.nblocks; ++b) k += "/" + std::to_string(c->cd.type[b]) + ":" + std::to_string(c->cd.seg[b]);
   for (const AxisTables& a : c->ax)
     k += "/" + std::to_string(a.N1) + "," + std::to_string(a.N2) + "," + std::to_string(a.waves) + "," +
@@ -1578,6 +1709,9 @@ template <typename T>
 std::string autotune_impl(adscf_ctx* c, int reps) {
   std::string rep;
   char line[256];
+  const int plane_req = c->plane;
+  c->plane = 0;  // the line schedules first; plane passes are tried last
+  double best_lines = 1e300;
   for (int a : active_axes(c)) {
     AxisTables& ax = c->ax[a];
     if (ax.n < 2) continue;
@@ -1609,6 +1743,7 @@ std::string autotune_impl(adscf_ctx* c, int reps) {
       }
     }
     ax = best;
+    best_lines = best_ms;
     snprintf(line, sizeof line, "axis %d chosen: %s waves=%d tiles=%d (%.4f ms)\n", a,
              shape_name(best.N1, best.N2).c_str(), best.waves, best.tiles, best_ms);
     rep += line;
@@ -1627,7 +1762,38 @@ std::string autotune_impl(adscf_ctx* c, int reps) {
       }
     }
     c->mid_axis = best_mid;
+    best_lines = best_ms;
     snprintf(line, sizeof line, "middle axis chosen: %d (%.4f ms)\n", best_mid, best_ms);
+    rep += line;
+  }
+  // plane passes (x and y of a step in one pass over each z-plane) against the line passes
+  c->plane = plane_req;
+  if (plane_req != 0 && active_axes(c).size() == 3) {
+    int bN1, bN2;
+    plane_shape<T>(c, bN1, bN2);
+    const bool had_shape = c->plane_shape_set;
+    double best_plane = 1e300;
+    for (const PlaneShape& sh : kPlaneShapes) {
+      if (sh.n != c->nx || !plane_allowed<T>(c, sh.N1, sh.N2)) continue;
+      c->plane = 1;
+      c->plane_shape_set = true;
+      c->plane_N1 = sh.N1;
+      c->plane_N2 = sh.N2;
+      const double ms = sweep_timing<T>(c, reps);
+      snprintf(line, sizeof line, "plane shape=%s sweep=%.4f ms\n", shape_name(sh.N1, sh.N2).c_str(), ms);
+      rep += line;
+      if (ms < best_plane) {
+        best_plane = ms;
+        bN1 = sh.N1;
+        bN2 = sh.N2;
+      }
+    }
+    c->plane_N1 = bN1;
+    c->plane_N2 = bN2;
+    c->plane_shape_set = had_shape || best_plane < 1e300;
+    c->plane = best_plane < best_lines ? 1 : (plane_req == 1 ? 1 : 0);
+    snprintf(line, sizeof line, "plane passes chosen: %s (%s, %.4f ms; line passes %.4f ms)\n",
+             c->plane == 1 ? "yes" : "no", shape_name(bN1, bN2).c_str(), best_plane, best_lines);
     rep += line;
   }
   c->keep.retire(c->graph_src, c->graph);
@@ -3660,6 +3826,43 @@ int adscf_get_middle_axis(const adscf_ctx* c, int* axis) {
   return ADSCF_OK;
 }
 
+int adscf_set_plane_passes(adscf_ctx* c, int mode, int n1, int n2) {
+  return guarded(c, [&]() -> int {
+    if (mode < -1 || mode > 1) throw ArgError{"mode must be -1, 0 or 1"};
+    if (n1 != 0 && !plane_launcher<double>(c->nx, n1, n2, nullptr))
+      throw ArgError{"no plane kernel for this shape"};
+    const int old_mode = c->plane, old_n1 = c->plane_N1, old_n2 = c->plane_N2;
+    const bool old_set = c->plane_shape_set;
+    c->plane = mode;
+    if (n1 != 0) {
+      c->plane_shape_set = true;
+      c->plane_N1 = n1;
+      c->plane_N2 = n2;
+    }
+    if (mode == 1 && !(c->prec == ADSCF_FP64 ? plane_sweep<double>(c) : plane_sweep<float>(c))) {
+      c->plane = old_mode;
+      c->plane_shape_set = old_set;
+      c->plane_N1 = old_n1;
+      c->plane_N2 = old_n2;
+      throw ArgError{"plane passes need nx == ny, nz > 1 and a plane shape whose LDS fits"};
+    }
+    invalidate_tables(c);
+    ensure_current_tables(c);
+    return ADSCF_OK;
+  });
+}
+
+int adscf_get_plane_passes(const adscf_ctx* c, int* active, int* n1, int* n2) {
+  if (!c) return ADSCF_ERR_ARG;
+  int N1, N2;
+  if (c->prec == ADSCF_FP64) plane_shape<double>(c, N1, N2);
+  else plane_shape<float>(c, N1, N2);
+  if (active) *active = c->prec == ADSCF_FP64 ? plane_sweep<double>(c) : plane_sweep<float>(c);
+  if (n1) *n1 = N1;
+  if (n2) *n2 = N2;
+  return ADSCF_OK;
+}
+
 int adscf_get_schedule(const adscf_ctx* c, int axis, int* n, int* n1, int* n2, int* waves) {
   if (!c || axis < 0 || axis > 2) return ADSCF_ERR_ARG;
   if (n) *n = c->ax[axis].n;
@@ -3727,6 +3930,7 @@ int adscf_batch_create(adscf_batch** out, int ninst, const int* grid, const doub
       (void)hipStreamSynchronize(c->stream);
       (void)hipStreamDestroy(c->stream);
       c->stream = b->stream;  // every instance runs on the batch stream
+      c->plane = 0;           // the batched passes are line passes
       b->inst.push_back(c);
       bcheck(b, c, adscf_set_interactions(c, kappa[i], chi + 4 * i, vol + 2 * i));
       bcheck(b, c, adscf_scf_setup(c, nblocks[i], monomer_type + boff, block_segments + boff, ensemble, rho, 0.0,

@@ -66,10 +66,29 @@ def run_gpu(adscf_amd, fields, precision, steps, warmup, w0, hipfft=True, tune=F
     ms_iter_ev, ms_sweep = ctx.scf_timing()
     hipfft_us = ctx.hipfft_step_us(200) if hipfft else None
     wall = t1 - t0
+    plane = ctx.plane_passes()
     out = dict(iters_per_s=steps / wall, ms_per_iter=1000.0 * wall / steps, ms_per_iter_events=ms_iter_ev,
                ms_sweep=ms_sweep, iterations_run=st.iters, stopped_early=st.converged, hipfft_step_us=hipfft_us,
-               schedule=[l for l in report.splitlines() if "chosen" in l])
+               schedule=[l for l in report.splitlines() if "chosen" in l], plane=plane,
+               npass=sweep_passes(NS, plane["active"]))
+    if plane["active"]:
+        out["schedule"].append("xy-plane passes: %s" % shape_name(plane))
     return ctx, out
+
+
+def sweep_passes(ns, plane):
+    """Dependent passes of one propagator sweep of both chains over a 3-D grid: with xy-plane
+    passes one per contour step plus one (plane and z passes alternate, the operators of
+    consecutive steps meeting in one pass), with line passes two per step plus one."""
+    return ns + 1 if plane else 2 * ns + 1
+
+
+def sweep_kernels(prec, plane):
+    t = "float" if prec == "fp32" else "double"
+    if plane["active"]:
+        return ("k_plane_pass<%s, %d, %d, *> (x and y operators of a step, one workgroup per chain and z-plane) "
+                "alternating with k_line_pass_reg<%s, *> (z)" % (t, plane["N1"], plane["N2"], t))
+    return "k_line_pass_reg<%s, *> (propagator line passes, all variants of the sweep)" % t
 
 
 def _metric_ctx(adscf_amd, fields, prec, w0):
@@ -302,10 +321,14 @@ def run_epd(adscf_amd, fields, nsteps=10):
     return out
 
 
-def profiled_traffic():
-    """Traffic per line-pass launch (bytes) of the sweep from the committed rocprofv3 PMC
-    summary (FETCH_SIZE and WRITE_SIZE runs, copy-calibrated); None if it is absent."""
-    path = os.path.join(ROOT, "profiles", "line_pass_64cubed_pmc.md")
+TRAFFIC_SOURCES = {False: "profiles/line_pass_64cubed_pmc.md", True: "profiles/sweep_passes_64cubed_pmc.md"}
+
+
+def profiled_traffic(plane):
+    """HBM traffic per pass launch (bytes, mean over a sweep) from the committed rocprofv3 PMC
+    summary of the sweep mode in use (FETCH_SIZE and WRITE_SIZE runs, copy-calibrated):
+    line passes, or xy-plane passes alternating with z passes.  None if it is absent."""
+    path = os.path.join(ROOT, TRAFFIC_SOURCES[bool(plane)])
     try:
         with open(path) as f:
             for line in f:
@@ -360,7 +383,7 @@ def run_large(adscf_amd, fields, steps, warmup):
             st = ctx.scf_iterate(steps, tol=-1.0)
             wall = time.perf_counter() - t0
             _, ms_sweep = ctx.scf_timing()
-        npass = 2 * ns + 1
+            npass = sweep_passes(ns, ctx.plane_passes()["active"])
         bytes_launch = 2 * ns * 3 * n ** 3 * p / npass
         ach = bytes_launch / (ms_sweep * 1e-3 / npass) / 1e9
         out[name] = dict(iters_per_s=steps / wall, ms_per_iter=1e3 * wall / steps, us_per_pass=1e3 * ms_sweep / npass,
@@ -612,12 +635,13 @@ def main():
     ctx32, r32 = run_gpu(adscf_amd, fields, adscf_amd.FP32, args.steps, args.warmup, w0, not args.no_hipfft,
                          args.autotune)
     ctx32.close()
-    npass = 2 * NS + 1
+    npass, npass64 = r32["npass"], r64["npass"]
     sweep_bytes32 = 2 * NS * 3 * ng * 4
     sweep_bytes64 = 2 * NS * 3 * ng * 8
     ach32 = sweep_bytes32 / (r32["ms_sweep"] * 1e-3) / 1e9
     ach64 = sweep_bytes64 / (r64["ms_sweep"] * 1e-3) / 1e9
-    traffic = profiled_traffic()
+    traffic = profiled_traffic(r32["plane"]["active"])
+    traffic64 = profiled_traffic(r64["plane"]["active"])
     biter32 = algorithmic_bytes_iter(ng, NS, M, 4)
     biter64 = algorithmic_bytes_iter(ng, NS, M, 8)
     line = {
@@ -638,15 +662,17 @@ def main():
                    "grid": [N, N, N], "contour_steps": NS, "mixing_dim": M, "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach32 / HBM_PEAK_GBS, "traffic": traffic.get("fp32") if traffic else None,
-                     "traffic_source": "profiles/line_pass_64cubed_pmc.md" if traffic else None,
-                     "kernel": "k_line_pass_reg<float, *> (propagator line passes, all variants of the sweep)",
+                     "traffic_source": TRAFFIC_SOURCES[r32["plane"]["active"]] if traffic else None,
+                     "kernel": sweep_kernels("fp32", r32["plane"]),
+                     "launches_per_sweep": npass,
                      "bytes_per_launch": sweep_bytes32 / npass,
                      "us_per_launch": r32["ms_sweep"] * 1e3 / npass},
         "schedule": r32["schedule"],
         "fp64": {"iters_per_s": r64["iters_per_s"], "ms_per_iter": r64["ms_per_iter"], "schedule": r64["schedule"],
                  "roofline_achieved_GBs": ach64, "roofline_frac": ach64 / HBM_PEAK_GBS,
-                 "us_per_pass": r64["ms_sweep"] * 1e3 / npass,
-                 "traffic_per_pass": traffic.get("fp64") if traffic else None,
+                 "kernel": sweep_kernels("fp64", r64["plane"]), "launches_per_sweep": npass64,
+                 "us_per_pass": r64["ms_sweep"] * 1e3 / npass64,
+                 "traffic_per_pass": traffic64.get("fp64") if traffic64 else None,
                  "iteration_bytes_frac": biter64 * r64["iters_per_s"] / (HBM_PEAK_GBS * 1e9)},
         "fp32mixed_iteration_bytes_frac": biter32 * r32["iters_per_s"] / (HBM_PEAK_GBS * 1e9),
         "speedup_fp32mixed_vs_fp64": r32["iters_per_s"] / r64["iters_per_s"],

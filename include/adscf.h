@@ -300,6 +300,16 @@ int adscf_set_schedule(adscf_ctx* ctx, int axis, int n1, int n2, int waves);
  * run one tile); adscf_autotune also chooses it */
 int adscf_set_pass_tiles(adscf_ctx* ctx, int axis, int tiles);
 int adscf_get_pass_tiles(const adscf_ctx* ctx, int axis, int* tiles);
+/* xy-plane passes (no counterpart in the reference): the x and y operators of a contour step
+ * applied by one workgroup per chain and z-plane, so that a 3-D sweep alternates plane passes
+ * and z passes and costs one dependent pass per contour step instead of two.  mode -1 (the
+ * default) uses them where the grid allows (nx == ny in {32, 48, 64}, nz > 1), 0 never, 1
+ * requires them (ADSCF_ERR_ARG where the grid does not allow them).  n1, n2 select the
+ * engine shape of the plane operators as in adscf_set_schedule (n1 = 0 keeps the current
+ * one).  adscf_autotune also chooses them.  active reports whether the sweep of the
+ * context's precision uses them. */
+int adscf_set_plane_passes(adscf_ctx* ctx, int mode, int n1, int n2);
+int adscf_get_plane_passes(const adscf_ctx* ctx, int* active, int* n1, int* n2);
 
 /* ---- batched independent instances (no counterpart in the reference) ---------- */
 /* Many independent two-type SCF problems (parameter sweeps) of uneven grid and contour

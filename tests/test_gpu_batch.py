@@ -36,6 +36,7 @@ def test_batch_matches_single_instances(prec):
         fb = b.energy()
     for i, d in enumerate(inst):
         with adscf_amd.Context(*d["grid"], *d["box"], d["ds"], precision=P) as ctx:
+            ctx.set_plane_passes(0)  # batched sweeps run line passes
             for a in range(3):
                 n = d["grid"][a]
                 sch = {16: (1, 16), 32: (4, 8), 48: (8, 6)}[n]

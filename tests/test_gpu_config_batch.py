@@ -100,9 +100,10 @@ def test_sweep_instances_match_single_contexts_past_m(sweep, persistent):
         got = [b.get(k) for k in range(len(sub))]
     assert np.all(it == iters) and not conv.any()
     for k, d in enumerate(sub):
-        # a context's default schedules (FP64: cube4 at 64, 4x8 at 32, 8x6 at 48) are the
-        # one-wave register schedules a batch instance keeps
+        # a context's default line schedules (FP64: cube4 at 64, 4x8 at 32, 8x6 at 48) are the
+        # one-wave register schedules a batch instance keeps; batched sweeps run line passes
         with adscf_amd.Context(*d["grid"], *d["box"], d["ds"], precision=adscf_amd.FP64) as ctx:
+            ctx.set_plane_passes(0)
             ctx.set_interactions(d["kappa"], d["chi"], d["vol"])
             ctx.scf_setup(d["blocks"], d["segments"], mixing_dim=M, lam_mix=LAM)
             ctx.scf_set_field(ws[k])
