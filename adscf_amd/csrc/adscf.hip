@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <type_traits>
@@ -407,6 +408,119 @@ std::string shape_name(int N1, int N2) {
 
 int reg_tpl_rt(int N1, int N2) { return N2 <= 0 ? N1 * N1 : (N1 == 1 ? 1 : std::max(N1, N2)); }
 
+// ---------------------------------------------------------------------------
+// LDS line strides.  A workgroup keeps one exchange area per line; where the areas start
+// decides which banks the lines of a wave hit together.  The best offset depends on the
+// engine's access pattern and on how lines map to lanes (x passes and the row layout of a
+// plane pass: a line's threads on consecutive lanes; y/z passes and the column layout: lines
+// on consecutive lanes), so the host picks it per pass from a model of the LDS: per
+// instruction, the lanes of a wave go through in groups that fill the banks once, and a group
+// costs the largest number of its accesses that fall in one bank.  The model is evaluated for
+// 64 banks (the tie broken by 32) over strides from the engine's element count up.
+// ---------------------------------------------------------------------------
+
+// pat[i][t]: element offset in a line's exchange area of thread t's access in LDS instruction
+// i of one line operator (-1: idle), as the engines of fft_reg.hpp and fft_cube.hpp issue them;
+// each listed pattern is issued twice per operator (a store and a load of the same slots)
+std::vector<std::vector<int>> engine_lds_pattern(int N1, int N2, int* tpl) {
+  std::vector<std::vector<int>> pat;
+  const int TPL = reg_tpl_rt(N1, N2);
+  *tpl = TPL;
+  auto add = [&](auto f) {
+    std::vector<int> r(TPL);
+    for (int t = 0; t < TPL; ++t) r[t] = f(t);
+    pat.push_back(r);
+  };
+  if (N2 > 0) {  // four-step (fft_reg.hpp): rows of N1 + 1
+    if (N1 == 1) return pat;
+    const int n1p = N1 + 1;
+    for (int k2 = 0; k2 < N2; ++k2) add([&](int t) { return t < N1 ? t + n1p * k2 : -1; });  // A, C
+    for (int j1 = 0; j1 < N1; ++j1) add([&](int t) { return t < N2 ? j1 + n1p * t : -1; });  // B
+  } else if (N2 == 0) {  // cube (fft_cube.hpp): S then S2, rows of R + 1
+    const int R = N1, sp = R + 1, S2 = R * R * sp;
+    for (int q = 0; q < R; ++q) add([&](int t) { return t * sp + q; });                            // a, ai
+    for (int b = 0; b < R; ++b) add([&](int t) { return (t % R + R * b) * sp + t / R; });           // b, bi
+    for (int c = 0; c < R; ++c) add([&](int t) { return S2 + ((t / R) * R + c) * sp + t % R; });   // b, bi
+    for (int q = 0; q < R; ++q) add([&](int t) { return S2 + t * sp + q; });                        // c
+  } else {  // A R^2 (fft_cube.hpp, tri_*): S rows of A + 1, S2 rows of R + 1
+    const int A = -N2, R = N1, sa = A + 1, sb = R + 1, TT = R * R, S2 = TT * sa, P = A / R;
+    for (int q = 0; q < A; ++q) add([&](int t) { return t * sa + q; });
+    for (int i = 0; i < P; ++i) {
+      for (int b = 0; b < R; ++b)
+        add([&](int t) {
+          const int p = t + TT * i;
+          return (p % R + R * b) * sa + p / R;
+        });
+      for (int c = 0; c < R; ++c)
+        add([&](int t) {
+          const int p = t + TT * i;
+          return S2 + ((p / R) * R + c) * sb + p % R;
+        });
+      for (int q = 0; q < R; ++q) add([&](int t) { return S2 + (t + TT * i) * sb + q; });
+    }
+  }
+  return pat;
+}
+
+// Model cost of the exchanges of `lines` lines with their areas `stride` elements apart;
+// thread t of line l is block thread l TPL + t (across = false) or l + lines t (across = true).
+long lds_model_cost(const std::vector<std::vector<int>>& pat, int tpl, int lines, bool across, int stride,
+                    int words, int banks) {
+  const int group = std::max(1, banks / words);  // lanes per bank cycle
+  const int nthreads = tpl * lines;
+  long cost = 0;
+  std::vector<int> cnt(banks);
+  for (const auto& ins : pat)
+    for (int w0 = 0; w0 < nthreads; w0 += 64)
+      for (int g0 = w0; g0 < std::min(w0 + 64, nthreads); g0 += group) {
+        std::fill(cnt.begin(), cnt.end(), 0);
+        int worst = 0;
+        for (int i = g0; i < std::min(g0 + group, nthreads); ++i) {
+          const int l = across ? i % lines : i / tpl, t = across ? i / lines : i % tpl;
+          const int e = ins[t];
+          if (e < 0) continue;
+          const long addr = ((long)l * stride + e) * words;
+          for (int k = 0; k < words; ++k) worst = std::max(worst, ++cnt[(addr + k) % banks]);
+        }
+        cost += worst;
+      }
+  return cost;
+}
+
+// stride (elements, at least the engine's count) of the exchange areas of a pass, cached
+int line_stride(int N1, int N2, int elem_bytes, int lines, bool across, int lds) {
+  if (lds <= 0) return 0;
+  static std::mutex mu;
+  static std::map<std::vector<int>, int> cache;
+  const std::vector<int> key = {N1, N2, elem_bytes, lines, across ? 1 : 0, lds};
+  std::lock_guard<std::mutex> g(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int tpl = 1;
+  const auto pat = engine_lds_pattern(N1, N2, &tpl);
+  const int words = elem_bytes / 4;
+  int best = lds;
+  long b64 = -1, b32 = -1;
+  for (int s = lds; s < lds + 64; ++s) {
+    const long c64 = lds_model_cost(pat, tpl, lines, across, s, words, 64);
+    const long c32 = lds_model_cost(pat, tpl, lines, across, s, words, 32);
+    if (b64 < 0 || c64 < b64 || (c64 == b64 && c32 < b32)) {
+      b64 = c64;
+      b32 = c32;
+      best = s;
+    }
+  }
+  cache[key] = best;
+  return best;
+}
+
+// LDS elements of one line's exchange area in the register engine (N1, N2)
+int reg_lds_elems(int N1, int N2) {
+  return N2 == 0  ? 2 * N1 * N1 * (N1 + 1)
+         : N2 < 0 ? N1 * N1 * (1 - N2) + (-N2) * N1 * (N1 + 1)
+                  : (N1 == 1 ? 0 : N2 * (N1 + 1));
+}
+
 // Tile geometry of a register-resident pass: `waves` wavefronts per block, L lines
 // per block; fills p.L and p.gx and returns the number of blocks.
 template <typename T>
@@ -421,10 +535,8 @@ int prep_reg(const adscf_ctx* c, PassArgs<T>& p, int N1, int N2, int waves, int*
   else if (p.axis == 1) items = c->nz * p.gx;
   else items = c->ny * p.gx;
   if (threads) *threads = TPL * L;
-  const size_t per_line = N2 == 0  ? (size_t)2 * N1 * N1 * (N1 + 1)
-                          : N2 < 0 ? (size_t)N1 * N1 * (1 - N2) + (size_t)(-N2) * N1 * (N1 + 1)
-                                   : (N1 == 1 ? 0 : (size_t)N2 * (N1 + 1));
-  if (smem) *smem = (size_t)L * line_lds_stride((int)per_line) * sizeof(C2<T>);
+  p.lds_line = line_stride(N1, N2, (int)sizeof(C2<T>), L, p.axis != 0, reg_lds_elems(N1, N2));
+  if (smem) *smem = (size_t)L * p.lds_line * sizeof(C2<T>);
   return items;
 }
 
@@ -505,19 +617,31 @@ struct PlaneShape {
 };
 const PlaneShape kPlaneShapes[] = {{32, 4, 8}, {32, 8, 4}, {48, 8, 6}, {48, 6, 8}, {64, 4, 0}, {64, 8, 8}};
 
+// exchange-area strides of a plane pass (row and column layouts) and its LDS bytes
 template <typename T, int N1, int N2>
-int launch_plane_shape(adscf_ctx* c, const PassArgs<T>& p) {
+size_t plane_strides(int* sx, int* sy) {
   using G = PlaneGeom<T, PassEngine<T, N1, N2>>;
+  const int lds = reg_lds_elems(N1, N2);
+  *sx = line_stride(N1, N2, (int)sizeof(C2<T>), G::NL, false, lds);
+  *sy = line_stride(N1, N2, (int)sizeof(C2<T>), G::NL, true, lds);
+  return G::lds_bytes(std::max(*sx, *sy));
+}
+
+template <typename T, int N1, int N2>
+int launch_plane_shape(adscf_ctx* c, const PassArgs<T>& p0) {
+  using G = PlaneGeom<T, PassEngine<T, N1, N2>>;
+  PassArgs<T> p = p0;
+  const size_t smem = plane_strides<T, N1, N2>(&p.lds_line, &p.lds_line2);
   const dim3 grid(16 * ((c->nz + 7) / 8)), block(G::threads);
   switch (pass_mode(p)) {
     case PM_FIRST:
-      hipLaunchKernelGGL((k_plane_pass<T, N1, N2, PM_FIRST>), grid, block, G::lds_bytes, c->stream, p);
+      hipLaunchKernelGGL((k_plane_pass<T, N1, N2, PM_FIRST>), grid, block, smem, c->stream, p);
       break;
     case PM_MERGED:
-      hipLaunchKernelGGL((k_plane_pass<T, N1, N2, PM_MERGED>), grid, block, G::lds_bytes, c->stream, p);
+      hipLaunchKernelGGL((k_plane_pass<T, N1, N2, PM_MERGED>), grid, block, smem, c->stream, p);
       break;
     case PM_LAST:
-      hipLaunchKernelGGL((k_plane_pass<T, N1, N2, PM_LAST>), grid, block, G::lds_bytes, c->stream, p);
+      hipLaunchKernelGGL((k_plane_pass<T, N1, N2, PM_LAST>), grid, block, smem, c->stream, p);
       break;
     default:
       throw ArgError{"plane pass of a kind the sweep does not build"};
@@ -532,10 +656,11 @@ using PlaneLauncher = int (*)(adscf_ctx*, const PassArgs<T>&);
 // launcher and LDS bytes of a plane shape, or null / 0 if it is not compiled
 template <typename T>
 PlaneLauncher<T> plane_launcher(int n, int N1, int N2, size_t* lds) {
-#define ADSCF_PLANE(a, b, cc)                                               \
-  if (n == a && N1 == b && N2 == cc) {                                      \
-    if (lds) *lds = PlaneGeom<T, PassEngine<T, b, cc>>::lds_bytes;          \
-    return &launch_plane_shape<T, b, cc>;                                   \
+#define ADSCF_PLANE(a, b, cc)                                  \
+  if (n == a && N1 == b && N2 == cc) {                         \
+    int sx, sy;                                                \
+    if (lds) *lds = plane_strides<T, b, cc>(&sx, &sy);         \
+    return &launch_plane_shape<T, b, cc>;                      \
   }
   ADSCF_PLANE(32, 4, 8) ADSCF_PLANE(32, 8, 4) ADSCF_PLANE(48, 8, 6) ADSCF_PLANE(48, 6, 8) ADSCF_PLANE(64, 4, 0)
   ADSCF_PLANE(64, 8, 8)

@@ -52,6 +52,8 @@ struct PassArgs {
   const T* kf;      // n multipliers exp(-ds q_k^2) / n
   const C2<T>* tw2; // plane passes (axis 3): twiddles and multipliers of the y operator
   const T* kf2;
+  int lds_line;     // LDS elements between the exchange areas of consecutive lines (register
+  int lds_line2;    // engines; lds_line2: the column layout of a plane pass), chosen by the host
   int nops;         // 1, or 2 for a merged pass (op, mid store, op)
   int in_kind;
   int pre_E;        // +1: multiply the loaded lines by (Ea1, Ea2); -1: divide by them
@@ -364,12 +366,6 @@ struct EngTri {  // n = A R^2 on R^2 threads (fft_cube.hpp, tri_*)
   }
 };
 
-// LDS elements between the exchange areas of consecutive lines of a workgroup: the engine's
-// count made odd.  Lines that sit on consecutive lanes (y and z passes, the column layout of
-// plane passes) then start in different banks; with an even count (the cube engine's 160,
-// the 8 x 8 four-step's 72) they all start in one bank group and their accesses serialise.
-constexpr int line_lds_stride(int lds) { return lds == 0 ? 0 : (lds | 1); }
-
 // shape (N1, N2) of the schedule tables: N2 == 0 selects the cube engine with R = N1,
 // N2 < 0 the A R^2 engine with A = -N2 and R = N1, N2 > 0 the N1 x N2 four-step
 template <typename T, int N1, int N2>
@@ -485,7 +481,7 @@ __device__ __forceinline__ void reg_pass_body(const PassArgs<T>& a, int item, un
   const unsigned fstep = (unsigned)(STRIDE * a.stride) * (unsigned)sizeof(T);
   auto fo = [&](int j2) { return f0 + (unsigned)j2 * fstep; };
   auto fc = [&](int j2) { return 2u * (f0 + (unsigned)j2 * fstep); };
-  C2<T>* lds = reinterpret_cast<C2<T>*>(smem_raw) + l * line_lds_stride(ENG::LDS);
+  C2<T>* lds = reinterpret_cast<C2<T>*>(smem_raw) + l * a.lds_line;
   const bool wl = lines_in_wave<TPL>(a.axis);
   const bool act = t < ENG::NACT && valid;
   const int stop = *(a.done ? a.done : &g_no_stop);
@@ -638,7 +634,7 @@ __device__ __forceinline__ void reg_pass_tiles(const PassArgs<T>& a, int nitems,
     l = tid % a.L;
     t = tid / a.L;
   }
-  C2<T>* lds = reinterpret_cast<C2<T>*>(smem_raw) + l * line_lds_stride(ENG::LDS);
+  C2<T>* lds = reinterpret_cast<C2<T>*>(smem_raw) + l * a.lds_line;
   const bool wl = lines_in_wave<TPL>(a.axis);
   const int stop = *(a.done ? a.done : &g_no_stop);
   const unsigned fstep = (unsigned)(STRIDE * a.stride) * (unsigned)sizeof(T);

@@ -21,7 +21,9 @@
 //   y layout  thread id = t NL + j: line j (columns j, j + NL), points y = t + STRIDE m;
 //             consecutive lanes hold consecutive x, so the E loads and the q(s) stores
 //             of the step boundary are coalesced rows
-// The plane moves between the layouts through an LDS array of n padded rows.  The packed
+// The plane moves between the layouts through an LDS array of n padded rows.  The exchange
+// areas of the lines are a.lds_line (x layout) or a.lds_line2 (y layout) elements apart, so
+// that each layout's lines start in banks the host's model finds free of conflicts.  The packed
 // line buffer between passes keeps the (q, q+) pairs of the z passes; a plane workgroup
 // reads and writes its chain's component.
 #pragma once
@@ -39,9 +41,8 @@ struct PlaneGeom {
   static constexpr int threads = ENG::TPL * NL;
   static constexpr int ld = n + 1;  // padded row of the real plane in LDS
   static constexpr size_t plane_bytes = ((size_t)n * ld * sizeof(T) + 15) / 16 * 16;
-  static constexpr int lds_line = line_lds_stride(ENG::LDS);  // scratch elements per line
-  static constexpr size_t scratch_bytes = (size_t)NL * lds_line * sizeof(C2<T>);
-  static constexpr size_t lds_bytes = plane_bytes + scratch_bytes;
+  // LDS of a workgroup whose lines' exchange areas are `stride` elements apart
+  static constexpr size_t lds_bytes(int stride) { return plane_bytes + (size_t)NL * stride * sizeof(C2<T>); }
 };
 
 // Workgroup b takes chain (b / 8) % 2 of plane 8 (b / 16) + b % 8: both chains of a plane
@@ -123,7 +124,7 @@ __global__ void __launch_bounds__((PlaneGeom<T, PassEngine<T, N1, N2>>::threads)
       v[m].y *= e1[m];
     }
   }
-  ENG::op(v, scr + jx * G::lds_line, tx, Rx, wlx);
+  ENG::op(v, scr + jx * a.lds_line, tx, Rx, wlx);
   if (stop) return;  // uniform over the block; nothing stored yet
   // x layout -> y layout
   if (actx) {
@@ -141,7 +142,7 @@ __global__ void __launch_bounds__((PlaneGeom<T, PassEngine<T, N1, N2>>::threads)
       v[m].y = P[(ty + STRIDE * m) * LD + jy + NL];
     }
   }
-  ENG::op(v, scr + jy * G::lds_line, ty, Ry, false);
+  ENG::op(v, scr + jy * a.lds_line2, ty, Ry, false);
   if (FIRST) {
     if (acty) {
 #pragma unroll
@@ -176,7 +177,7 @@ __global__ void __launch_bounds__((PlaneGeom<T, PassEngine<T, N1, N2>>::threads)
       v[m] = x;
     }
   }
-  ENG::op(v, scr + jy * G::lds_line, ty, Ry, false);
+  ENG::op(v, scr + jy * a.lds_line2, ty, Ry, false);
   // y layout -> x layout (the y operators' barriers have retired every read of the
   // first transpose)
   if (acty) {
@@ -194,7 +195,7 @@ __global__ void __launch_bounds__((PlaneGeom<T, PassEngine<T, N1, N2>>::threads)
       v[m].y = P[(jx + NL) * LD + tx + STRIDE * m];
     }
   }
-  ENG::op(v, scr + jx * G::lds_line, tx, Rx, wlx);
+  ENG::op(v, scr + jx * a.lds_line, tx, Rx, wlx);
   if (actx) {
 #pragma unroll
     for (int m = 0; m < NP; ++m) {
@@ -205,3 +206,4 @@ __global__ void __launch_bounds__((PlaneGeom<T, PassEngine<T, N1, N2>>::threads)
 }
 
 }  // namespace adscf
+
