@@ -615,15 +615,15 @@ RegLauncher<T> reg_launcher(int n, int N1, int N2) {
 struct PlaneShape {
   int n, N1, N2;
 };
-const PlaneShape kPlaneShapes[] = {{32, 4, 8}, {32, 8, 4}, {48, 8, 6}, {48, 6, 8}, {64, 4, 0}, {64, 8, 8}};
+const PlaneShape kPlaneShapes[] = {{32, 4, 8}, {32, 8, 4}, {48, 8, 6}, {48, 6, 8}, {64, 4, 0}, {64, 8, 8}, {128, 4, -8}};
 
 // exchange-area strides of a plane pass (row and column layouts) and its LDS bytes
 template <typename T, int N1, int N2>
 size_t plane_strides(int* sx, int* sy) {
   using G = PlaneGeom<T, PassEngine<T, N1, N2>>;
   const int lds = reg_lds_elems(N1, N2);
-  *sx = line_stride(N1, N2, (int)sizeof(C2<T>), G::NL, false, lds);
-  *sy = line_stride(N1, N2, (int)sizeof(C2<T>), G::NL, true, lds);
+  *sx = line_stride(N1, N2, (int)sizeof(C2<T>), G::NLT, false, lds);
+  *sy = line_stride(N1, N2, (int)sizeof(C2<T>), G::NLT, true, lds);
   return G::lds_bytes(std::max(*sx, *sy));
 }
 
@@ -664,6 +664,11 @@ PlaneLauncher<T> plane_launcher(int n, int N1, int N2, size_t* lds) {
   }
   ADSCF_PLANE(32, 4, 8) ADSCF_PLANE(32, 8, 4) ADSCF_PLANE(48, 8, 6) ADSCF_PLANE(48, 6, 8) ADSCF_PLANE(64, 4, 0)
   ADSCF_PLANE(64, 8, 8)
+  // 128-point planes in FP32 only (the 8 x 4 x 4 engine, two lines per thread): an FP64
+  // plane of 128 x 128 and its exchange areas exceed the workgroup's LDS
+  if constexpr (sizeof(T) == 4) {
+    ADSCF_PLANE(128, 4, -8)
+  }
 #undef ADSCF_PLANE
   if (lds) *lds = 0;
   return nullptr;
@@ -3954,8 +3959,9 @@ int adscf_get_middle_axis(const adscf_ctx* c, int* axis) {
 int adscf_set_plane_passes(adscf_ctx* c, int mode, int n1, int n2) {
   return guarded(c, [&]() -> int {
     if (mode < -1 || mode > 1) throw ArgError{"mode must be -1, 0 or 1"};
-    if (n1 != 0 && !plane_launcher<double>(c->nx, n1, n2, nullptr))
-      throw ArgError{"no plane kernel for this shape"};
+    if (n1 != 0 && !(c->prec == ADSCF_FP64 ? (bool)plane_launcher<double>(c->nx, n1, n2, nullptr)
+                                            : (bool)plane_launcher<float>(c->nx, n1, n2, nullptr)))
+      throw ArgError{"no plane kernel for this shape in this precision"};
     const int old_mode = c->plane, old_n1 = c->plane_N1, old_n2 = c->plane_N2;
     const bool old_set = c->plane_shape_set;
     c->plane = mode;

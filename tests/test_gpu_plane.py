@@ -14,7 +14,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 FP64_RTOL = 1e-10
-SHAPES = {32: [(4, 8), (8, 4)], 48: [(8, 6), (6, 8)], 64: [(4, 0), (8, 8)]}
+SHAPES = {32: [(4, 8), (8, 4)], 48: [(8, 6), (6, 8)], 64: [(4, 0), (8, 8)], 128: [(4, -8)]}
 CASES = [
     # dims, box, ds, block types, segments, ensemble
     ((32, 32, 12), (4.0, 4.5, 3.0), 0.02, [1, 2], [7, 6], 1),          # odd Ns: last pass along z
@@ -69,7 +69,7 @@ def test_plane_sweep_matches_oracle_and_line_passes(amd, orc, case, which):
     assert abs(got.Qsum - lines.Qsum) < 1e-12 * abs(lines.Qsum)
 
 
-@pytest.mark.parametrize("n,shape", [(32, (4, 8)), (64, (4, 0)), (64, (8, 8))])
+@pytest.mark.parametrize("n,shape", [(32, (4, 8)), (64, (4, 0)), (64, (8, 8)), (128, (4, -8))])
 def test_plane_sweep_fp32_error_bound(amd, orc, n, shape):
     dims, L, ds = (n, n, 16), (4.0, 4.0, 3.0), 0.01
     print(f"plane sweep FP32 dims={dims} shape={shape}", flush=True)
@@ -118,3 +118,31 @@ def test_plane_scf_iteration_matches_line_passes(amd):
             acc[mode] = np.array(hist)
     print("line passes", acc[0], "plane passes", acc[1])
     assert np.all(np.abs(acc[1] - acc[0]) <= 1e-9 * np.abs(acc[0]))
+
+
+@pytest.mark.parametrize("ns", [(7, 6), (5, 5)])
+def test_plane_sweep_128_fp32_matches_line_passes(amd, orc, ns):
+    """128-point planes (two lines per thread, FP32 only): against the line passes of the same
+    context at the FP32 rounding level and against the oracle at the stated 2e-5, with the
+    sweep ending on a z pass (odd Ns) and on a plane pass (even Ns)."""
+    dims, L, ds = (128, 128, 10), (6.0, 6.0, 2.0), 0.01
+    print(f"plane sweep 128 FP32 Ns={sum(ns)}", flush=True)
+    g = orc.Grid(*dims, *L, ds)
+    w = 2.0 * np.random.default_rng(8).standard_normal((2,) + g.shape)
+    E = orc.expfield(g, w)
+    ref = orc.propagate_chain(g, [1, 2], list(ns), E)
+    with amd.Context(*dims, *L, ds, precision=amd.FP32) as ctx:
+        ctx.set_plane_passes(1, 4, -8)
+        assert ctx.plane_passes()["active"]
+        got = ctx.propagate_chain([1, 2], list(ns), E)
+        ctx.set_plane_passes(0)
+        lines = ctx.propagate_chain([1, 2], list(ns), E)
+    assert _rel(got.density, ref.density) < 2e-5 and abs(got.Qsum - ref.Qsum) < 2e-5 * abs(ref.Qsum)
+    assert _rel(got.density, lines.density) < 2e-6
+
+
+def test_plane_passes_not_offered_for_fp64_128(amd):
+    with amd.Context(128, 128, 8, 6.0, 6.0, 2.0, 0.01) as ctx:  # FP64: the plane does not fit the LDS
+        assert not ctx.plane_passes()["active"]
+        with pytest.raises(amd.AdscfError):
+            ctx.set_plane_passes(1, 4, -8)
