@@ -115,6 +115,7 @@ SIGNATURES = {
     "adscf_batch_set_fields": (C.c_int, [_ctxp, _dp]),
     "adscf_batch_set_precision": (C.c_int, [_ctxp, C.c_int]),
     "adscf_batch_set_persistent": (C.c_int, [_ctxp, C.c_int, _ip]),
+    "adscf_batch_set_plane_passes": (C.c_int, [_ctxp, C.c_int, _ip]),
     "adscf_batch_iterate": (C.c_int, [_ctxp, C.c_int, C.c_double, _ip, _dp, _ip]),
     "adscf_batch_energy": (C.c_int, [_ctxp, _dp]),
     "adscf_batch_get": (C.c_int, [_ctxp, C.c_int, _dp, _dp]),
@@ -685,6 +686,13 @@ class Batch:
         act = np.zeros(1, dtype=np.int32)
         self._chk(self._lib.adscf_batch_set_persistent(self._h, int(bool(enable)), _pi(act)))
         return bool(act[0])
+
+    def set_plane_passes(self, mode=-1):
+        """xy-plane passes of the instances' sweeps: -1 where the grid allows, 0 never, 1 on
+        every instance; returns the number of instances whose sweep uses them."""
+        act = np.zeros(1, dtype=np.int32)
+        self._chk(self._lib.adscf_batch_set_plane_passes(self._h, int(mode), _pi(act)))
+        return int(act[0])
 
     def iterate(self, niter, tol=0.0):
         it = np.zeros(self.n, dtype=np.int32)

@@ -8,6 +8,7 @@
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -653,25 +654,42 @@ int launch_plane_shape(adscf_ctx* c, const PassArgs<T>& p0) {
 template <typename T>
 using PlaneLauncher = int (*)(adscf_ctx*, const PassArgs<T>&);
 
+// The compiled plane shapes, (n, N1, N2); 128-point planes in FP32 only (the 8 x 4 x 4
+// engine, two lines per thread): an FP64 plane of 128 x 128 and its exchange areas exceed the
+// workgroup's LDS.
+#define ADSCF_PLANE_SHAPES(X)                                                          \
+  X(32, 4, 8) X(32, 8, 4) X(48, 8, 6) X(48, 6, 8) X(64, 4, 0) X(64, 8, 8)              \
+  if constexpr (sizeof(T) == 4) {                                                      \
+    X(128, 4, -8)                                                                      \
+  }
+
 // launcher and LDS bytes of a plane shape, or null / 0 if it is not compiled
 template <typename T>
 PlaneLauncher<T> plane_launcher(int n, int N1, int N2, size_t* lds) {
-#define ADSCF_PLANE(a, b, cc)                                  \
-  if (n == a && N1 == b && N2 == cc) {                         \
-    int sx, sy;                                                \
-    if (lds) *lds = plane_strides<T, b, cc>(&sx, &sy);         \
-    return &launch_plane_shape<T, b, cc>;                      \
+#define ADSCF_PLANE(a, b, cc)                          \
+  if (n == a && N1 == b && N2 == cc) {                 \
+    int sx, sy;                                        \
+    if (lds) *lds = plane_strides<T, b, cc>(&sx, &sy); \
+    return &launch_plane_shape<T, b, cc>;              \
   }
-  ADSCF_PLANE(32, 4, 8) ADSCF_PLANE(32, 8, 4) ADSCF_PLANE(48, 8, 6) ADSCF_PLANE(48, 6, 8) ADSCF_PLANE(64, 4, 0)
-  ADSCF_PLANE(64, 8, 8)
-  // 128-point planes in FP32 only (the 8 x 4 x 4 engine, two lines per thread): an FP64
-  // plane of 128 x 128 and its exchange areas exceed the workgroup's LDS
-  if constexpr (sizeof(T) == 4) {
-    ADSCF_PLANE(128, 4, -8)
-  }
+  ADSCF_PLANE_SHAPES(ADSCF_PLANE)
 #undef ADSCF_PLANE
   if (lds) *lds = 0;
   return nullptr;
+}
+
+// Batched plane passes: exchange strides, LDS bytes and threads of a plane shape
+template <typename T>
+bool plane_batch_geom(int n, int N1, int N2, int* sx, int* sy, size_t* smem, int* threads) {
+#define ADSCF_PLANE(a, b, cc)                                   \
+  if (n == a && N1 == b && N2 == cc) {                          \
+    *smem = plane_strides<T, b, cc>(sx, sy);                    \
+    *threads = PlaneGeom<T, PassEngine<T, b, cc>>::threads;     \
+    return true;                                                \
+  }
+  ADSCF_PLANE_SHAPES(ADSCF_PLANE)
+#undef ADSCF_PLANE
+  return false;
 }
 
 size_t device_lds_max() {
@@ -2367,6 +2385,7 @@ struct BatchTables {
     int blocks, N1, N2;
     size_t smem;
     int lane;
+    int plane = 0, mode = 0, threads = 64;  // plane passes: fixed variant and workgroup size
   };
   std::vector<Launch> launches;  // pass-major, one per lane and shape present in the pass
 };
@@ -2410,6 +2429,7 @@ constexpr int kBatchLanes = 4;
 
 struct adscf_batch {
   hipStream_t stream = nullptr;
+  int plane_req = -1;  // xy-plane passes of the instances (adscf_batch_set_plane_passes)
   std::vector<int> lane_of;  // lane of each instance (batch_build_tables)
   hipStream_t side[kBatchLanes - 1] = {};
   hipEvent_t fork = nullptr, join[kBatchLanes - 1] = {};
@@ -2454,7 +2474,8 @@ constexpr auto persist_kernel() {
 // that fills the workgroup as the cube engine needs
 template <typename T>
 bool persist_shapes_ok(adscf_batch* b) {
-  for (adscf_ctx* c : b->inst)
+  for (adscf_ctx* c : b->inst) {
+    if (active_axes(c).size() == 3 && plane_sweep<T>(c)) return false;  // line passes only
     for (int a : active_axes(c)) {
       const AxisTables& ax = c->ax[a];
       if (persist_shape_id(ax.N1, ax.N2) < 0) return false;
@@ -2463,6 +2484,7 @@ bool persist_shapes_ok(adscf_batch* b) {
       prep_reg<T>(c, p, ax.N1, ax.N2, 4, &threads, nullptr);
       if (threads > kPersistThreads || (ax.N2 <= 0 && threads != kPersistThreads)) return false;
     }
+  }
   return true;
 }
 
@@ -2486,6 +2508,7 @@ void batch_build_persist(adscf_batch* b) {
     int prev_tiles = 0;
     for (size_t q = 0; q < ps.size(); ++q) {
       PassArgs<T> p = ps[q];
+      if (p.axis == kAxisPlane) return;  // the persistent kernel carries line passes only
       const AxisTables& ax = c->ax[p.axis];
       const int id = persist_shape_id(ax.N1, ax.N2);
       if (id < 0) return;
@@ -2575,6 +2598,8 @@ void batch_build_tables(adscf_batch* b) {
     PassArgs<T> args;
     int items, N1, N2;
     size_t smem;
+    int plane, mode, threads;  // plane pass: its fixed variant and workgroup size
+    std::array<int, 4> key() const { return {plane, N1, N2, mode}; }
   };
   std::vector<std::vector<Entry>> per;
   size_t npass = 0;
@@ -2584,8 +2609,16 @@ void batch_build_tables(adscf_batch* b) {
     auto ps = build_sweep<T>(c, c->cd, En.E.p, En.h1.p, En.h2.p, c->d1.p, c->d2.p, &c->st.p->done);
     std::vector<Entry> es;
     for (auto& p : ps) {
+      if (p.axis == kAxisPlane) {
+        Entry e{p, 16 * ((c->nz + 7) / 8), 0, 0, 0, 1, pass_mode(p), 0};
+        plane_shape<T>(c, e.N1, e.N2);
+        if (!plane_batch_geom<T>(c->nx, e.N1, e.N2, &e.args.lds_line, &e.args.lds_line2, &e.smem, &e.threads))
+          throw ArgError{"no plane kernel for this shape"};
+        es.push_back(e);
+        continue;
+      }
       const AxisTables& ax = c->ax[p.axis];
-      Entry e{p, 0, ax.N1, ax.N2, 0};
+      Entry e{p, 0, ax.N1, ax.N2, 0, 0, PM_GENERIC, 64};
       e.items = prep_reg<T>(c, e.args, ax.N1, ax.N2, 1, nullptr, &e.smem);
       if (e.items >= (1 << kBatchItemBits)) throw ArgError{"batched pass has too many blocks per instance"};
       es.push_back(e);
@@ -2597,12 +2630,12 @@ void batch_build_tables(adscf_batch* b) {
   // grids, one per axis length); groups go to lanes, largest work first onto the least loaded
   // lane.  A lane's sweep and post-sweep kernels depend on nothing in another lane, so the
   // captured iteration runs the lanes as parallel graph branches.
-  std::vector<std::vector<std::pair<int, int>>> sigs;
+  std::vector<std::vector<std::array<int, 4>>> sigs;
   std::vector<int> group_of(per.size());
   std::vector<double> gcost;
   for (size_t i = 0; i < per.size(); ++i) {
-    std::vector<std::pair<int, int>> sg;
-    for (const Entry& e : per[i]) sg.push_back({e.N1, e.N2});
+    std::vector<std::array<int, 4>> sg;
+    for (const Entry& e : per[i]) sg.push_back({e.plane, e.N1, e.N2, 0});
     auto it = std::find(sigs.begin(), sigs.end(), sg);
     int g = (int)(it - sigs.begin());
     if (it == sigs.end()) {
@@ -2632,23 +2665,25 @@ void batch_build_tables(adscf_batch* b) {
   tb.launches.clear();
   for (size_t p = 0; p < npass; ++p) {
     for (int lane = 0; lane < kBatchLanes; ++lane) {
-      std::vector<std::pair<int, int>> shapes;  // distinct shapes of this pass in the lane
+      std::vector<std::array<int, 4>> shapes;  // distinct kernels of this pass in the lane
       for (size_t i = 0; i < per.size(); ++i) {
         if (p >= per[i].size() || b->lane_of[i] != lane) continue;
-        const std::pair<int, int> sh{per[i][p].N1, per[i][p].N2};
+        const std::array<int, 4> sh = per[i][p].key();
         if (std::find(shapes.begin(), shapes.end(), sh) == shapes.end()) shapes.push_back(sh);
       }
       for (const auto& sh : shapes) {
-        typename BatchTables<T>::Launch L{args.size(), blk.size(), 0, sh.first, sh.second, 0, lane};
+        typename BatchTables<T>::Launch L{args.size(), blk.size(), 0, sh[1], sh[2], 0, lane};
+        L.plane = sh[0];
+        L.mode = sh[3];
         int slot = 0;
         for (size_t i = 0; i < per.size(); ++i) {
-          if (p >= per[i].size() || b->lane_of[i] != lane || per[i][p].N1 != sh.first || per[i][p].N2 != sh.second)
-            continue;
+          if (p >= per[i].size() || b->lane_of[i] != lane || per[i][p].key() != sh) continue;
           const Entry& e = per[i][p];
           args.push_back(e.args);
           for (int it = 0; it < e.items; ++it) blk.push_back((slot << kBatchItemBits) | it);
           L.blocks += e.items;
           L.smem = std::max(L.smem, e.smem);
+          L.threads = e.threads;
           ++slot;
         }
         tb.launches.push_back(L);
@@ -2662,8 +2697,29 @@ void batch_build_tables(adscf_batch* b) {
   batch_build_persist<T>(b);
 }
 
+template <typename T, int N1, int N2>
+void launch_plane_batch(hipStream_t s, const typename BatchTables<T>::Launch& L, BatchGroup<T> g) {
+  const dim3 grid(L.blocks), block(PlaneGeom<T, PassEngine<T, N1, N2>>::threads);
+  switch (L.mode) {
+    case PM_FIRST: hipLaunchKernelGGL((k_plane_pass_batch<T, N1, N2, PM_FIRST>), grid, block, L.smem, s, g); break;
+    case PM_MERGED: hipLaunchKernelGGL((k_plane_pass_batch<T, N1, N2, PM_MERGED>), grid, block, L.smem, s, g); break;
+    case PM_LAST: hipLaunchKernelGGL((k_plane_pass_batch<T, N1, N2, PM_LAST>), grid, block, L.smem, s, g); break;
+    default: throw ArgError{"plane pass of a kind the sweep does not build"};
+  }
+}
+
 template <typename T>
 void launch_batch_group(hipStream_t s, const typename BatchTables<T>::Launch& L, BatchGroup<T> g) {
+  if (L.plane) {
+#define ADSCF_PLANE(a, b, cc)                    \
+  if (L.N1 == b && L.N2 == cc) {                 \
+    launch_plane_batch<T, b, cc>(s, L, g);       \
+    return;                                      \
+  }
+    ADSCF_PLANE_SHAPES(ADSCF_PLANE)
+#undef ADSCF_PLANE
+    throw ArgError{"no batched plane kernel for this shape"};
+  }
   const dim3 grid(L.blocks), block(64);
 #define ADSCF_BATCH_SHAPE(a, c)                                                             \
   if (L.N1 == a && L.N2 == c) {                                                             \
@@ -4061,7 +4117,7 @@ int adscf_batch_create(adscf_batch** out, int ninst, const int* grid, const doub
       (void)hipStreamSynchronize(c->stream);
       (void)hipStreamDestroy(c->stream);
       c->stream = b->stream;  // every instance runs on the batch stream
-      c->plane = 0;           // the batched passes are line passes
+      c->plane = b->plane_req;
       b->inst.push_back(c);
       bcheck(b, c, adscf_set_interactions(c, kappa[i], chi + 4 * i, vol + 2 * i));
       bcheck(b, c, adscf_scf_setup(c, nblocks[i], monomer_type + boff, block_segments + boff, ensemble, rho, 0.0,
@@ -4087,6 +4143,26 @@ int adscf_batch_set_persistent(adscf_batch* b, int enable, int* active) {
     batch_drop_graph(b);
     if (active)
       *active = b->persist_req && (b->prec == ADSCF_FP64 ? persist_shapes_ok<double>(b) : persist_shapes_ok<float>(b));
+    return ADSCF_OK;
+  });
+}
+
+int adscf_batch_set_plane_passes(adscf_batch* b, int mode, int* active) {
+  return bguard(b, [&]() -> int {
+    if (mode < -1 || mode > 1) throw ArgError{"mode must be -1, 0 or 1"};
+    const bool fp64 = b->prec == ADSCF_FP64;
+    int n_active = 0;
+    for (adscf_ctx* c : b->inst) {
+      c->plane = mode;
+      n_active += fp64 ? plane_sweep<double>(c) : plane_sweep<float>(c);
+    }
+    if (mode == 1 && n_active != (int)b->inst.size()) {
+      for (adscf_ctx* c : b->inst) c->plane = b->plane_req;
+      throw ArgError{"plane passes need nx == ny, nz > 1 and a compiled plane shape on every instance"};
+    }
+    b->plane_req = mode;
+    batch_drop_graph(b);
+    if (active) *active = n_active;
     return ADSCF_OK;
   });
 }

@@ -58,8 +58,9 @@ __device__ __forceinline__ void plane_of_block(int b, int& chain, int& z) {
   z = (b >> 4) * 8 + (b & 7);
 }
 
+// one z-plane of one chain; every thread of the workgroup calls it (it has block barriers)
 template <typename T, int N1, int N2, int MODE>
-__global__ void __launch_bounds__((PlaneGeom<T, PassEngine<T, N1, N2>>::threads)) k_plane_pass(PassArgs<T> a) {
+__device__ __forceinline__ void plane_body(const PassArgs<T>& a, int chain, int z, unsigned char* smem_raw) {
   using ENG = PassEngine<T, N1, N2>;
   using G = PlaneGeom<T, ENG>;
   constexpr int n = G::n, NL = G::NL, NP = ENG::NP, STRIDE = ENG::STRIDE, TPL = ENG::TPL, LD = G::ld;
@@ -71,12 +72,8 @@ __global__ void __launch_bounds__((PlaneGeom<T, PassEngine<T, N1, N2>>::threads)
   static_assert(FIRST || MERGED || LAST, "plane passes come in the first, merged and last variants");
   // the transposes rely on the block barriers inside the y operator (engines with N1 > 1)
   static_assert(ENG::LDS > 0, "plane passes need an engine that exchanges through LDS");
-  extern __shared__ __align__(16) unsigned char smem_raw[];
   T* P = reinterpret_cast<T*>(smem_raw);
   C2<T>* scr = reinterpret_cast<C2<T>*>(smem_raw + G::plane_bytes);
-  int chain, z;
-  plane_of_block(blockIdx.x, chain, z);
-  if (z >= a.nz) return;  // uniform over the block
   const int tid = threadIdx.x;
   const int jx = tid / TPL, tx = tid % TPL;  // x layout: lines jx + r NLT
   const int jy = tid % NLT, ty = tid / NLT;  // y layout
@@ -247,6 +244,29 @@ __global__ void __launch_bounds__((PlaneGeom<T, PassEngine<T, N1, N2>>::threads)
         sto(reinterpret_cast<T*>(a.tmp_out), pair_off(px(r, m) + kRow), v[r][m].y);
       }
   }
+}
+
+template <typename T, int N1, int N2, int MODE>
+__global__ void __launch_bounds__((PlaneGeom<T, PassEngine<T, N1, N2>>::threads)) k_plane_pass(PassArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  int chain, z;
+  plane_of_block(blockIdx.x, chain, z);
+  if (z >= a.nz) return;  // uniform over the block
+  plane_body<T, N1, N2, MODE>(a, chain, z, smem_raw);
+}
+
+// Batched independent instances: block b runs item (code & mask) of the instance in slot
+// (code >> kBatchItemBits) of the launch, an item being one (chain, z-plane) as above.
+template <typename T, int N1, int N2, int MODE>
+__global__ void __launch_bounds__((PlaneGeom<T, PassEngine<T, N1, N2>>::threads))
+    k_plane_pass_batch(BatchGroup<T> g) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int code = g.blk[blockIdx.x];
+  const PassArgs<T> a = g.args[code >> kBatchItemBits];
+  int chain, z;
+  plane_of_block(code & ((1 << kBatchItemBits) - 1), chain, z);
+  if (z >= a.nz) return;  // uniform over the block
+  plane_body<T, N1, N2, MODE>(a, chain, z, smem_raw);
 }
 
 }  // namespace adscf

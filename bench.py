@@ -179,39 +179,48 @@ def converge(adscf_amd, fields, w0, max_iter):
 
 def run_batched(adscf_amd, fields, steps, warmup):
     """The 64-instance parameter sweep (32^3/48^3/64^3 grids, Ns in {100,150,200}), FP32-mixed,
-    one graph replay per SCF iteration of all instances, schedules from the batch autotuner."""
+    one graph replay per SCF iteration of all instances, schedules from the batch autotuner.
+    The default sweep (xy-plane passes alternating with z passes) is the reported one; the
+    same iteration with line passes only, as per-pass launches and as one persistent launch,
+    is timed beside it."""
     inst, w = fields.batched_instances(seed=2007)
-    with adscf_amd.Batch(inst, mixing_dim=M, lam_mix=LAM, precision=adscf_amd.FP32) as b:
-        b.set_fields(w)
-        report = b.autotune(reps=2)
+
+    def timed(b):
         b.set_fields(w)
         b.iterate(warmup, tol=-1.0)
         t0 = time.perf_counter()
         it, acc, conv = b.iterate(steps, tol=-1.0)
         wall = time.perf_counter() - t0
-        ms_iter_ev, ms_sweep = b.timing()
-        # the same iteration with the sweep as one persistent launch
+        return steps / wall, b.timing()[1], bool(conv.any())
+
+    with adscf_amd.Batch(inst, mixing_dim=M, lam_mix=LAM, precision=adscf_amd.FP32) as b:
+        b.set_fields(w)
+        report = b.autotune(reps=2)
+        nplane = b.set_plane_passes(-1)
+        ips, ms_sweep, conv = timed(b)
+        b.set_plane_passes(0)
+        l_ips, l_sweep, l_conv = timed(b)
+        lines = dict(batch_iters_per_s=l_ips, ms_per_batch_iter=1000.0 / l_ips, ms_sweep=l_sweep)
+        # the line-pass iteration with the sweep as one persistent launch
         persistent = {"active": b.set_persistent(True)}
         if persistent["active"]:
-            b.set_fields(w)
-            b.iterate(warmup, tol=-1.0)
-            t0 = time.perf_counter()
-            b.iterate(steps, tol=-1.0)
-            pwall = time.perf_counter() - t0
-            persistent.update(batch_iters_per_s=steps / pwall, ms_per_batch_iter=1000.0 * pwall / steps,
-                              ms_sweep=b.timing()[1])
+            p_ips, p_sweep, _ = timed(b)
+            persistent.update(batch_iters_per_s=p_ips, ms_per_batch_iter=1000.0 / p_ips, ms_sweep=p_sweep)
         b.set_persistent(False)
-    ips = steps / wall
+        b.set_plane_passes(-1)
     bytes_iter = sum(algorithmic_bytes_iter(d["grid"][0] ** 3, sum(d["segments"]), M, 4) for d in inst)
     sweep_bytes = sum(2 * sum(d["segments"]) * 3 * d["grid"][0] ** 3 * 4 for d in inst)
+    lines["iteration_bytes_frac"] = bytes_iter * l_ips / (HBM_PEAK_GBS * 1e9)
+    lines["sweep_roofline_frac"] = sweep_bytes / (l_sweep * 1e-3) / (HBM_PEAK_GBS * 1e9)
     return dict(instances=len(inst), batch_iters_per_s=ips, instance_iters_per_s=ips * len(inst),
-                ms_per_batch_iter=1000.0 * wall / steps,
+                ms_per_batch_iter=1000.0 / ips,
                 contour_steps_per_s=ips * sum(2 * sum(d["segments"]) for d in inst),
                 iteration_bytes_frac=bytes_iter * ips / (HBM_PEAK_GBS * 1e9),
                 sweep_roofline_frac=sweep_bytes / (ms_sweep * 1e-3) / (HBM_PEAK_GBS * 1e9),
-                timing_valid=not bool(conv.any()), ms_sweep=ms_sweep,
-                sweep_mode="one launch per pass and register shape (faster than the persistent launch below)",
-                persistent_sweep=persistent,
+                timing_valid=not (conv or l_conv), ms_sweep=ms_sweep,
+                sweep_mode="xy-plane passes alternating with z passes on %d of %d instances, one launch per "
+                           "pass, kernel and lane" % (nplane, len(inst)),
+                line_passes=lines, persistent_sweep=persistent,
                 schedules=[l for l in report.splitlines() if "chosen" in l])
 
 

@@ -335,6 +335,11 @@ int adscf_batch_set_precision(adscf_batch* batch, int precision);
  * the shapes 4x8, 8x4, 8x6, 8x8 and cube4 (axis lengths 32, 48 and 64); active receives 1 when
  * the batch's schedules allow it, else the per-pass launches stay.  Results are identical. */
 int adscf_batch_set_persistent(adscf_batch* batch, int enable, int* active);
+/* xy-plane passes of the batched sweep (see adscf_set_plane_passes): -1 (the default) on every
+ * instance whose grid allows them, 0 line passes only, 1 required on every instance
+ * (ADSCF_ERR_ARG otherwise).  active (may be NULL) receives the number of instances whose
+ * sweep uses them.  The persistent sweep carries line passes only. */
+int adscf_batch_set_plane_passes(adscf_batch* batch, int mode, int* active);
 /* per-instance outputs (arrays of ninst); stops when every instance has converged */
 int adscf_batch_iterate(adscf_batch* batch, int niter, double tol, int* iters, double* accuracy, int* converged);
 int adscf_batch_energy(adscf_batch* batch, double* F_per_chain);

@@ -1,5 +1,6 @@
 """Batched instances (adscf_batch_*): every instance of a batch reproduces the
-single-instance device loop with the same line-FFT schedule bit for bit in FP64,
+single-instance device loop with the same line-FFT schedule bit for bit, with line passes
+only and with the default xy-plane passes (on the 32^3 and 48^3 instances),
 the batched sweep tolerates uneven grids and contour lengths, and the batch
 autotuner keeps results unchanged."""
 import numpy as np
@@ -21,14 +22,17 @@ def _instances():
     return inst, w
 
 
+@pytest.mark.parametrize("plane", [0, -1])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_batch_matches_single_instances(prec):
+def test_batch_matches_single_instances(prec, plane):
     import adscf_amd
 
     P = adscf_amd.FP64 if prec == "fp64" else adscf_amd.FP32
     inst, w = _instances()
     iters = 12
+    print(f"batch vs single {prec} plane mode {plane}", flush=True)
     with adscf_amd.Batch(inst, mixing_dim=4, lam_mix=0.2, precision=P) as b:
+        assert b.set_plane_passes(plane) == (0 if plane == 0 else 3)  # 16^3 has no plane kernel
         b.set_fields(w)
         it, acc, conv = b.iterate(iters, tol=0.0)
         assert np.all(it == iters) and not conv.any()
@@ -36,7 +40,7 @@ def test_batch_matches_single_instances(prec):
         fb = b.energy()
     for i, d in enumerate(inst):
         with adscf_amd.Context(*d["grid"], *d["box"], d["ds"], precision=P) as ctx:
-            ctx.set_plane_passes(0)  # batched sweeps run line passes
+            ctx.set_plane_passes(plane)
             for a in range(3):
                 n = d["grid"][a]
                 sch = {16: (1, 16), 32: (4, 8), 48: (8, 6)}[n]
@@ -85,6 +89,8 @@ def test_batch_continues_after_convergence(persistent):
     runs = {}
     for mode in ("two calls", "one call"):
         with adscf_amd.Batch(inst, mixing_dim=4, lam_mix=0.2, precision=adscf_amd.FP64) as b:
+            if persistent:
+                b.set_plane_passes(0)  # the persistent sweep carries line passes
             assert b.set_persistent(persistent) == persistent
             b.set_fields(w)
             if mode == "two calls":

@@ -10,9 +10,10 @@ General_Routines/mod_iterate.f90:92-150), all run by one batched graph replay pe
   Find_SCF_Saddle_Point iteration from the same field (FP64 to 1e-10 relative; FP32-mixed
   to the stated bounds: densities 5e-5, mixed field 1e-7).
 * 23 FP64 iterations (three Anderson solves past m = 20) of one instance per grid size and
-  contour length, each bit for bit equal to a single-instance context on the same line-pass
-  schedules: the batched passes, Gram sums, wavefront LU and mix are the single-instance
-  ones; with the sweep as per-pass launches and as one persistent launch.
+  contour length, each bit for bit equal to a single-instance context on the same schedules:
+  the batched passes, Gram sums, wavefront LU and mix are the single-instance ones; with the
+  sweep as per-pass launches of xy-plane and z passes (the default), as per-pass line passes
+  and as one persistent launch (line passes).
 """
 import numpy as np
 import pytest
@@ -77,8 +78,8 @@ def test_sweep_one_iteration_against_oracle(orc, sweep, prec):
     assert np.all(np.isfinite(F))
 
 
-@pytest.mark.parametrize("persistent", [False, True])
-def test_sweep_instances_match_single_contexts_past_m(sweep, persistent):
+@pytest.mark.parametrize("persistent,plane", [(False, -1), (False, 0), (True, 0)])
+def test_sweep_instances_match_single_contexts_past_m(sweep, persistent, plane):
     import adscf_amd
 
     inst, w = sweep
@@ -92,8 +93,10 @@ def test_sweep_instances_match_single_contexts_past_m(sweep, persistent):
     sub = [inst[i] for i in pick]
     ws = [w[i] for i in pick]
     iters = M + 3
-    print(f"instances {pick} ({sorted(seen)}), {iters} FP64 iterations, persistent {persistent}", flush=True)
+    print(f"instances {pick} ({sorted(seen)}), {iters} FP64 iterations, persistent {persistent}, plane {plane}",
+          flush=True)
     with adscf_amd.Batch(sub, mixing_dim=M, lam_mix=LAM, precision=adscf_amd.FP64) as b:
+        assert b.set_plane_passes(plane) == (0 if plane == 0 else len(sub))
         assert b.set_persistent(persistent) == persistent
         b.set_fields(ws)
         it, acc, conv = b.iterate(iters, tol=-1.0)
@@ -101,9 +104,10 @@ def test_sweep_instances_match_single_contexts_past_m(sweep, persistent):
     assert np.all(it == iters) and not conv.any()
     for k, d in enumerate(sub):
         # a context's default line schedules (FP64: cube4 at 64, 4x8 at 32, 8x6 at 48) are the
-        # one-wave register schedules a batch instance keeps; batched sweeps run line passes
+        # one-wave register schedules a batch instance keeps, and its default plane shapes the
+        # batch's
         with adscf_amd.Context(*d["grid"], *d["box"], d["ds"], precision=adscf_amd.FP64) as ctx:
-            ctx.set_plane_passes(0)
+            ctx.set_plane_passes(plane)
             ctx.set_interactions(d["kappa"], d["chi"], d["vol"])
             ctx.scf_setup(d["blocks"], d["segments"], mixing_dim=M, lam_mix=LAM)
             ctx.scf_set_field(ws[k])
@@ -124,6 +128,7 @@ def test_sweep_persistent_fp32_matches_per_pass(sweep):
     out = {}
     for persistent in (False, True):
         with adscf_amd.Batch(inst, mixing_dim=M, lam_mix=LAM, precision=adscf_amd.FP32) as b:
+            b.set_plane_passes(0)  # the persistent sweep carries line passes
             assert b.set_persistent(persistent) == persistent
             b.set_fields(w)
             it, acc, conv = b.iterate(5, tol=-1.0)
