@@ -748,8 +748,8 @@ bool plane_sweep(const adscf_ctx* c) {
 
 // Default schedule for an axis of n points: the first registered shape, one wave
 // per block; at n = 64 and 128 the choice the autotuner makes on the MI355X for the
-// SCF sweep (n = 64: the cube engine for FP64 on every axis and for FP32 on x, 8 x 8
-// for FP32 on the strided y and z axes; n = 128: 16 x 8).
+// SCF sweep (n = 64: the cube engine on every axis, FP32 y and z with two waves per
+// workgroup; n = 128: 16 x 8 with the exceptions below).
 void default_schedule(AxisTables& ax, int axis, int prec) {
   ax.N1 = ax.N2 = 0;
   ax.waves = 1;
@@ -760,9 +760,10 @@ void default_schedule(AxisTables& ax, int axis, int prec) {
       ax.N2 = s.N2;
       break;
     }
-  if (ax.n == 64 && (prec == ADSCF_FP64 || axis == 0)) {
+  if (ax.n == 64) {  // the tuner's choice at 64^3 x 200: the cube engine, FP32 y/z with 2 waves
     ax.N1 = 4;
     ax.N2 = 0;
+    if (prec == ADSCF_FP32 && axis != 0) ax.waves = 2;
   }
   if (ax.n == 128) {  // the tuner's choice at 128^3 x 400 on the MI355X
     ax.N1 = 16;
