@@ -14,6 +14,11 @@
 
 namespace adscf {
 
+// Twiddle multiplies are unconditional in the lane index: the twiddle of lane 0 is
+// W^0 = (1, -0), and z (1, -0) = z exactly (z.x 1 - z.y (-0) = z.x, z.x (-0) + z.y 1 = z.y, up
+// to the sign of an exact zero), so skipping it for lane 0 would only add a per-lane branch,
+// which serialises the phase's LDS reads behind it.
+
 template <int R>
 struct Cube {
   static constexpr int n = R * R * R;
@@ -46,7 +51,7 @@ ADSCF_HD void cube_a(C2<T>* v, C2<T>* lds, int t, const CubeTables<T, R>& W) {
   constexpr int sp = Cube<R>::sp;
   dft_reg<T, R, -1>(v);
 #pragma unroll
-  for (int q = 0; q < R; ++q) lds[t * sp + q] = (q > 0 && t > 0) ? cmul(v[q], W.wa[q]) : v[q];
+  for (int q = 0; q < R; ++q) lds[t * sp + q] = q > 0 ? cmul(v[q], W.wa[q]) : v[q];
 }
 template <typename T, int R>
 ADSCF_HD void cube_b(C2<T>* lds, int t, const CubeTables<T, R>& W) {
@@ -58,7 +63,7 @@ ADSCF_HD void cube_b(C2<T>* lds, int t, const CubeTables<T, R>& W) {
   for (int b = 0; b < R; ++b) u[b] = lds[(a + R * b) * sp + k2];
   dft_reg<T, R, -1>(u);
 #pragma unroll
-  for (int c = 0; c < R; ++c) S2[(k2 * R + c) * sp + a] = (c > 0 && a > 0) ? cmul(u[c], W.wb[c]) : u[c];
+  for (int c = 0; c < R; ++c) S2[(k2 * R + c) * sp + a] = c > 0 ? cmul(u[c], W.wb[c]) : u[c];
 }
 template <typename T, int R>
 ADSCF_HD void cube_c(C2<T>* lds, int t, const CubeTables<T, R>& W) {
@@ -83,7 +88,7 @@ ADSCF_HD void cube_bi(C2<T>* lds, int t, const CubeTables<T, R>& W) {
 #pragma unroll
   for (int c = 0; c < R; ++c) {
     const C2<T> z = S2[(k2 * R + c) * sp + a];
-    u[c] = (c > 0 && a > 0) ? cmulc(z, W.wb[c]) : z;
+    u[c] = c > 0 ? cmulc(z, W.wb[c]) : z;
   }
   dft_reg<T, R, +1>(u);
 #pragma unroll
@@ -95,7 +100,7 @@ ADSCF_HD void cube_ai(C2<T>* v, const C2<T>* lds, int t, const CubeTables<T, R>&
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const C2<T> z = lds[t * sp + q];
-    v[q] = (q > 0 && t > 0) ? cmulc(z, W.wa[q]) : z;
+    v[q] = q > 0 ? cmulc(z, W.wa[q]) : z;
   }
   dft_reg<T, R, +1>(v);
 }
@@ -150,7 +155,7 @@ ADSCF_HD void tri_a(C2<T>* v, C2<T>* lds, int t, const TriTables<T, A, R>& W) {
   constexpr int sa = Tri<A, R>::sa;
   dft_reg<T, A, -1>(v);
 #pragma unroll
-  for (int q = 0; q < A; ++q) lds[t * sa + q] = (q > 0 && t > 0) ? cmul(v[q], W.wa[q]) : v[q];
+  for (int q = 0; q < A; ++q) lds[t * sa + q] = q > 0 ? cmul(v[q], W.wa[q]) : v[q];
 }
 template <typename T, int A, int R>
 ADSCF_HD void tri_b(C2<T>* lds, int t, const TriTables<T, A, R>& W) {
@@ -165,7 +170,7 @@ ADSCF_HD void tri_b(C2<T>* lds, int t, const TriTables<T, A, R>& W) {
     for (int b = 0; b < R; ++b) u[b] = lds[(a + R * b) * sa + k2];
     dft_reg<T, R, -1>(u);
 #pragma unroll
-    for (int c = 0; c < R; ++c) S2[(k2 * R + c) * sb + a] = (c > 0 && a > 0) ? cmul(u[c], W.wb[i][c]) : u[c];
+    for (int c = 0; c < R; ++c) S2[(k2 * R + c) * sb + a] = c > 0 ? cmul(u[c], W.wb[i][c]) : u[c];
   }
 }
 template <typename T, int A, int R>
@@ -197,7 +202,7 @@ ADSCF_HD void tri_bi(C2<T>* lds, int t, const TriTables<T, A, R>& W) {
 #pragma unroll
     for (int c = 0; c < R; ++c) {
       const C2<T> z = S2[(k2 * R + c) * sb + a];
-      u[c] = (c > 0 && a > 0) ? cmulc(z, W.wb[i][c]) : z;
+      u[c] = c > 0 ? cmulc(z, W.wb[i][c]) : z;
     }
     dft_reg<T, R, +1>(u);
 #pragma unroll
@@ -210,7 +215,7 @@ ADSCF_HD void tri_ai(C2<T>* v, const C2<T>* lds, int t, const TriTables<T, A, R>
 #pragma unroll
   for (int q = 0; q < A; ++q) {
     const C2<T> z = lds[t * sa + q];
-    v[q] = (q > 0 && t > 0) ? cmulc(z, W.wa[q]) : z;
+    v[q] = q > 0 ? cmulc(z, W.wa[q]) : z;
   }
   dft_reg<T, A, +1>(v);
 }

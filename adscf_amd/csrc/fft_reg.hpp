@@ -154,7 +154,7 @@ ADSCF_HD void fs_phase_a(C2<T>* v, C2<T>* lds, int t, const RegTables<T, N1, N2>
 #pragma unroll
   for (int k2 = 0; k2 < N2; ++k2) {
     C2<T> y = v[k2];
-    if (k2 > 0 && t > 0) y = cmul(y, R.ta[k2]);
+    if (k2 > 0) y = cmul(y, R.ta[k2]);  // lane 0: W^0 = (1, -0), an exact multiply
     lds[t + FS::n1p * k2] = y;
   }
 }
@@ -173,7 +173,7 @@ ADSCF_HD void fs_phase_b(C2<T>* lds, int t, const RegTables<T, N1, N2>& R) {
 #pragma unroll
   for (int j1 = 0; j1 < N1; ++j1) {
     C2<T> z = u[j1];
-    if (j1 > 0 && t > 0) z = cmulc(z, R.tb[j1]);
+    if (j1 > 0) z = cmulc(z, R.tb[j1]);  // lane 0: W^0, exact (no per-lane branch)
     lds[j1 + FS::n1p * t] = z;
   }
 }
