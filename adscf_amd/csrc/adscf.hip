@@ -2483,7 +2483,8 @@ bool persist_shapes_ok(adscf_batch* b) {
       PassArgs<T> p = base_pass<T>(c, a);
       int threads = 0;
       prep_reg<T>(c, p, ax.N1, ax.N2, 4, &threads, nullptr);
-      if (threads > kPersistThreads || (ax.N2 <= 0 && threads != kPersistThreads)) return false;
+      // every thread of the workgroup must sit on a line (the engines take t < threads per line)
+      if (threads != kPersistThreads) return false;
     }
   }
   return true;
@@ -2516,9 +2517,7 @@ void batch_build_persist(adscf_batch* b) {
       int threads = 0;
       size_t sm = 0;
       const int tiles = prep_reg<T>(c, p, ax.N1, ax.N2, 4, &threads, &sm);
-      // the cube engine needs every thread of the workgroup on a line
-      if (ax.N2 <= 0 && threads != kPersistThreads) return;
-      if (threads > kPersistThreads) return;
+      if (threads != kPersistThreads) return;  // every thread on a line, as the engines assume
       smem = std::max(smem, sm);
       args.push_back(p);
       shape.push_back(id);
@@ -2620,8 +2619,11 @@ void batch_build_tables(adscf_batch* b) {
       }
       const AxisTables& ax = c->ax[p.axis];
       Entry e{p, 0, ax.N1, ax.N2, 0, 0, PM_GENERIC, 64};
-      e.items = prep_reg<T>(c, e.args, ax.N1, ax.N2, 1, nullptr, &e.smem);
+      int threads = 0;
+      e.items = prep_reg<T>(c, e.args, ax.N1, ax.N2, 1, &threads, &e.smem);
       if (e.items >= (1 << kBatchItemBits)) throw ArgError{"batched pass has too many blocks per instance"};
+      // the batched kernel runs 64 threads, and the engines assume each of them on a line
+      if (ax.N1 > 1 && threads != 64) throw ArgError{"batched pass shape does not fill a wavefront"};
       es.push_back(e);
     }
     npass = std::max(npass, es.size());

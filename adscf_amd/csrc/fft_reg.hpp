@@ -121,7 +121,9 @@ struct FourStep {
 };
 
 // Per-thread twiddles and multipliers of one line operator, loaded once per
-// pass so that their memory latency overlaps the loads of the data.
+// pass so that their memory latency overlaps the loads of the data.  Callers run
+// exactly max(N1, N2) threads per line (t < max(N1, N2)), so a guard against the
+// larger of the two is always true.
 //   ta[k2] = W_n^(t*k2)       phase A, thread t < N1
 //   tb[j1] = W_n^(j1*t)       phase B, thread t < N2
 //   k[k1]  = K[t + N2*k1]     phase B; with N1 == 1, k[k] = K[k] for the whole line
@@ -137,11 +139,11 @@ struct RegTables {
       for (int i = 0; i < N2; ++i) k[i] = kf[i];
     } else {
 #pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) ta[k2] = t < N1 ? tw[t * k2] : C2<T>{T(1), T(0)};
+      for (int k2 = 0; k2 < N2; ++k2) ta[k2] = (N1 >= N2 || t < N1) ? tw[t * k2] : C2<T>{T(1), T(0)};
 #pragma unroll
-      for (int j1 = 0; j1 < N1; ++j1) tb[j1] = t < N2 ? tw[j1 * t] : C2<T>{T(1), T(0)};
+      for (int j1 = 0; j1 < N1; ++j1) tb[j1] = (N2 >= N1 || t < N2) ? tw[j1 * t] : C2<T>{T(1), T(0)};
 #pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) k[k1] = t < N2 ? kf[t + N2 * k1] : T(0);
+      for (int k1 = 0; k1 < N1; ++k1) k[k1] = (N2 >= N1 || t < N2) ? kf[t + N2 * k1] : T(0);
     }
   }
 };

@@ -305,11 +305,14 @@ __device__ __forceinline__ void reg_line_op(C2<T> (&v)[N2], C2<T>* lds, int t, c
   if constexpr (N1 == 1) {
     line_op_single<T, N2>(v, R);
   } else {
-    if (t < N1) fs_phase_a<T, N1, N2>(v, lds, t, R);
+    // every caller runs exactly max(N1, N2) threads per line (t < TPL), so the guard of the
+    // phase whose thread count is TPL is always true and is resolved at compile time
+    constexpr bool allA = N1 >= N2, allB = N2 >= N1;
+    if (allA || t < N1) fs_phase_a<T, N1, N2>(v, lds, t, R);
     line_sync(wl);
-    if (t < N2) fs_phase_b<T, N1, N2>(lds, t, R);
+    if (allB || t < N2) fs_phase_b<T, N1, N2>(lds, t, R);
     line_sync(wl);
-    if (t < N1) fs_phase_c<T, N1, N2>(v, lds, t);
+    if (allA || t < N1) fs_phase_c<T, N1, N2>(v, lds, t);
   }
 }
 
