@@ -26,11 +26,14 @@ def _ctx(amd, g, types, segs, rho, m=2):
     return ctx
 
 
+# (24, 6, 4): line passes; (32, 32, 4): square xy planes, so the sweeps run xy-plane passes
+@pytest.mark.parametrize("dims", [(24, 6, 4, (6.0, 1.5, 1.0)), (32, 32, 4, (6.0, 6.0, 1.0))])
 @pytest.mark.parametrize("types,segs", [([1, 2], [10, 10]), ([1, 2, 1], [10, 14, 6])])
-def test_blend_densities_match_oracle(orc, types, segs):
+def test_blend_densities_match_oracle(orc, types, segs, dims):
     import adscf_amd as amd
 
-    g = _grid(orc, 24, 6, 4, (6.0, 1.5, 1.0))
+    g = _grid(orc, *dims)
+    print(f"blend densities dims={dims[:3]} types={types}", flush=True)
     vol, dvol = g.volume()
     rho = [20.0 / vol * (i + 1) for i in range(len(types))]
     field = orc.MT(2007).grnd(2 * g.ng).reshape(2, g.nz, g.ny, g.nx).copy()
