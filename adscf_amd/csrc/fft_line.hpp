@@ -16,8 +16,10 @@
 
 namespace adscf {
 
+// Aligned to its own size, so that one complex value moves with one 8-byte (FP32) or 16-byte
+// (FP64) LDS or global access whose offset folds into the instruction.
 template <typename T>
-struct C2 {
+struct alignas(2 * sizeof(T)) C2 {
   T x, y;
 };
 

@@ -173,7 +173,7 @@ __host__ __device__ constexpr size_t line_pass_lds_bytes(int L, int ldn, int n) 
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_line_pass(PassArgs<T> a) {
   if (a.done && *a.done) return;
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   C2<T>* b0 = reinterpret_cast<C2<T>*>(smem_raw);
   C2<T>* b1 = b0 + a.L * a.ldn;
   C2<T>* twl = b1 + a.L * a.ldn;  // the axis twiddles: every butterfly reads them
@@ -761,7 +761,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 
 template <typename T, int N1, int N2, int MODE>
 __global__ void __launch_bounds__(kThreads) k_line_pass_reg(PassArgs<T> a) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   if constexpr (MODE != PM_GENERIC) pin_pass_args(a);
   reg_pass_body<T, PassEngine<T, N1, N2>, MODE>(a, xcd_tile(blockIdx.x, gridDim.x), smem_raw);
 }
@@ -769,7 +769,7 @@ __global__ void __launch_bounds__(kThreads) k_line_pass_reg(PassArgs<T> a) {
 // K tiles per workgroup (reg_pass_tiles); nitems is the tile count of the pass
 template <typename T, int N1, int N2, int MODE, int K>
 __global__ void __launch_bounds__(kThreads) k_line_pass_tiles(PassArgs<T> a, int nitems) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   pin_pass_args(a);
   reg_pass_tiles<T, PassEngine<T, N1, N2>, MODE, K>(a, nitems, smem_raw);
 }
@@ -794,7 +794,7 @@ __global__ void __launch_bounds__(64) k_line_pass_batch(BatchGroup<T> g) {
   const int i = code >> kBatchItemBits;
   const int item = code & ((1 << kBatchItemBits) - 1);
   const PassArgs<T> a = g.args[i];
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   reg_pass_body<T, PassEngine<T, N1, N2>>(a, item, smem_raw);
 }
 
@@ -846,7 +846,7 @@ struct PersistShapes<T, A, B, R...> {
 
 template <typename T, int... S>
 __global__ void __launch_bounds__(kPersistThreads) k_batch_persist(PersistArgs<T> pa) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int s_fail;
   for (int k = blockIdx.x; k < pa.nitems; k += gridDim.x) {
     const PersistItem it = pa.items[k];
@@ -913,7 +913,7 @@ struct FftArgs {
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_fft_axis(FftArgs<T> a) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   C2<T>* b0 = reinterpret_cast<C2<T>*>(smem_raw);
   C2<T>* b1 = b0 + a.L * a.ldn;
   C2<T>* twl = b1 + a.L * a.ldn;
