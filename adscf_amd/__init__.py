@@ -116,6 +116,7 @@ SIGNATURES = {
     "adscf_batch_set_precision": (C.c_int, [_ctxp, C.c_int]),
     "adscf_batch_set_persistent": (C.c_int, [_ctxp, C.c_int, _ip]),
     "adscf_batch_set_plane_passes": (C.c_int, [_ctxp, C.c_int, _ip]),
+    "adscf_batch_set_schedule": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int]),
     "adscf_batch_iterate": (C.c_int, [_ctxp, C.c_int, C.c_double, _ip, _dp, _ip]),
     "adscf_batch_energy": (C.c_int, [_ctxp, _dp]),
     "adscf_batch_get": (C.c_int, [_ctxp, C.c_int, _dp, _dp]),
@@ -693,6 +694,11 @@ class Batch:
         act = np.zeros(1, dtype=np.int32)
         self._chk(self._lib.adscf_batch_set_plane_passes(self._h, int(mode), _pi(act)))
         return int(act[0])
+
+    def set_schedule(self, n, n1, n2):
+        """Register shape n1 x n2 (one wavefront per workgroup) of every instance axis of
+        length n."""
+        self._chk(self._lib.adscf_batch_set_schedule(self._h, int(n), int(n1), int(n2)))
 
     def iterate(self, niter, tol=0.0):
         it = np.zeros(self.n, dtype=np.int32)

@@ -2468,7 +2468,7 @@ int persist_shape_id(int N1, int N2) {
 // the persistent sweep kernel of precision T over kPersistShapes
 template <typename T>
 constexpr auto persist_kernel() {
-  return &k_batch_persist<T, 4, 8, 8, 4, 8, 6, 8, 8, 4, 0>;
+  return &k_batch_persist<T, 4, 8, 8, 4, 8, 6, 6, 8, 8, 8, 4, 0>;
 }
 
 // whether every pass of every instance has a shape the persistent kernel carries and a tile
@@ -4166,6 +4166,26 @@ int adscf_batch_set_plane_passes(adscf_batch* b, int mode, int* active) {
     b->plane_req = mode;
     batch_drop_graph(b);
     if (active) *active = n_active;
+    return ADSCF_OK;
+  });
+}
+
+int adscf_batch_set_schedule(adscf_batch* b, int n, int n1, int n2) {
+  return bguard(b, [&]() -> int {
+    bool known = false;
+    for (const RegShape& s : kRegShapes) known = known || (s.n == n && s.N1 == n1 && s.N2 == n2);
+    if (!known || !batch_shape_ok(n1, n2)) throw ArgError{"no batched register shape n1 x n2 of this length"};
+    int hit = 0;
+    for (adscf_ctx* c : b->inst)
+      for (int a = 0; a < 3; ++a)
+        if (c->ax[a].n == n) {
+          c->ax[a].N1 = n1;
+          c->ax[a].N2 = n2;
+          c->ax[a].waves = 1;
+          ++hit;
+        }
+    if (!hit) throw ArgError{"no instance has an axis of this length"};
+    batch_drop_graph(b);
     return ADSCF_OK;
   });
 }

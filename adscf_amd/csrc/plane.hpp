@@ -188,8 +188,8 @@ __device__ __forceinline__ void plane_body(const PassArgs<T>& a, int chain, int 
       for (int r = 0; r < LPT; ++r)
 #pragma unroll
         for (int m = 0; m < NP; ++m) {
-          sto(h, py(r, m) * B, v[r][m].x * e0[r][m] * sc);
-          sto(h, (py(r, m) + kCol) * B, v[r][m].y * e1[r][m] * sc);
+          sto_nt(h, py(r, m) * B, v[r][m].x * e0[r][m] * sc);
+          sto_nt(h, (py(r, m) + kCol) * B, v[r][m].y * e1[r][m] * sc);
         }
     }
     return;
@@ -203,8 +203,8 @@ __device__ __forceinline__ void plane_body(const PassArgs<T>& a, int chain, int 
         C2<T> x = v[r][m];
         x.x = x.x * e0[r][m] * sc;
         x.y = x.y * e1[r][m] * sc;
-        sto(h, py(r, m) * B, x.x);
-        sto(h, (py(r, m) + kCol) * B, x.y);
+        sto_nt(h, py(r, m) * B, x.x);
+        sto_nt(h, (py(r, m) + kCol) * B, x.y);
         x.x *= f0[r][m];
         x.y *= f1[r][m];
         v[r][m] = x;

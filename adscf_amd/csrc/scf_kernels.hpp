@@ -306,7 +306,16 @@ __global__ void __launch_bounds__(kThreads) k_contour(ContourArgs<T> a) {
   const double ds = a.ds, hds = 0.5 * a.ds;
   const double norm = a.st->norm;
   double rmax = 0.0, dsum = 0.0;
-  auto ld = [&](const T* base, int s, long i0) { return *reinterpret_cast<const P*>(base + (long)s * ng + i0); };
+  // nontemporal loads: each slice is read once, as a stream (the slices of a 64^3 FP32
+  // sweep, ~400 MB, exceed what the caches could keep)
+  typedef T vecT __attribute__((ext_vector_type(VP)));
+  auto ld = [&](const T* base, int s, long i0) {
+    const vecT x = __builtin_nontemporal_load(reinterpret_cast<const vecT*>(base + (long)s * ng + i0));
+    P r;
+#pragma unroll
+    for (int v = 0; v < VP; ++v) r.v[v] = x[v];
+    return r;
+  };
   auto scl = [&](int s) { return ldexp(1.0, -(a.e1[s] + a.e2[ns - s])); };
   const long npk = ng / VP;
   for (long ip = blockIdx.x * (long)blockDim.x + threadIdx.x; ip < npk; ip += (long)gridDim.x * blockDim.x) {

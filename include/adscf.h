@@ -332,7 +332,7 @@ int adscf_batch_set_precision(adscf_batch* batch, int precision);
  * persistent launch (workgroups take tiles pass by pass and wait only for the previous pass of
  * their own instance; bounded waits, a wait that expires makes adscf_batch_iterate fail);
  * 0 (the default) runs one launch per pass and register shape.  The persistent sweep carries
- * the shapes 4x8, 8x4, 8x6, 8x8 and cube4 (axis lengths 32, 48 and 64); active receives 1 when
+ * the shapes 4x8, 8x4, 8x6, 6x8, 8x8 and cube4 (axis lengths 32, 48 and 64); active receives 1 when
  * the batch's schedules allow it, else the per-pass launches stay.  Results are identical. */
 int adscf_batch_set_persistent(adscf_batch* batch, int enable, int* active);
 /* xy-plane passes of the batched sweep (see adscf_set_plane_passes): -1 (the default) on every
@@ -340,6 +340,10 @@ int adscf_batch_set_persistent(adscf_batch* batch, int enable, int* active);
  * (ADSCF_ERR_ARG otherwise).  active (may be NULL) receives the number of instances whose
  * sweep uses them.  The persistent sweep carries line passes only. */
 int adscf_batch_set_plane_passes(adscf_batch* batch, int mode, int* active);
+/* register shape (as in adscf_set_schedule, one wavefront per workgroup) of every instance axis
+ * of length n, in place of the default or adscf_batch_autotune's choice; ADSCF_ERR_ARG when
+ * the batched kernels have no such shape or no instance has an axis of that length. */
+int adscf_batch_set_schedule(adscf_batch* batch, int n, int n1, int n2);
 /* per-instance outputs (arrays of ninst); stops when every instance has converged */
 int adscf_batch_iterate(adscf_batch* batch, int niter, double tol, int* iters, double* accuracy, int* converged);
 int adscf_batch_energy(adscf_batch* batch, double* F_per_chain);

@@ -14,6 +14,9 @@ General_Routines/mod_iterate.f90:92-150), all run by one batched graph replay pe
   the batched passes, Gram sums, wavefront LU and mix are the single-instance ones; with the
   sweep as per-pass launches of xy-plane and z passes (the default), as per-pass line passes
   and as one persistent launch (line passes).
+* The persistent sweep with the 48-point axes pinned (adscf_batch_set_schedule) to each shape
+  it carries for that length, against the per-pass launches bit for bit; the refusal of
+  shapes the batched kernels lack.
 """
 import numpy as np
 import pytest
@@ -136,3 +139,37 @@ def test_sweep_persistent_fp32_matches_per_pass(sweep):
     assert np.array_equal(out[False][0], out[True][0])
     for a, c in zip(out[False][1], out[True][1]):
         assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
+
+
+def test_sweep_persistent_pinned_shapes_match_per_pass(sweep):
+    """The persistent sweep with the 48-point axes pinned to each shape the persistent kernel
+    carries for that length (8x6 and 6x8): the same fields, densities and residuals as the
+    per-pass launches, bit for bit (FP32-mixed, all 64 instances, 3 iterations)."""
+    import adscf_amd
+
+    inst, w = sweep
+    for shape in ((8, 6), (6, 8)):
+        print(f"48-point axes pinned to {shape}", flush=True)
+        out = {}
+        for persistent in (False, True):
+            with adscf_amd.Batch(inst, mixing_dim=M, lam_mix=LAM, precision=adscf_amd.FP32) as b:
+                b.set_plane_passes(0)
+                b.set_schedule(48, *shape)
+                assert b.set_persistent(persistent) == persistent
+                b.set_fields(w)
+                it, acc, conv = b.iterate(3, tol=-1.0)
+                out[persistent] = (acc, [b.get(i) for i in range(len(inst))])
+        assert np.array_equal(out[False][0], out[True][0])
+        for a, c in zip(out[False][1], out[True][1]):
+            assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
+
+
+def test_batch_schedule_refuses_unknown_shapes(sweep):
+    import adscf_amd
+
+    inst, _ = sweep
+    with adscf_amd.Batch(inst[:2], mixing_dim=M, lam_mix=LAM, precision=adscf_amd.FP32) as b:
+        with pytest.raises(adscf_amd.AdscfError):
+            b.set_schedule(48, 4, 8)  # 4 x 8 is 32 points
+        with pytest.raises(adscf_amd.AdscfError):
+            b.set_schedule(40, 8, 5)  # no batched 8 x 5 kernel
