@@ -2618,12 +2618,13 @@ void batch_build_tables(adscf_batch* b) {
         continue;
       }
       const AxisTables& ax = c->ax[p.axis];
-      Entry e{p, 0, ax.N1, ax.N2, 0, 0, PM_GENERIC, 64};
+      Entry e{p, 0, ax.N1, ax.N2, 0, 0, PM_GENERIC, 64 * kBatchLineWaves};
       int threads = 0;
-      e.items = prep_reg<T>(c, e.args, ax.N1, ax.N2, 1, &threads, &e.smem);
+      e.items = prep_reg<T>(c, e.args, ax.N1, ax.N2, kBatchLineWaves, &threads, &e.smem);
       if (e.items >= (1 << kBatchItemBits)) throw ArgError{"batched pass has too many blocks per instance"};
-      // the batched kernel runs 64 threads, and the engines assume each of them on a line
-      if (ax.N1 > 1 && threads != 64) throw ArgError{"batched pass shape does not fill a wavefront"};
+      // the batched kernel runs 64 kBatchLineWaves threads, and the engines assume each of
+      // them on a line
+      if (ax.N1 > 1 && threads != 64 * kBatchLineWaves) throw ArgError{"batched pass shape does not fill its wavefronts"};
       es.push_back(e);
     }
     npass = std::max(npass, es.size());
@@ -2679,16 +2680,22 @@ void batch_build_tables(adscf_batch* b) {
         L.plane = sh[0];
         L.mode = sh[3];
         int slot = 0;
+        std::vector<int> codes;
         for (size_t i = 0; i < per.size(); ++i) {
           if (p >= per[i].size() || b->lane_of[i] != lane || per[i][p].key() != sh) continue;
           const Entry& e = per[i][p];
           args.push_back(e.args);
-          for (int it = 0; it < e.items; ++it) blk.push_back((slot << kBatchItemBits) | it);
+          for (int it = 0; it < e.items; ++it) codes.push_back((slot << kBatchItemBits) | it);
           L.blocks += e.items;
           L.smem = std::max(L.smem, e.smem);
           L.threads = e.threads;
           ++slot;
         }
+        // line passes: block b runs on XCD b % 8, and XCD k takes a contiguous range of the
+        // tiles (xcd_tile), so neighbouring tiles that share cache lines meet in one L2.
+        // Plane passes keep the order their kernel's own mapping assumes.
+        for (int bk = 0; bk < (int)codes.size(); ++bk)
+          blk.push_back(codes[L.plane ? bk : xcd_tile(bk, (int)codes.size())]);
         tb.launches.push_back(L);
       }
     }
@@ -2723,7 +2730,7 @@ void launch_batch_group(hipStream_t s, const typename BatchTables<T>::Launch& L,
 #undef ADSCF_PLANE
     throw ArgError{"no batched plane kernel for this shape"};
   }
-  const dim3 grid(L.blocks), block(64);
+  const dim3 grid(L.blocks), block(64 * kBatchLineWaves);
 #define ADSCF_BATCH_SHAPE(a, c)                                                             \
   if (L.N1 == a && L.N2 == c) {                                                             \
     hipLaunchKernelGGL((k_line_pass_batch<T, a, c>), grid, block, L.smem, s, g);             \

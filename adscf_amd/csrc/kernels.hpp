@@ -771,7 +771,7 @@ __device__ __forceinline__ void pin_pass_args(const PassArgs<T>& a) {
 // pass is found in that XCD's L2 by the next.
 constexpr int kXcds = 8;
 
-__device__ __forceinline__ int xcd_tile(int b, int nb) {
+__host__ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   if (nb < kXcds) return b;
   const int q = nb / kXcds, r = nb % kXcds;
   const int k = b % kXcds, i = b / kXcds;
@@ -806,9 +806,11 @@ struct BatchGroup {
 };
 
 constexpr int kBatchItemBits = 20;
+// wavefronts per workgroup of the batched line passes (lines per tile: 64 kBatchLineWaves / TPL)
+constexpr int kBatchLineWaves = 4;
 
 template <typename T, int N1, int N2>
-__global__ void __launch_bounds__(64) k_line_pass_batch(BatchGroup<T> g) {
+__global__ void __launch_bounds__(64 * kBatchLineWaves) k_line_pass_batch(BatchGroup<T> g) {
   const int code = g.blk[blockIdx.x];
   const int i = code >> kBatchItemBits;
   const int item = code & ((1 << kBatchItemBits) - 1);
