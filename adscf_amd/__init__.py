@@ -696,8 +696,8 @@ class Batch:
         return int(act[0])
 
     def set_schedule(self, n, n1, n2):
-        """Register shape n1 x n2 (one wavefront per workgroup) of every instance axis of
-        length n."""
+        """Register shape n1 x n2 of every instance axis of length n (the batched line
+        passes run four wavefronts of lines per workgroup)."""
         self._chk(self._lib.adscf_batch_set_schedule(self._h, int(n), int(n1), int(n2)))
 
     def iterate(self, niter, tol=0.0):

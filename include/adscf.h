@@ -340,8 +340,8 @@ int adscf_batch_set_persistent(adscf_batch* batch, int enable, int* active);
  * (ADSCF_ERR_ARG otherwise).  active (may be NULL) receives the number of instances whose
  * sweep uses them.  The persistent sweep carries line passes only. */
 int adscf_batch_set_plane_passes(adscf_batch* batch, int mode, int* active);
-/* register shape (as in adscf_set_schedule, one wavefront per workgroup) of every instance axis
- * of length n, in place of the default or adscf_batch_autotune's choice; ADSCF_ERR_ARG when
+/* register shape (as in adscf_set_schedule; the batched line passes run four wavefronts of
+ * lines per workgroup) of every instance axis of length n, in place of the default or adscf_batch_autotune's choice; ADSCF_ERR_ARG when
  * the batched kernels have no such shape or no instance has an axis of that length. */
 int adscf_batch_set_schedule(adscf_batch* batch, int n, int n1, int n2);
 /* per-instance outputs (arrays of ninst); stops when every instance has converged */

@@ -1,8 +1,9 @@
 """Batched instances (adscf_batch_*): every instance of a batch reproduces the
 single-instance device loop with the same line-FFT schedule bit for bit, with line passes
 only and with the default xy-plane passes (on the 32^3 and 48^3 instances),
-the batched sweep tolerates uneven grids and contour lengths, and the batch
-autotuner keeps results unchanged."""
+the batched sweep tolerates uneven grids and contour lengths, the batch
+autotuner keeps results unchanged, and every register shape of the batched line passes,
+pinned with adscf_batch_set_schedule, reproduces single-instance contexts bit for bit."""
 import numpy as np
 import pytest
 
@@ -104,3 +105,48 @@ def test_batch_continues_after_convergence(persistent):
     assert np.array_equal(it2, it1c) and np.array_equal(acc2, acc1c) and np.all(it2 > it1)
     for a, c in zip(g2, g1c):
         assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
+
+
+# every register shape the batched line passes carry, by axis length
+BATCH_SHAPES = {8: [(1, 8)], 16: [(1, 16)], 32: [(4, 8), (8, 4), (2, 16)], 48: [(8, 6), (6, 8), (16, 3)],
+                64: [(8, 8), (4, 16), (16, 4), (4, 0)], 128: [(16, 8), (8, 16)]}
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_pinned_shapes_match_single_instances(prec):
+    """Each batched line shape (four-wavefront tiles in XCD order), pinned with
+    adscf_batch_set_schedule, against single-instance contexts on the same schedules, bit
+    for bit after 3 iterations.  Non-square xy planes keep the sweeps on line passes."""
+    import adscf_amd
+    from adscf_amd import fields
+
+    P = adscf_amd.FP64 if prec == "fp64" else adscf_amd.FP32
+    grids = [(32, 48, 16), (64, 8, 128)]
+    inst, w = [], []
+    for k, g in enumerate(grids):
+        ns = 16 + 4 * k
+        segs = fields.block_segments([0.4, 0.6], ns)
+        inst.append(dict(grid=g, box=(3.0, 3.5, 4.0), ds=1.0 / ns, blocks=[1, 2], segments=segs, kappa=25.0,
+                         chi=[[0.0, 15.0], [15.0, 0.0]], vol=[1.0, 1.0]))
+        w.append(2.0 * np.random.default_rng(31 + k).standard_normal((2,) + g))
+    for r in range(4):
+        pick = {n: s[r % len(s)] for n, s in BATCH_SHAPES.items()}
+        print(f"{prec} round {r}: {pick}", flush=True)
+        with adscf_amd.Batch(inst, mixing_dim=4, lam_mix=0.2, precision=P) as b:
+            assert b.set_plane_passes(-1) == 0
+            for n in sorted({x for g in grids for x in g}):
+                b.set_schedule(n, *pick[n])
+            b.set_fields(w)
+            it, acc, conv = b.iterate(3, tol=0.0)
+            got = [b.get(i) for i in range(len(inst))]
+        for i, d in enumerate(inst):
+            with adscf_amd.Context(*d["grid"], *d["box"], d["ds"], precision=P) as ctx:
+                for a in range(3):
+                    ctx.set_schedule(a, *pick[d["grid"][a]], 1)
+                ctx.set_interactions(d["kappa"], d["chi"], d["vol"])
+                ctx.scf_setup(d["blocks"], d["segments"], mixing_dim=4, lam_mix=0.2)
+                ctx.scf_set_field(w[i])
+                st = ctx.scf_iterate(3, tol=0.0)
+                f1, d1 = ctx.scf_get()
+            assert st.iters == it[i] and st.accuracy == acc[i]
+            assert np.array_equal(f1, got[i][0]) and np.array_equal(d1, got[i][1])
