@@ -2400,7 +2400,8 @@ bool batch_shape_ok(int N1, int N2) {
   return false;
 }
 
-// every axis of a batch instance needs a one-wave register schedule the batched kernel knows
+// every axis of a batch instance needs a register shape the batched kernel knows (the batched
+// line passes size their tiles by kBatchLineWaves, not by the axis's waves)
 void batch_fix_schedule(adscf_ctx* c) {
   for (int a = 0; a < 3; ++a) {
     AxisTables& ax = c->ax[a];
@@ -4348,8 +4349,8 @@ int adscf_batch_timing(adscf_batch* b, double* ms_per_iter, double* ms_sweep) {
   });
 }
 
-// Autotuner for the batch: for every distinct axis length, try each one-wave
-// register shape on all instances together and keep the fastest batched sweep.
+// Autotuner for the batch: for every distinct axis length, try each batched register
+// shape on all instances together and keep the fastest batched sweep.
 int adscf_batch_autotune(adscf_batch* b, int reps, char* report, int report_cap) {
   return bguard(b, [&]() -> int {
     std::string rep;

@@ -796,7 +796,8 @@ __global__ void __launch_bounds__(kThreads) k_line_pass_tiles(PassArgs<T> a, int
 // One pass of many independent instances that share a register shape, in one
 // launch: blk[b] packs the instance slot of block b (bits 20 and up) and the
 // block's item within that instance (low 20 bits); every instance carries its own
-// arguments (grid, buffers, step types).  Blocks are one wavefront.  A pass of a
+// arguments (grid, buffers, step types).  Line-pass blocks are kBatchLineWaves
+// wavefronts of lines, taken in xcd_tile order by the host's table.  A pass of a
 // batch is one such launch per shape present (the shape is a template argument,
 // so each launch has the register budget of its own engine).
 template <typename T>
