@@ -771,12 +771,13 @@ void default_schedule(AxisTables& ax, int axis, int prec) {
     if (prec == ADSCF_FP32) {
       ax.waves = axis == 0 ? 1 : 2;
       ax.tiles = axis == 2 ? 2 : 1;
-    } else if (axis == 0) {
-      ax.N1 = 4;  // the 8 x 4 x 4 engine
-      ax.N2 = -8;
     } else if (axis == 1) {
-      ax.N1 = 8;
-      ax.N2 = 16;
+      ax.waves = 2;  // 16 x 8, two tiles per workgroup
+      ax.tiles = 2;
+    } else {
+      ax.N1 = 4;  // the 8 x 4 x 4 engine; four waves on z
+      ax.N2 = -8;
+      if (axis == 2) ax.waves = 4;
     }
   }
 }
