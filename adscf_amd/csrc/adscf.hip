@@ -1942,8 +1942,11 @@ std::string autotune_impl(adscf_ctx* c, int reps) {
     c->plane_N2 = bN2;
     c->plane_shape_set = had_shape || best_plane < 1e300;
     c->plane = best_plane < best_lines ? 1 : (plane_req == 1 ? 1 : 0);
-    snprintf(line, sizeof line, "plane passes chosen: %s (%s, %.4f ms; line passes %.4f ms)\n",
-             c->plane == 1 ? "yes" : "no", shape_name(bN1, bN2).c_str(), best_plane, best_lines);
+    if (best_plane < 1e300)
+      snprintf(line, sizeof line, "plane passes chosen: %s (%s, %.4f ms; line passes %.4f ms)\n",
+               c->plane == 1 ? "yes" : "no", shape_name(bN1, bN2).c_str(), best_plane, best_lines);
+    else
+      snprintf(line, sizeof line, "plane passes chosen: no (no plane kernel fits this grid and precision)\n");
     rep += line;
   }
   c->keep.retire(c->graph_src, c->graph);
