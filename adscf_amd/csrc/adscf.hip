@@ -2635,39 +2635,71 @@ void batch_build_tables(adscf_batch* b) {
     npass = std::max(npass, es.size());
     per.push_back(std::move(es));
   }
-  // Lanes: instances whose passes have the same register shapes form a group (on cubic
-  // grids, one per axis length); groups go to lanes, largest work first onto the least loaded
-  // lane.  A lane's sweep and post-sweep kernels depend on nothing in another lane, so the
-  // captured iteration runs the lanes as parallel graph branches.
-  std::vector<std::vector<std::array<int, 4>>> sigs;
+  // Lanes: a lane's sweep and post-sweep kernels depend on nothing in another lane, so the
+  // captured iteration runs the lanes as parallel graph branches.  Within a lane, pass p of
+  // all its instances with one kernel shape is one launch, and launches of different shapes
+  // queue behind each other.  Instances form groups, largest work first, and an instance
+  // goes to the least loaded lane that already holds its group and stays within 5% over the
+  // even share of the work, else to the least loaded lane; work is points x passes.  With
+  // plane passes a group is the set of kernel shapes of the sweep (on cubic grids, one per
+  // axis length: 69.0 -> 71.7 batch iterations/s on the 64-instance sweep against the
+  // grouping below); with line passes only it is the sequence of shapes, pass by pass (so
+  // also the contour length), which measured 10% faster there than grouping by shape.
+  bool planes = false;
+  for (const auto& es : per)
+    for (const Entry& e : es) planes = planes || e.plane;
+  std::vector<std::vector<std::array<int, 3>>> sigs;
   std::vector<int> group_of(per.size());
-  std::vector<double> gcost;
+  std::vector<double> gcost, icost(per.size());
+  double total = 0.0;
   for (size_t i = 0; i < per.size(); ++i) {
-    std::vector<std::array<int, 4>> sg;
-    for (const Entry& e : per[i]) sg.push_back({e.plane, e.N1, e.N2, 0});
+    std::vector<std::array<int, 3>> sg;
+    for (const Entry& e : per[i]) {
+      const std::array<int, 3> k = {e.plane, e.N1, e.N2};
+      if (!planes || std::find(sg.begin(), sg.end(), k) == sg.end()) sg.push_back(k);
+    }
+    if (planes) std::sort(sg.begin(), sg.end());
     auto it = std::find(sigs.begin(), sigs.end(), sg);
-    int g = (int)(it - sigs.begin());
+    const int g = (int)(it - sigs.begin());
     if (it == sigs.end()) {
       sigs.push_back(sg);
       gcost.push_back(0.0);
     }
     group_of[i] = g;
-    gcost[g] += (double)b->inst[i]->ng * (double)per[i].size();
+    icost[i] = (double)b->inst[i]->ng * (double)per[i].size();
+    gcost[g] += icost[i];
+    total += icost[i];
   }
-  std::vector<int> order(gcost.size());
-  for (size_t g = 0; g < order.size(); ++g) order[g] = (int)g;
-  std::sort(order.begin(), order.end(), [&](int x, int y) { return gcost[x] > gcost[y]; });
-  std::vector<int> lane_of_group(gcost.size());
+  std::vector<int> order(per.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::sort(order.begin(), order.end(), [&](int x, int y) {
+    if (group_of[x] != group_of[y]) {
+      if (gcost[group_of[x]] != gcost[group_of[y]]) return gcost[group_of[x]] > gcost[group_of[y]];
+      return group_of[x] < group_of[y];
+    }
+    if (icost[x] != icost[y]) return icost[x] > icost[y];
+    return x < y;
+  });
+  const double share = total / kBatchLanes;
   double load[kBatchLanes] = {};
-  for (int g : order) {
-    int best = 0;
-    for (int k = 1; k < kBatchLanes; ++k)
-      if (load[k] < load[best]) best = k;
-    lane_of_group[g] = best;
-    load[best] += gcost[g];
+  std::vector<std::vector<int>> holds(kBatchLanes);  // shape groups in each lane
+  b->lane_of.assign(per.size(), 0);
+  for (int i : order) {
+    const int g = group_of[i];
+    int best = -1;
+    for (int k = 0; k < kBatchLanes; ++k)
+      if (std::find(holds[k].begin(), holds[k].end(), g) != holds[k].end() && load[k] + icost[i] <= 1.05 * share &&
+          (best < 0 || load[k] < load[best]))
+        best = k;
+    if (best < 0) {
+      best = 0;
+      for (int k = 1; k < kBatchLanes; ++k)
+        if (load[k] < load[best]) best = k;
+    }
+    b->lane_of[i] = best;
+    load[best] += icost[i];
+    if (std::find(holds[best].begin(), holds[best].end(), g) == holds[best].end()) holds[best].push_back(g);
   }
-  b->lane_of.resize(per.size());
-  for (size_t i = 0; i < per.size(); ++i) b->lane_of[i] = lane_of_group[group_of[i]];
 
   std::vector<PassArgs<T>> args;
   std::vector<int> blk;
@@ -4354,7 +4386,8 @@ int adscf_batch_timing(adscf_batch* b, double* ms_per_iter, double* ms_sweep) {
 }
 
 // Autotuner for the batch: for every distinct axis length, try each batched register
-// shape on all instances together and keep the fastest batched sweep.
+// shape on all instances together and keep the fastest batched sweep (the shape in place
+// unless another is at least 2% faster).
 int adscf_batch_autotune(adscf_batch* b, int reps, char* report, int report_cap) {
   return bguard(b, [&]() -> int {
     std::string rep;
@@ -4365,9 +4398,18 @@ int adscf_batch_autotune(adscf_batch* b, int reps, char* report, int report_cap)
         if (c->ax[a].n > 1 && std::find(lengths.begin(), lengths.end(), c->ax[a].n) == lengths.end())
           lengths.push_back(c->ax[a].n);
     for (int n : lengths) {
+      // the shape in place (the default, or an earlier choice) is timed first and a
+      // candidate replaces it only when its sweep is at least 2% faster: the shapes of one
+      // length change only part of a batched sweep, and timings of equal schedules differ
+      // by about 1% from run to run
+      std::pair<int, int> cur = {0, 0};
+      for (adscf_ctx* c : b->inst)
+        for (int a = 0; a < 3; ++a)
+          if (c->ax[a].n == n && cur.first == 0) cur = {c->ax[a].N1, c->ax[a].N2};
       std::vector<std::pair<int, int>> cands;
+      if (batch_shape_ok(cur.first, cur.second)) cands.push_back(cur);
       for (const RegShape& s : kRegShapes)
-        if (s.n == n && batch_shape_ok(s.N1, s.N2)) cands.push_back({s.N1, s.N2});
+        if (s.n == n && batch_shape_ok(s.N1, s.N2) && std::make_pair(s.N1, s.N2) != cur) cands.push_back({s.N1, s.N2});
       double best = 1e300;
       std::pair<int, int> bestsh = cands.empty() ? std::make_pair(0, 0) : cands[0];
       for (auto sh : cands) {
@@ -4383,7 +4425,7 @@ int adscf_batch_autotune(adscf_batch* b, int reps, char* report, int report_cap)
         snprintf(line, sizeof line, "n=%d shape=%s batched sweep=%.4f ms\n", n, shape_name(sh.first, sh.second).c_str(),
                  ms);
         rep += line;
-        if (ms < best) {
+        if (ms < (best < 1e300 ? 0.98 * best : best)) {
           best = ms;
           bestsh = sh;
         }
