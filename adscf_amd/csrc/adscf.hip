@@ -240,12 +240,13 @@ int grid_for(long n) {
   return (int)std::max<long>(1, std::min<long>(b, kRedBlocks));
 }
 
-// contour kernel: 16-byte point packs when ng allows; returns the blocks launched
-// (one rpart/spart slot each)
+// contour kernel: 16-byte point packs when ng allows and gives at least 128 blocks of them,
+// else one point per thread (more threads in flight on small grids); returns the blocks
+// launched (one rpart/spart slot each)
 template <typename T>
 int launch_contour(hipStream_t stream, const ContourArgs<T>& a) {
   constexpr int VP = 16 / sizeof(T);
-  if (a.ng % VP == 0) {
+  if (a.ng % VP == 0 && a.ng / VP >= (long)kThreads * 128) {
     const int nb = grid_for(a.ng / VP);
     hipLaunchKernelGGL((k_contour<T, VP>), dim3(nb), dim3(kThreads), 0, stream, a);
     return nb;
